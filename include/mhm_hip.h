@@ -1,0 +1,211 @@
+/*
+ * mhm_hip.h - C ABI of the MI355X (gfx950) MHM local-problem stage.
+ *
+ * One library, libmhm_hip.so, built from pymhm_amd/csrc/mhm_hip.hip.  Every
+ * entry point returns an int status (MHM_OK on success); mhm_last_error()
+ * returns the message of the last failure on the calling thread.  Array
+ * arguments named d_* are device pointers on the handle's device; the caller
+ * owns them.  Opaque handles own only their internal device workspaces.
+ * Kernels are enqueued on the given HIP stream (NULL = default stream) and
+ * are asynchronous unless the comment says the call synchronises.
+ *
+ * The reference (PyMHM) is pure Python and has no FFI; each entry point is
+ * the native replacement of the Python function cited beside it (paths
+ * relative to the reference repository root).  The Python binding that a
+ * maintainer adds on the reference side is shown in INTEGRATION.md.
+ *
+ * Layout conventions
+ *   n    local DOFs of one macroelement, k retained modes (0 or 1), N = n + k
+ *   m    multiplier (skeleton) DOFs of the macroelement
+ *   R    standard response columns: 1 (source) + m (lifts) + k (retained action)
+ *   Rows of per-macro vectors use the caller's ORIGINAL local numbering; the
+ *   layout's band permutation is internal to the library.
+ *   Column-major per macro: X[macro][column][row], row count N.
+ */
+#ifndef MHM_HIP_H
+#define MHM_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  MHM_OK = 0,
+  MHM_EINVAL = 1,    /* invalid argument -> ValueError                      */
+  MHM_ESINGULAR = 2, /* non-positive / rank-deficient pivot -> LinearSolveError */
+  MHM_ERESIDUAL = 3, /* residual criterion not met -> LinearSolveError      */
+  MHM_ENOMEM = 4,    /* device allocation failed                            */
+  MHM_EDEVICE = 5    /* HIP runtime error                                   */
+};
+
+/* Precision policy of factorisation and triangular solves.  Residuals,
+ * refinement, the 2x2 bordered correction, condensation and reconstruction
+ * are always binary64. */
+enum { MHM_FP64 = 0, MHM_MIXED = 1 };
+
+/* Per-macro status bits (mhm_batch_status). */
+enum {
+  MHM_STATUS_PIVOT = 1,    /* non-positive or tiny pivot in the band factor      */
+  MHM_STATUS_RESIDUAL = 2, /* some column missed the residual criterion          */
+  MHM_STATUS_NONFINITE = 4 /* non-finite operator, pivot or solution             */
+};
+
+typedef struct mhm_layout mhm_layout;
+typedef struct mhm_batch mhm_batch;
+
+/* Mesh-only description of one local problem shape (a "layout").  All arrays
+ * are HOST arrays, copied by mhm_layout_create.  Index arrays use the band
+ * (factorisation) numbering; perm maps band row -> original local row. */
+typedef struct mhm_layout_desc {
+  int32_t n;              /* local DOFs                                         */
+  int32_t k;              /* retained modes: 0 (SPD operator) or 1              */
+  int32_t m;              /* multiplier columns                                 */
+  int32_t nnz;            /* stored entries of the full symmetric pattern        */
+  int32_t bandwidth;      /* max |i - j| over the pattern in band numbering      */
+  int32_t pin;            /* band row receiving sigma = A_pp (k == 1), else -1   */
+  const int32_t *perm;    /* [n] band row -> original row                        */
+  const int32_t *rowptr;  /* [n+1] CSR of the pattern, band numbering            */
+  const int32_t *cols;    /* [nnz] sorted column indices per row                 */
+  /* template coupling B (may be NULL when every batch supplies its own) */
+  const double *coupling; /* [m][n] column-major, band rows                      */
+  const int32_t *coupling_rows; /* [ncoupling_rows] band rows where B may be nonzero */
+  int32_t ncoupling_rows;
+  /* template retained mode Z and constraint C (NULL: Z = 1; C required if k) */
+  const double *kernel;     /* [n] band rows                                     */
+  const double *constraint; /* [n] band rows                                     */
+  /* optional assembly description (ne == 0: operators are supplied assembled) */
+  int32_t ne;             /* local elements                                     */
+  int32_t nb;             /* basis functions per element (3, 6 or 10)            */
+  int32_t dim;            /* 2 or 3                                             */
+  int32_t nq;             /* quadrature points per element                      */
+  const int32_t *slot_ptr;   /* [nnz+1] contributions of each CSR slot           */
+  const int32_t *slot_code;  /* [ne*nb*nb] (i*nb + j)*ne + e, fixed order         */
+  const int32_t *load_ptr;   /* [n+1] contributions of each load row             */
+  const int32_t *load_code;  /* [ne*nb] b*ne + e, fixed order                      */
+  const double *grad_lambda; /* [ne][dim+1][dim] physical barycentric gradients  */
+  const double *measure;     /* [ne] element areas / volumes                     */
+  const double *qweights;    /* [nq] reference weights, unit sum                 */
+  const double *dtable;      /* [nq][nb][dim+1] d phi_b / d lambda_i              */
+  const double *vtable;      /* [nq][nb] phi_b                                    */
+} mhm_layout_desc;
+
+/* Diagnostics. */
+int mhm_last_error(char *buf, size_t len);
+int mhm_device_count(int32_t *count);
+int mhm_version(int32_t *major, int32_t *minor);
+
+/* Layout (mesh-only data, shared by every macro of one shape).
+ * Replaces the per-cell mesh/DOF/trace preparation of the providers:
+ * src/pymhm/_legacy/models/darcy/primal.py:44-78 (trace_coupling, nodal
+ * space, constraints = M Z / Z^T M Z), fem/scalar/triangle.py:65-92,160-205. */
+int mhm_layout_create(int32_t device, const mhm_layout_desc *desc, mhm_layout **out);
+int mhm_layout_destroy(mhm_layout *layout);
+int mhm_layout_info(const mhm_layout *layout, int32_t *n, int32_t *bandwidth, int32_t *nnz);
+
+/* Batch of `macros` local problems of one layout with resident factors.
+ * Replaces BatchedFactorization.__init__ (src/pymhm/execution/cuda.py:178-215)
+ * and the grouping of condense_batched (cuda.py:291-312).  Allocates its
+ * workspaces on the layout's device. */
+int mhm_batch_create(const mhm_layout *layout, int32_t macros, int32_t precision,
+                     int32_t max_columns, mhm_batch **out);
+int mhm_batch_destroy(mhm_batch *batch);
+
+/* Local assembly from coefficients tabulated at quadrature points:
+ * d_kq[macro][element][q] (scalar diffusion), d_fq[macro][element][q] source
+ * (NULL: constant f_const).  Produces the stiffness in the layout pattern, the
+ * load vector and A Z (accumulated in double-double).
+ * Replaces scalar_operators/p1_operators + the Numba Gram kernel:
+ * src/pymhm/fem/scalar/triangle.py:208-257, fem/scalar/operators.py:154-197,
+ * fem/scalar/_integration.py:43-88, and LocalProblem._retained_action
+ * (core/contracts.py:201-211). */
+int mhm_batch_assemble(mhm_batch *batch, const double *d_kq, const double *d_fq, double f_const,
+                       void *stream);
+
+/* Assembled operators instead of assembly: d_values[macro][nnz] in the
+ * layout's band-numbered CSR slot order, d_loads[macro][n] ORIGINAL rows.
+ * Replaces the dense upload of condense_batched (cuda.py:304-309). */
+int mhm_batch_set_operator(mhm_batch *batch, const double *d_values, const double *d_loads,
+                           void *stream);
+
+/* Per-macro coupling data (all optional, NULL keeps the layout default):
+ * d_colmap[macro][m] (int32) and d_colscale[macro][m] give column a of the
+ * macro's coupling as colscale[a] * (layout coupling column colmap[a]) - a
+ * signed column permutation that carries macroface orientation (segment
+ * order reversal and odd Legendre modes, fem/traces/interval.py:54-89, with
+ * the orientation signs of meshes/triangle.py:53-74); d_couplings[macro][m][n]
+ * full per-macro couplings (ORIGINAL rows); d_constraints[macro][n] and
+ * d_kernels[macro][n] (ORIGINAL rows). */
+int mhm_batch_set_coupling(mhm_batch *batch, const int32_t *d_colmap, const double *d_colscale,
+                           const double *d_couplings, const double *d_constraints,
+                           const double *d_kernels, void *stream);
+
+/* Scale the constraint, equilibrate, pin the retained mode and factor every
+ * macro's operator (band Cholesky of D (A + sigma e_p e_p^T) D).
+ * Replaces local_condensation_matrix (core/condensation.py:150-174),
+ * _equilibrated_lu and the pivot check (linalg/linear.py:400-422),
+ * _factor_batch (execution/cuda.py:114-136). */
+int mhm_batch_factor(mhm_batch *batch, void *stream);
+
+/* Solve the standard columns [f, B, A Z] of the augmented system with up to
+ * `max_refine` binary64 refinement steps (residual criterion rtol, per column,
+ * as linalg/linear.py:132-152), then condense into
+ * d_blocks[macro][(m+k)*(m+k)] (row-major) and d_loads[macro][m+k].
+ * Replaces condense_local (core/condensation.py:118-122,125-147),
+ * LinearFactorization.solve (linalg/linear.py:262-310),
+ * local_response_from_solution (:177-191), local_global_contribution
+ * (core/contributions.py:14-74) and local_condensed_load
+ * (core/reconstruction.py:91-127). */
+int mhm_batch_condense(mhm_batch *batch, int32_t max_refine, double rtol, double *d_blocks,
+                       double *d_loads, void *stream);
+
+/* Copy the responses of the last mhm_batch_condense in ORIGINAL row order:
+ * d_source[macro][n], d_lifts[macro][n][m] (row-major like numpy (n, m)),
+ * d_retained[macro][n] (E = Z - R A Z, or Z when A Z == 0).  Any may be NULL.
+ * Replaces LocalResponse (core/contracts.py:321-333). */
+int mhm_batch_responses(mhm_batch *batch, double *d_source, double *d_lifts, double *d_retained,
+                        void *stream);
+
+/* Solve `ncols` new right-hand sides of the augmented system against the
+ * resident factors: d_rhs[macro][col][N] -> d_x[macro][col][N], ORIGINAL rows,
+ * row n (k == 1) is the constraint row.  ncols <= max_columns.
+ * Replaces BatchedFactorization.solve (execution/cuda.py:223-255) and
+ * OfflineLocalProblem.response (core/offline.py:121-127). */
+int mhm_batch_solve(mhm_batch *batch, const double *d_rhs, double *d_x, int32_t ncols,
+                    int32_t max_refine, double rtol, void *stream);
+
+/* Fields u = s - L lambda + E c for `ncases` trace/coarse vectors:
+ * d_trace[macro][case][m] (local trace order), d_coarse[macro][case][k],
+ * d_fields[macro][case][n] ORIGINAL rows.
+ * Replaces reconstruct_response (core/reconstruction.py:130-141). */
+int mhm_batch_reconstruct(mhm_batch *batch, const double *d_trace, const double *d_coarse,
+                          double *d_fields, int32_t ncases, void *stream);
+
+/* Synchronise the stream and copy the per-macro status bits to host memory
+ * (host_status[macros]); also returns the bitwise OR in *any. */
+int mhm_batch_status(mhm_batch *batch, int32_t *host_status, int32_t *any, void *stream);
+
+/* Per-macro refinement counts of the last solve (host_counts[macros]); synchronises. */
+int mhm_batch_iterations(mhm_batch *batch, int32_t *host_counts, void *stream);
+
+/* Row-major COO coordinates of `count` blocks of width w:
+ * d_rows[c][i*w+j] = d_indices[c][i], d_cols[c][i*w+j] = d_indices[c][j].
+ * Replaces contribution_coordinates (core/_assembly.py:9-25). */
+int mhm_scatter_coordinates(const int64_t *d_indices, int32_t width, int32_t count,
+                            int64_t *d_rows, int64_t *d_cols, void *stream);
+
+/* Ordered gather of contributions into global vectors/values: for every
+ * output g, out[g] = sum over t in [ptr[g], ptr[g+1]) of src[code[t]] in the
+ * stored (cell) order; when d_abs_out != NULL it also receives sum |src|.
+ * Replaces accumulate_load (core/_assembly.py:28-42) and the duplicate
+ * summation of the COO -> CSC conversion (core/contributions.py:175-185). */
+int mhm_ordered_gather(const int64_t *d_ptr, const int64_t *d_code, const double *d_src,
+                       int64_t count, double *d_out, double *d_abs_out, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MHM_HIP_H */

@@ -1,0 +1,1471 @@
+// MI355X (gfx950) kernels and C ABI of the MHM local-problem stage.
+//
+// Per macroelement the stage assembles the local stiffness A (scalar
+// diffusion, P1/P2 simplices) and load f from coefficients tabulated at
+// quadrature points, factors the augmented operator
+//     S = [[A, c], [c^T, 0]],  c = C min(1, max|A| / max|C|),
+// solves S X = [f, B, A Z] (source, one lift per multiplier, retained action),
+// refines in binary64, condenses into the (m+k)^2 block of the global hybrid
+// matrix and reconstructs fields u = s - L lambda + E c.
+//
+// Factorisation.  A is symmetric positive semidefinite with kernel Z (the
+// constants), so the library factors the SPD band matrix
+//     Ahat = A + sigma e_p e_p^T,  sigma = A_pp,
+// after symmetric Jacobi scaling D Ahat D (unit diagonal), with a band
+// Cholesky whose (b+1)x(b+1) active window stays resident in LDS while the
+// band streams through it.  The saddle solution follows from three Ahat
+// solves (rhs, e_p, c) and a 2x2 bordered system for (x_p, mu):
+//     x = y + sigma x_p u - mu w,   y = Ahat^-1 g, u = Ahat^-1 e_p, w = Ahat^-1 c
+//     [1 - sigma u_p,  w_p ] [x_p]   [ y_p         ]
+//     [-sigma c^T u,  c^T w] [mu ] = [ c^T y - beta]
+// which is exact for any nonsingular S (no assumption A Z == 0).  Binary64
+// residuals of the true S, accumulated in double-double, drive iterative
+// refinement; the factor and triangular solves run in binary64 (MHM_FP64) or
+// binary32 (MHM_MIXED).
+//
+// Every reduction has a fixed order, so a macro's results do not depend on
+// which other macros share the launch.
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/mhm_hip.h"
+
+// ----------------------------------------------------------------- errors
+static thread_local std::string g_error;
+
+static int fail(int code, const std::string &msg) {
+  g_error = msg;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                 \
+  do {                                                                                \
+    hipError_t _e = (expr);                                                           \
+    if (_e != hipSuccess)                                                             \
+      return fail(_e == hipErrorOutOfMemory ? MHM_ENOMEM : MHM_EDEVICE,               \
+                  std::string(#expr) + ": " + hipGetErrorString(_e));                 \
+  } while (0)
+
+#define LAUNCH_CHECK(name)                                                            \
+  do {                                                                                \
+    hipError_t _e = hipGetLastError();                                                \
+    if (_e != hipSuccess)                                                             \
+      return fail(MHM_EDEVICE, std::string("launch ") + name + ": " + hipGetErrorString(_e)); \
+  } while (0)
+
+// ----------------------------------------------------------------- device helpers
+__device__ __forceinline__ void two_sum(double a, double b, double &s, double &e) {
+  s = a + b;
+  double bb = s - a;
+  e = (a - (s - bb)) + (b - bb);
+}
+
+// Accumulate the exact product a*b into the double-double (hi, lo).
+__device__ __forceinline__ void dd_fma(double &hi, double &lo, double a, double b) {
+  double p = a * b;
+  double pe = fma(a, b, -p);
+  double s, e;
+  two_sum(hi, p, s, e);
+  hi = s;
+  lo += e + pe;
+}
+
+__device__ __forceinline__ void dd_add(double &hi, double &lo, double v) {
+  double s, e;
+  two_sum(hi, v, s, e);
+  hi = s;
+  lo += e;
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+__device__ __forceinline__ double wave_max(double v) {
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+// Fixed-order block reductions (result broadcast to every thread).
+template <int NT>
+__device__ double block_sum(double v, double *sh) {
+  v = wave_sum(v);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double t = 0.0;
+#pragma unroll
+  for (int i = 0; i < NT / 64; ++i) t += sh[i];
+  return t;
+}
+
+template <int NT>
+__device__ double block_max(double v, double *sh) {
+  v = wave_max(v);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double t = sh[0];
+#pragma unroll
+  for (int i = 1; i < NT / 64; ++i) t = fmax(t, sh[i]);
+  return t;
+}
+
+// ----------------------------------------------------------------- kernel arguments
+struct LayoutK {
+  int n, k, m, nnz, b, W, pin, ne, nb, dim, nq, ncr;
+  const int *perm, *rowptr, *cols, *diagslot, *crows;
+  const double *coupling, *kernel, *constraint;
+  const int *slot_ptr, *slot_code, *load_ptr, *load_code;
+  const double *glam, *measure, *qw, *dtab, *vtab;
+};
+
+// Per-macro scalars kept in scal[mac * NSCAL + ...].
+enum { S_AMAX = 0, S_SIGMA, S_CS, S_A11, S_A12, S_A21, S_A22, S_DET, NSCAL };
+
+struct BatchK {
+  int M, R, N, ldy;
+  double *aval, *load, *az, *dscale, *scal, *uw;
+  int *status, *iters;
+  const int *colmap;                          // per-macro signed column map or null
+  const double *colscale, *bcoup, *cons, *kern;  // per-macro overrides (band rows) or null
+};
+
+__device__ __forceinline__ double coupling_at(const LayoutK &L, const BatchK &B, int mac, int a,
+                                              int i) {
+  if (B.bcoup) return B.bcoup[((size_t)mac * L.m + a) * L.n + i];
+  const int src = B.colmap ? B.colmap[(size_t)mac * L.m + a] : a;
+  const double v = L.coupling[(size_t)src * L.n + i];
+  return B.colscale ? v * B.colscale[(size_t)mac * L.m + a] : v;
+}
+
+__device__ __forceinline__ double constraint_at(const LayoutK &L, const BatchK &B, int mac,
+                                                int i) {
+  return B.cons ? B.cons[(size_t)mac * L.n + i] : L.constraint[i];
+}
+
+__device__ __forceinline__ double kernel_at(const LayoutK &L, const BatchK &B, int mac, int i) {
+  if (B.kern) return B.kern[(size_t)mac * L.n + i];
+  return L.kernel ? L.kernel[i] : 1.0;
+}
+
+// ----------------------------------------------------------------- assembly
+// One thread per element: element stiffness (upper triangle in registers) and
+// load from K and f at the element's quadrature points.  Element matrices are
+// stored code-major, ework[mac][i*NB+j][e], so the stores coalesce.
+template <int NB, int D>
+__global__ void __launch_bounds__(128)
+    k_assemble(LayoutK L, const double *__restrict__ kq, const double *__restrict__ fq,
+               double fconst, double *__restrict__ ework, double *__restrict__ fwork) {
+  extern __shared__ double sh[];
+  const int nq = L.nq;
+  double *sd = sh;
+  double *sv = sd + nq * NB * (D + 1);
+  double *sw = sv + nq * NB;
+  for (int t = threadIdx.x; t < nq * NB * (D + 1); t += blockDim.x) sd[t] = L.dtab[t];
+  for (int t = threadIdx.x; t < nq * NB; t += blockDim.x) sv[t] = L.vtab[t];
+  for (int t = threadIdx.x; t < nq; t += blockDim.x) sw[t] = L.qw[t];
+  __syncthreads();
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  const int mac = blockIdx.y;
+  if (e >= L.ne) return;
+  double gl[D + 1][D];
+#pragma unroll
+  for (int i = 0; i <= D; ++i)
+#pragma unroll
+    for (int a = 0; a < D; ++a) gl[i][a] = L.glam[((size_t)e * (D + 1) + i) * D + a];
+  const double meas = L.measure[e];
+  double K[NB * (NB + 1) / 2];
+  double F[NB];
+#pragma unroll
+  for (int t = 0; t < NB * (NB + 1) / 2; ++t) K[t] = 0.0;
+#pragma unroll
+  for (int t = 0; t < NB; ++t) F[t] = 0.0;
+  const double *kp = kq + ((size_t)mac * L.ne + e) * nq;
+  const double *fp = fq ? fq + ((size_t)mac * L.ne + e) * nq : nullptr;
+  for (int q = 0; q < nq; ++q) {
+    double g[NB][D];
+#pragma unroll
+    for (int b = 0; b < NB; ++b)
+#pragma unroll
+      for (int a = 0; a < D; ++a) {
+        double s = 0.0;
+#pragma unroll
+        for (int i = 0; i <= D; ++i) s += sd[(q * NB + b) * (D + 1) + i] * gl[i][a];
+        g[b][a] = s;
+      }
+    const double kw = sw[q] * kp[q] * meas;
+    int t = 0;
+#pragma unroll
+    for (int i = 0; i < NB; ++i)
+#pragma unroll
+      for (int j = i; j < NB; ++j) {
+        double c = 0.0;
+#pragma unroll
+        for (int a = 0; a < D; ++a) c += g[i][a] * g[j][a];
+        K[t++] += kw * c;
+      }
+    const double fw = sw[q] * (fp ? fp[q] : fconst) * meas;
+#pragma unroll
+    for (int b = 0; b < NB; ++b) F[b] += fw * sv[q * NB + b];
+  }
+  const size_t base = (size_t)mac * NB * NB * L.ne;
+  int t = 0;
+#pragma unroll
+  for (int i = 0; i < NB; ++i)
+#pragma unroll
+    for (int j = i; j < NB; ++j) {
+      const double v = K[t++];
+      ework[base + (size_t)(i * NB + j) * L.ne + e] = v;
+      if (j != i) ework[base + (size_t)(j * NB + i) * L.ne + e] = v;
+    }
+  const size_t fb = (size_t)mac * NB * L.ne;
+#pragma unroll
+  for (int b = 0; b < NB; ++b) fwork[fb + (size_t)b * L.ne + e] = F[b];
+}
+
+// One thread per band row: ordered gather of the element contributions of each
+// CSR slot (fixed element order -> deterministic, bitwise symmetric A), the
+// load row, and A Z accumulated in double-double.
+__global__ void __launch_bounds__(128)
+    k_gather(LayoutK L, BatchK B, const double *__restrict__ ework,
+             const double *__restrict__ fwork) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int mac = blockIdx.y;
+  if (i >= L.n) return;
+  const double *ew = ework + (size_t)mac * L.nb * L.nb * L.ne;
+  double hi = 0.0, lo = 0.0;
+  for (int s = L.rowptr[i]; s < L.rowptr[i + 1]; ++s) {
+    double v = 0.0;
+    for (int t = L.slot_ptr[s]; t < L.slot_ptr[s + 1]; ++t) v += ew[L.slot_code[t]];
+    B.aval[(size_t)mac * L.nnz + s] = v;
+    dd_fma(hi, lo, v, kernel_at(L, B, mac, L.cols[s]));
+  }
+  B.az[(size_t)mac * L.n + i] = hi + lo;
+  const double *fw = fwork + (size_t)mac * L.nb * L.ne;
+  double f = 0.0;
+  for (int t = L.load_ptr[i]; t < L.load_ptr[i + 1]; ++t) f += fw[L.load_code[t]];
+  B.load[(size_t)mac * L.n + i] = f;
+}
+
+// Assembled-operator path: A Z from supplied CSR values; loads permuted to band rows.
+__global__ void __launch_bounds__(128)
+    k_rowsum(LayoutK L, BatchK B, const double *__restrict__ loads_orig) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int mac = blockIdx.y;
+  if (i >= L.n) return;
+  double hi = 0.0, lo = 0.0;
+  for (int s = L.rowptr[i]; s < L.rowptr[i + 1]; ++s)
+    dd_fma(hi, lo, B.aval[(size_t)mac * L.nnz + s], kernel_at(L, B, mac, L.cols[s]));
+  B.az[(size_t)mac * L.n + i] = hi + lo;
+  B.load[(size_t)mac * L.n + i] = loads_orig[(size_t)mac * L.n + L.perm[i]];
+}
+
+// ----------------------------------------------------------------- factor preparation
+// Block per macro: max|A| and max|C| (constraint scaling of
+// core/condensation.py:160-166), Jacobi scaling of Ahat, band fill.
+template <typename T>
+__global__ void __launch_bounds__(256) k_prepare(LayoutK L, BatchK B, T *__restrict__ band) {
+  __shared__ double red[4];
+  const int mac = blockIdx.x;
+  const double *a = B.aval + (size_t)mac * L.nnz;
+  double amax = 0.0;
+  bool finite = true;
+  for (int s = threadIdx.x; s < L.nnz; s += 256) {
+    double v = a[s];
+    finite &= isfinite(v);
+    amax = fmax(amax, fabs(v));
+  }
+  amax = block_max<256>(amax, red);
+  double cmax = 0.0;
+  if (L.k)
+    for (int i = threadIdx.x; i < L.n; i += 256) cmax = fmax(cmax, fabs(constraint_at(L, B, mac, i)));
+  cmax = block_max<256>(cmax, red);
+  const double sigma = (L.k && L.pin >= 0 && L.diagslot[L.pin] >= 0) ? a[L.diagslot[L.pin]] : 0.0;
+  int bad = finite ? 0 : MHM_STATUS_NONFINITE;
+  for (int i = threadIdx.x; i < L.n; i += 256) {
+    double d = L.diagslot[i] >= 0 ? a[L.diagslot[i]] : 0.0;
+    if (i == L.pin) d += sigma;
+    if (!(d > 0.0) || !isfinite(d)) {
+      bad |= MHM_STATUS_PIVOT;
+      d = 1.0;
+    }
+    B.dscale[(size_t)mac * L.n + i] = 1.0 / sqrt(d);
+  }
+  T *bd = band + (size_t)mac * L.n * L.W;
+  for (size_t t = threadIdx.x; t < (size_t)L.n * L.W; t += 256) bd[t] = T(0);
+  __syncthreads();
+  const double *ds = B.dscale + (size_t)mac * L.n;
+  for (int i = threadIdx.x; i < L.n; i += 256) {
+    const double di = ds[i];
+    for (int s = L.rowptr[i]; s < L.rowptr[i + 1]; ++s) {
+      const int j = L.cols[s];
+      if (j > i) break;
+      double v = a[s];
+      if (i == j && i == L.pin) v += sigma;
+      bd[(size_t)j * L.W + (i - j)] = T(di * v * ds[j]);
+    }
+  }
+  if (bad) atomicOr(&B.status[mac], bad);
+  if (threadIdx.x == 0) {
+    double *sc = B.scal + (size_t)mac * NSCAL;
+    sc[S_AMAX] = amax;
+    sc[S_SIGMA] = sigma;
+    const double scale = amax > 0.0 ? amax : 1.0;
+    sc[S_CS] = (L.k && cmax > 0.0) ? fmin(1.0, scale / cmax) : 1.0;
+  }
+}
+
+// ----------------------------------------------------------------- band Cholesky
+// Block per macro.  The lower band of column j is band[j*W + r] = L(j+r, j).
+// Window slot (col % W) holds the b+1 band entries of one active column; the
+// new column j+W enters the slot freed by column j.  Two barriers per column.
+// pairs[t] = (cc << 16) | r enumerates the rank-1 update triangle.
+template <typename T, int NT>
+__global__ void __launch_bounds__(NT)
+    k_factor(int n, int b, T *__restrict__ band, int *__restrict__ status, double tiny) {
+  extern __shared__ unsigned char smem_raw[];
+  const int W = b + 1;
+  const int npairs = b * (b + 1) / 2;
+  T *win = reinterpret_cast<T *>(smem_raw);
+  T *lc = win + (size_t)W * W;
+  unsigned *pairs = reinterpret_cast<unsigned *>(lc + W);
+  const int mac = blockIdx.x;
+  const int tid = threadIdx.x;
+  T *Lg = band + (size_t)mac * n * W;
+  for (int t = tid; t < npairs; t += NT) {
+    // decode t -> (cc, r): cc = 1..b, r = 0..b-cc, column-offset major
+    int cc = 1, rem = t;
+    while (rem >= b - cc + 1) {
+      rem -= b - cc + 1;
+      ++cc;
+    }
+    pairs[t] = ((unsigned)cc << 16) | (unsigned)rem;
+  }
+  for (int t = tid; t < W * W; t += NT) {
+    const int c = t / W, r = t - c * W;
+    win[t] = c < n ? Lg[(size_t)c * W + r] : T(0);
+  }
+  __shared__ int bad;
+  if (tid == 0) bad = 0;
+  __syncthreads();
+  int s = 0;  // slot of column j
+  for (int j = 0; j < n; ++j) {
+    if (tid < W) {
+      T d = win[s * W];
+      if (!(d > T(tiny)) || !isfinite((double)d)) {
+        if (tid == 0) bad = MHM_STATUS_PIVOT;
+        d = T(1);
+      }
+      const T root = sqrt(d);
+      lc[tid] = tid == 0 ? root : win[s * W + tid] / root;
+    }
+    __syncthreads();
+    if (tid < W) {
+      Lg[(size_t)j * W + tid] = lc[tid];
+      const int cn = j + W;
+      win[s * W + tid] = cn < n ? Lg[(size_t)cn * W + tid] : T(0);
+    }
+    for (int t = tid; t < npairs; t += NT) {
+      const unsigned p = pairs[t];
+      const int cc = (int)(p >> 16), r = (int)(p & 0xffffu);
+      int slot = s + cc;
+      if (slot >= W) slot -= W;
+      win[slot * W + r] -= lc[cc + r] * lc[cc];
+    }
+    __syncthreads();
+    s = s + 1 == W ? 0 : s + 1;
+  }
+  if (tid == 0 && bad) atomicOr(&status[mac], bad);
+}
+
+// ----------------------------------------------------------------- band solves
+// Block per macro: forward L z = y then backward L^T x = z for ncols columns of
+// y[mac][row][col] (leading dimension ldy), each sweep through a (b+1)-row
+// LDS window.  Skips macros whose every column has converged.
+template <typename T, int NT>
+__global__ void __launch_bounds__(NT)
+    k_solve(int n, int b, int ldy, int ncols, const T *__restrict__ band, T *__restrict__ y,
+            const int *__restrict__ colstate, int cs_ld, int *__restrict__ iters) {
+  extern __shared__ unsigned char smem_raw[];
+  const int mac = blockIdx.x;
+  const int tid = threadIdx.x;
+  __shared__ int active;
+  if (colstate) {
+    if (tid == 0) active = 0;
+    __syncthreads();
+    if (tid < ncols && colstate[(size_t)mac * cs_ld + tid] == 0) active = 1;
+    __syncthreads();
+    if (!active) return;
+  }
+  if (iters && tid == 0) iters[mac] += 1;
+  const int W = b + 1;
+  T *win = reinterpret_cast<T *>(smem_raw);
+  T *piv = win + (size_t)W * ncols;
+  const T *Lg = band + (size_t)mac * n * W;
+  T *Y = y + (size_t)mac * n * ldy;
+  const int nupd = b * ncols;
+  // forward
+  for (int t = tid; t < W * ncols; t += NT) {
+    const int r = t / ncols, c = t - r * ncols;
+    win[t] = r < n ? Y[(size_t)r * ldy + c] : T(0);
+  }
+  __syncthreads();
+  int s = 0;
+  for (int j = 0; j < n; ++j) {
+    if (tid < ncols) {
+      const T z = win[s * ncols + tid] / Lg[(size_t)j * W];
+      piv[tid] = z;
+      Y[(size_t)j * ldy + tid] = z;
+    }
+    __syncthreads();
+    const T *col = Lg + (size_t)j * W;
+    for (int t = tid; t < nupd; t += NT) {
+      const int r = 1 + t / ncols, c = t - (r - 1) * ncols;
+      int slot = s + r;
+      if (slot >= W) slot -= W;
+      win[slot * ncols + c] -= col[r] * piv[c];
+    }
+    if (tid < ncols) {
+      const int rn = j + W;
+      win[s * ncols + tid] = rn < n ? Y[(size_t)rn * ldy + tid] : T(0);
+    }
+    __syncthreads();
+    s = s + 1 == W ? 0 : s + 1;
+  }
+  // backward: window holds rows j-b..j, slot = row % W
+  for (int t = tid; t < W * ncols; t += NT) {
+    const int rr = t / ncols, c = t - rr * ncols;
+    const int row = n - W + rr;
+    if (row >= 0) win[(row % W) * ncols + c] = Y[(size_t)row * ldy + c];
+  }
+  __syncthreads();
+  s = (n - 1) % W;
+  for (int j = n - 1; j >= 0; --j) {
+    if (tid < ncols) {
+      const T x = win[s * ncols + tid] / Lg[(size_t)j * W];
+      piv[tid] = x;
+      Y[(size_t)j * ldy + tid] = x;
+    }
+    __syncthreads();
+    for (int t = tid; t < nupd; t += NT) {
+      const int r = 1 + t / ncols, c = t - (r - 1) * ncols;
+      if (j - r >= 0) {
+        int slot = s - r;
+        if (slot < 0) slot += W;
+        win[slot * ncols + c] -= Lg[(size_t)(j - r) * W + r] * piv[c];
+      }
+    }
+    if (tid < ncols) {
+      const int rn = j - W;
+      if (rn >= 0) win[s * ncols + tid] = Y[(size_t)rn * ldy + tid];
+    }
+    __syncthreads();
+    s = s == 0 ? W - 1 : s - 1;
+  }
+}
+
+// ----------------------------------------------------------------- right-hand sides
+// Standard columns [f, B, A Z] (and, for the factor step, e_p and c) into the
+// binary64 rhs buffer rhs[mac][col][N] (band rows; row n = constraint row = 0).
+__global__ void __launch_bounds__(256) k_std_rhs(LayoutK L, BatchK B, double *__restrict__ rhs) {
+  const int mac = blockIdx.y;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int N = B.N;
+  if (t >= N * B.R) return;
+  const int c = t / N, i = t - c * N;
+  double v = 0.0;
+  if (i < L.n) {
+    if (c == 0)
+      v = B.load[(size_t)mac * L.n + i];
+    else if (c <= L.m)
+      v = coupling_at(L, B, mac, c - 1, i);
+    else
+      v = B.az[(size_t)mac * L.n + i];
+  }
+  rhs[((size_t)mac * B.R + c) * N + i] = v;
+}
+
+// Scaled copy of binary64 columns into the factor-precision workspace.
+template <typename T>
+__global__ void __launch_bounds__(256)
+    k_load_y(int n, int N, int ncols, int ldy, const double *__restrict__ rhs,
+             const double *__restrict__ dscale, T *__restrict__ y, double *__restrict__ beta) {
+  const int mac = blockIdx.y;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n * ncols) return;
+  const int i = t / ncols, c = t - i * ncols;
+  const double *r = rhs + ((size_t)mac * ncols + c) * N;
+  y[((size_t)mac * n + i) * ldy + c] = T(dscale[(size_t)mac * n + i] * r[i]);
+  if (i == 0 && beta) beta[(size_t)mac * ncols + c] = N > n ? r[n] : 0.0;
+}
+
+// D e_p and D c into columns 0, 1 of the workspace (factor step).
+template <typename T>
+__global__ void __launch_bounds__(256) k_load_uw(LayoutK L, BatchK B, T *__restrict__ y) {
+  const int mac = blockIdx.y;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= L.n) return;
+  const double d = B.dscale[(size_t)mac * L.n + i];
+  const double cs = B.scal[(size_t)mac * NSCAL + S_CS];
+  T *row = y + ((size_t)mac * L.n + i) * B.ldy;
+  row[0] = T(i == L.pin ? d : 0.0);
+  row[1] = T(d * cs * constraint_at(L, B, mac, i));
+}
+
+// u = D y0, w = D y1 and the 2x2 bordered matrix per macro.
+template <typename T>
+__global__ void __launch_bounds__(256) k_saddle_init(LayoutK L, BatchK B, const T *__restrict__ y) {
+  __shared__ double red[4];
+  const int mac = blockIdx.x;
+  double *u = B.uw + (size_t)mac * 2 * L.n;
+  double *w = u + L.n;
+  double *sc = B.scal + (size_t)mac * NSCAL;
+  const double cs = sc[S_CS], sigma = sc[S_SIGMA];
+  double cu = 0.0, cw = 0.0;
+  for (int i = threadIdx.x; i < L.n; i += 256) {
+    const double d = B.dscale[(size_t)mac * L.n + i];
+    const T *row = y + ((size_t)mac * L.n + i) * B.ldy;
+    const double ui = d * (double)row[0], wi = d * (double)row[1];
+    u[i] = ui;
+    w[i] = wi;
+    const double ci = cs * constraint_at(L, B, mac, i);
+    cu += ci * ui;
+    cw += ci * wi;
+  }
+  cu = block_sum<256>(cu, red);
+  cw = block_sum<256>(cw, red);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const double up = u[L.pin], wp = w[L.pin];
+    const double a11 = 1.0 - sigma * up, a12 = wp, a21 = -sigma * cu, a22 = cw;
+    sc[S_A11] = a11;
+    sc[S_A12] = a12;
+    sc[S_A21] = a21;
+    sc[S_A22] = a22;
+    sc[S_DET] = a11 * a22 - a12 * a21;
+    if (!(fabs(sc[S_DET]) > 0.0) || !isfinite(sc[S_DET])) atomicOr(&B.status[mac], MHM_STATUS_PIVOT);
+  }
+}
+
+// Block per (column, macro): x = D y (+ bordered correction when k == 1);
+// mode 0 sets X, mode 1 adds the correction and records ||delta||, ||X||.
+template <typename T>
+__global__ void __launch_bounds__(256)
+    k_saddle_apply(LayoutK L, BatchK B, int ncols, const T *__restrict__ y,
+                   const double *__restrict__ beta, double *__restrict__ X, int mode,
+                   const int *__restrict__ colstate, double *__restrict__ colnorm) {
+  __shared__ double red[4];
+  const int c = blockIdx.x, mac = blockIdx.y;
+  if (mode && colstate[(size_t)mac * ncols + c]) return;
+  const int n = L.n, N = B.N;
+  double *x = X + ((size_t)mac * ncols + c) * N;
+  const double *ds = B.dscale + (size_t)mac * n;
+  const T *Y = y + (size_t)mac * n * B.ldy + c;
+  double xp = 0.0, mu = 0.0;
+  const double *sc = B.scal + (size_t)mac * NSCAL;
+  const double *u = B.uw + (size_t)mac * 2 * n;
+  const double *w = u + n;
+  if (L.k) {
+    const double cs = sc[S_CS];
+    double cy = 0.0;
+    for (int i = threadIdx.x; i < n; i += 256)
+      cy += cs * constraint_at(L, B, mac, i) * (ds[i] * (double)Y[(size_t)i * B.ldy]);
+    cy = block_sum<256>(cy, red);
+    const double yp = ds[L.pin] * (double)Y[(size_t)L.pin * B.ldy];
+    const double r1 = yp, r2 = cy - beta[(size_t)mac * ncols + c];
+    const double det = sc[S_DET];
+    xp = (r1 * sc[S_A22] - sc[S_A12] * r2) / det;
+    mu = (sc[S_A11] * r2 - sc[S_A21] * r1) / det;
+  }
+  const double sigma = sc[S_SIGMA];
+  double dn = 0.0, xn = 0.0;
+  for (int i = threadIdx.x; i < n; i += 256) {
+    double v = ds[i] * (double)Y[(size_t)i * B.ldy];
+    if (L.k) v += sigma * xp * u[i] - mu * w[i];
+    if (mode) {
+      const double nv = x[i] + v;
+      x[i] = nv;
+      dn += v * v;
+      xn += nv * nv;
+    } else {
+      x[i] = v;
+    }
+  }
+  if (L.k && threadIdx.x == 0) {
+    if (mode) {
+      x[n] += mu;
+    } else {
+      x[n] = mu;
+    }
+  }
+  if (mode) {
+    dn = block_sum<256>(dn, red);
+    xn = block_sum<256>(xn, red);
+    if (threadIdx.x == 0) {
+      colnorm[((size_t)mac * ncols + c) * 2] = dn;
+      colnorm[((size_t)mac * ncols + c) * 2 + 1] = xn;
+    }
+  }
+}
+
+// Block per (column, macro): r = b - S x accumulated in double-double, the
+// per-column acceptance ||r/s|| <= rtol ||b/s|| (s = max|b|) of
+// linalg/linear.py:132-152, and the scaled residual as the next correction
+// right-hand side.  fwd_tol > 0 additionally requires the last correction to
+// satisfy ||delta|| <= fwd_tol ||x|| (forward convergence of refinement).
+// check_only: set MHM_STATUS_RESIDUAL for unconverged columns.
+template <typename T>
+__global__ void __launch_bounds__(256)
+    k_residual(LayoutK L, BatchK B, int ncols, const double *__restrict__ rhs,
+               const double *__restrict__ X, double rtol, double fwd_tol, int iteration,
+               T *__restrict__ y, double *__restrict__ beta, int *__restrict__ colstate,
+               const double *__restrict__ colnorm, int check_only) {
+  __shared__ double red[4];
+  const int c = blockIdx.x, mac = blockIdx.y;
+  int *state = colstate + (size_t)mac * ncols + c;
+  if (!check_only && *state) return;
+  const int n = L.n, N = B.N;
+  const double *x = X + ((size_t)mac * ncols + c) * N;
+  const double *bv = rhs + ((size_t)mac * ncols + c) * N;
+  const double *a = B.aval + (size_t)mac * L.nnz;
+  const double cs = B.scal[(size_t)mac * NSCAL + S_CS];
+  const double mu = L.k ? x[n] : 0.0;
+  double bmax = 0.0;
+  for (int i = threadIdx.x; i < N; i += 256) bmax = fmax(bmax, fabs(bv[i]));
+  bmax = block_max<256>(bmax, red);
+  const double scale = bmax > 0.0 ? bmax : 1.0;
+  double rr = 0.0, bb = 0.0, chi = 0.0, clo = 0.0;
+  bool finite = true;
+  T *Y = y + (size_t)mac * n * B.ldy + c;
+  const double *ds = B.dscale + (size_t)mac * n;
+  for (int i = threadIdx.x; i < n; i += 256) {
+    double hi = bv[i], lo = 0.0;
+    for (int s = L.rowptr[i]; s < L.rowptr[i + 1]; ++s) dd_fma(hi, lo, -a[s], x[L.cols[s]]);
+    double ci = 0.0;
+    if (L.k) {
+      ci = cs * constraint_at(L, B, mac, i);
+      dd_fma(hi, lo, -ci, mu);
+      dd_fma(chi, clo, ci, x[i]);
+    }
+    const double r = hi + lo;
+    finite &= isfinite(r);
+    rr += (r / scale) * (r / scale);
+    bb += (bv[i] / scale) * (bv[i] / scale);
+    if (!check_only) Y[(size_t)i * B.ldy] = T(ds[i] * r);
+  }
+  double rn = 0.0;
+  if (L.k) {
+    const double cx = block_sum<256>(chi, red) + block_sum<256>(clo, red);
+    rn = bv[n] - cx;
+    if (threadIdx.x == 0) {
+      rr += (rn / scale) * (rn / scale);
+      bb += (bv[n] / scale) * (bv[n] / scale);
+    }
+  }
+  rr = block_sum<256>(rr, red);
+  bb = block_sum<256>(bb, red);
+  __shared__ int anybad;
+  if (threadIdx.x == 0) anybad = 0;
+  __syncthreads();
+  if (!finite) anybad = 1;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    bool ok = sqrt(rr) <= rtol * sqrt(bb) && !anybad;
+    if (ok && fwd_tol > 0.0) {
+      if (iteration == 0) {
+        ok = false;
+      } else {
+        const double dn = colnorm[((size_t)mac * ncols + c) * 2];
+        const double xn = colnorm[((size_t)mac * ncols + c) * 2 + 1];
+        ok = dn <= fwd_tol * fwd_tol * xn;
+      }
+    }
+    if (check_only) {
+      if (anybad) atomicOr(&B.status[mac], MHM_STATUS_NONFINITE);
+      if (!(sqrt(rr) <= rtol * sqrt(bb))) atomicOr(&B.status[mac], MHM_STATUS_RESIDUAL);
+    } else {
+      *state = ok ? 1 : 0;
+      beta[(size_t)mac * ncols + c] = rn;
+      if (anybad) atomicOr(&B.status[mac], MHM_STATUS_NONFINITE);
+    }
+  }
+}
+
+__global__ void k_fill_int(int *p, int v, size_t count) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < count) p[t] = v;
+}
+
+// ----------------------------------------------------------------- condensation
+// Block per macro: the condensed block of core/contributions.py:40-47 in the
+// general (corrected retained) form and the reduced load of
+// core/reconstruction.py:122-127, from s = X0, L = X1..m, E = Z - X(m+1).
+__global__ void __launch_bounds__(256)
+    k_condense(LayoutK L, BatchK B, const double *__restrict__ X, double *__restrict__ blocks,
+               double *__restrict__ loads) {
+  const int mac = blockIdx.x;
+  const int n = L.n, m = L.m, k = L.k, N = B.N;
+  const int w = m + k;
+  const double *Xm = X + (size_t)mac * B.R * N;
+  const double *s = Xm;
+  const double *az = B.az + (size_t)mac * n;
+  const double *f = B.load + (size_t)mac * n;
+  double *blk = blocks + (size_t)mac * w * w;
+  double *ld = loads + (size_t)mac * w;
+  const int nrows = L.crows ? L.ncr : n;
+  for (int t = threadIdx.x; t < m * m; t += 256) {
+    const int a = t / m, b = t - a * m;
+    const double *lb = Xm + (size_t)(1 + b) * N;
+    double acc = 0.0;
+    for (int q = 0; q < nrows; ++q) {
+      const int i = L.crows ? L.crows[q] : q;
+      acc += coupling_at(L, B, mac, a, i) * lb[i];
+    }
+    blk[a * w + b] = acc;
+  }
+  for (int a = threadIdx.x; a < m; a += 256) {
+    double acc = 0.0;
+    for (int q = 0; q < nrows; ++q) {
+      const int i = L.crows ? L.crows[q] : q;
+      acc += coupling_at(L, B, mac, a, i) * s[i];
+    }
+    ld[a] = acc;
+  }
+  if (!k) return;
+  const double *xaz = Xm + (size_t)(1 + m) * N;
+  for (int a = threadIdx.x; a < m; a += 256) {
+    const double *la = Xm + (size_t)(1 + a) * N;
+    double g = 0.0, zb = 0.0;
+    for (int q = 0; q < nrows; ++q) {
+      const int i = L.crows ? L.crows[q] : q;
+      const double bi = coupling_at(L, B, mac, a, i);
+      g += bi * (kernel_at(L, B, mac, i) - xaz[i]);
+      zb += kernel_at(L, B, mac, i) * bi;
+    }
+    double al = 0.0;
+    for (int i = 0; i < n; ++i) al += az[i] * la[i];
+    blk[a * w + m] = -g;
+    blk[m * w + a] = -(zb - al);
+  }
+  if (threadIdx.x == 0) {
+    double cm = 0.0, as = 0.0, zf = 0.0;
+    for (int i = 0; i < n; ++i) {
+      cm += az[i] * (kernel_at(L, B, mac, i) - xaz[i]);
+      as += az[i] * s[i];
+      zf += kernel_at(L, B, mac, i) * f[i];
+    }
+    blk[m * w + m] = -cm;
+    ld[m] = as - zf;
+  }
+}
+
+// Responses in original rows: source, lifts (n x m row-major), retained basis.
+__global__ void __launch_bounds__(256)
+    k_responses(LayoutK L, BatchK B, const double *__restrict__ X, double *__restrict__ src,
+                double *__restrict__ lifts, double *__restrict__ ret) {
+  const int mac = blockIdx.y;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= L.n) return;
+  const int N = B.N, n = L.n, m = L.m;
+  const double *Xm = X + (size_t)mac * B.R * N;
+  const int o = L.perm[i];
+  if (src) src[(size_t)mac * n + o] = Xm[i];
+  if (lifts)
+    for (int a = 0; a < m; ++a) lifts[((size_t)mac * n + o) * m + a] = Xm[(size_t)(1 + a) * N + i];
+  if (ret && L.k) ret[(size_t)mac * n + o] = kernel_at(L, B, mac, i) - Xm[(size_t)(1 + m) * N + i];
+}
+
+// u = s - L lambda + E c, original rows.
+__global__ void __launch_bounds__(256)
+    k_reconstruct(LayoutK L, BatchK B, const double *__restrict__ X, const double *__restrict__ tr,
+                  const double *__restrict__ co, double *__restrict__ fields, int ncases) {
+  const int mac = blockIdx.y;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= L.n) return;
+  const int N = B.N, n = L.n, m = L.m, k = L.k;
+  const double *Xm = X + (size_t)mac * B.R * N;
+  const int o = L.perm[i];
+  const double e = k ? kernel_at(L, B, mac, i) - Xm[(size_t)(1 + m) * N + i] : 0.0;
+  for (int c = 0; c < ncases; ++c) {
+    const double *lam = tr + ((size_t)mac * ncases + c) * m;
+    double t = 0.0;
+    for (int a = 0; a < m; ++a) t += Xm[(size_t)(1 + a) * N + i] * lam[a];
+    double u = Xm[i] - t;
+    if (k) u += e * co[((size_t)mac * ncases + c) * k];
+    fields[((size_t)mac * ncases + c) * n + o] = u;
+  }
+}
+
+// Permute user columns between original and band rows.
+__global__ void __launch_bounds__(256)
+    k_permute_cols(const int *__restrict__ perm, int n, int N, int ncols, const double *__restrict__ in,
+                   double *__restrict__ out, int to_band) {
+  const int mac = blockIdx.y;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= N * ncols) return;
+  const int c = t / N, i = t - c * N;
+  const size_t base = ((size_t)mac * ncols + c) * N;
+  const int o = i < n ? perm[i] : i;
+  if (to_band)
+    out[base + i] = in[base + o];
+  else
+    out[base + o] = in[base + i];
+}
+
+// Copy a per-macro [macro][rows..] double array from original to band rows.
+__global__ void __launch_bounds__(256)
+    k_permute_rows(const int *__restrict__ perm, int n, int inner, const double *__restrict__ in,
+                   double *__restrict__ out) {
+  const int mac = blockIdx.y;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n * inner) return;
+  const int a = t / n, i = t - a * n;
+  out[((size_t)mac * inner + a) * n + i] = in[((size_t)mac * inner + a) * n + perm[i]];
+}
+
+__global__ void __launch_bounds__(256)
+    k_scatter(const int64_t *__restrict__ idx, int w, int count, int64_t *__restrict__ rows,
+              int64_t *__restrict__ cols) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t per = (size_t)w * w;
+  if (t >= per * count) return;
+  const size_t c = t / per;
+  const int rem = (int)(t - c * per);
+  const int i = rem / w, j = rem - i * w;
+  rows[t] = idx[c * w + i];
+  cols[t] = idx[c * w + j];
+}
+
+__global__ void __launch_bounds__(256)
+    k_ordered_gather(const int64_t *__restrict__ ptr, const int64_t *__restrict__ code,
+                     const double *__restrict__ src, int64_t count, double *__restrict__ out,
+                     double *__restrict__ aout) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= count) return;
+  double s = 0.0, a = 0.0;
+  for (int64_t t = ptr[g]; t < ptr[g + 1]; ++t) {
+    const double v = src[code[t]];
+    s += v;
+    a += fabs(v);
+  }
+  out[g] = s;
+  if (aout) aout[g] = a;
+}
+
+// ================================================================= host side
+struct mhm_layout {
+  int device = 0;
+  LayoutK k{};
+  std::vector<void *> owned;
+  double cmax_host = 0.0;
+};
+
+struct mhm_batch {
+  const mhm_layout *layout = nullptr;
+  int M = 0, precision = MHM_FP64, maxcols = 0, R = 0, N = 0, ldy = 0;
+  size_t tsize = 8;
+  BatchK bk{};
+  void *band = nullptr, *y = nullptr;
+  double *xstd = nullptr, *rhsstd = nullptr, *beta = nullptr, *colnorm = nullptr;
+  double *rhsu = nullptr, *xu = nullptr;
+  int *colstate = nullptr;
+  double *ework = nullptr, *fwork = nullptr;
+  int *colmap = nullptr;
+  double *colscale = nullptr, *bcoup = nullptr, *cons = nullptr, *kern = nullptr;
+  bool factored = false, condensed = false;
+  std::vector<void *> owned;
+};
+
+template <typename P>
+static int dev_alloc(std::vector<void *> &owned, P **ptr, size_t count) {
+  void *p = nullptr;
+  if (count == 0) count = 1;
+  hipError_t e = hipMalloc(&p, count * sizeof(P));
+  if (e != hipSuccess)
+    return fail(e == hipErrorOutOfMemory ? MHM_ENOMEM : MHM_EDEVICE,
+                std::string("hipMalloc: ") + hipGetErrorString(e));
+  owned.push_back(p);
+  *ptr = static_cast<P *>(p);
+  return MHM_OK;
+}
+
+template <typename P>
+static int upload(std::vector<void *> &owned, const P *host, size_t count, const P **out) {
+  if (!host) {
+    *out = nullptr;
+    return MHM_OK;
+  }
+  P *d = nullptr;
+  int rc = dev_alloc(owned, &d, count);
+  if (rc) return rc;
+  HIP_TRY(hipMemcpy(d, host, count * sizeof(P), hipMemcpyHostToDevice));
+  *out = d;
+  return MHM_OK;
+}
+
+struct DeviceGuard {
+  int prev = -1;
+  explicit DeviceGuard(int dev) {
+    if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+    if (prev != dev) (void)hipSetDevice(dev);
+  }
+  ~DeviceGuard() {
+    if (prev >= 0) (void)hipSetDevice(prev);
+  }
+};
+
+static size_t factor_lds(int b, size_t tsize) {
+  const size_t W = (size_t)b + 1;
+  return W * W * tsize + W * tsize + (size_t)(b * (b + 1) / 2) * 4 + 16;
+}
+
+static size_t solve_lds(int b, int ncols, size_t tsize) {
+  return ((size_t)b + 1) * ncols * tsize + (size_t)ncols * tsize + 16;
+}
+
+static int lds_limit(int device) {
+  int v = 0;
+  if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, device) != hipSuccess ||
+      v <= 0)
+    v = 65536;
+  return v;
+}
+
+// Exported entry points: C linkage comes from their declarations in mhm_hip.h.
+
+int mhm_last_error(char *buf, size_t len) {
+  if (buf && len) {
+    std::strncpy(buf, g_error.c_str(), len - 1);
+    buf[len - 1] = 0;
+  }
+  return MHM_OK;
+}
+
+int mhm_device_count(int32_t *count) {
+  if (!count) return fail(MHM_EINVAL, "count is NULL");
+  int c = 0;
+  hipError_t e = hipGetDeviceCount(&c);
+  *count = e == hipSuccess ? c : 0;
+  return MHM_OK;
+}
+
+int mhm_version(int32_t *major, int32_t *minor) {
+  if (major) *major = 1;
+  if (minor) *minor = 0;
+  return MHM_OK;
+}
+
+int mhm_layout_create(int32_t device, const mhm_layout_desc *d, mhm_layout **out) {
+  if (!d || !out) return fail(MHM_EINVAL, "desc/out is NULL");
+  *out = nullptr;
+  if (d->n <= 0 || d->m < 0 || d->k < 0 || d->k > 1 || d->nnz <= 0 || d->bandwidth < 0 ||
+      d->bandwidth >= d->n + 1 || !d->perm || !d->rowptr || !d->cols)
+    return fail(MHM_EINVAL, "layout sizes or pattern arrays are invalid");
+  if (d->k && (d->pin < 0 || d->pin >= d->n || !d->constraint))
+    return fail(MHM_EINVAL, "k == 1 requires a pin row and a constraint vector");
+  if (d->bandwidth >= 65535) return fail(MHM_EINVAL, "bandwidth exceeds 65534");
+  // host validation of the pattern: sorted rows, in range, symmetric bandwidth
+  std::vector<int> diag(d->n, -1);
+  for (int i = 0; i < d->n; ++i) {
+    if (d->rowptr[i] > d->rowptr[i + 1]) return fail(MHM_EINVAL, "rowptr not monotone");
+    for (int s = d->rowptr[i]; s < d->rowptr[i + 1]; ++s) {
+      const int j = d->cols[s];
+      if (j < 0 || j >= d->n || (s > d->rowptr[i] && d->cols[s - 1] >= j))
+        return fail(MHM_EINVAL, "pattern columns must be sorted and in range");
+      if (std::abs(i - j) > d->bandwidth) return fail(MHM_EINVAL, "entry outside bandwidth");
+      if (j == i) diag[i] = s;
+    }
+  }
+  if (d->rowptr[0] != 0 || d->rowptr[d->n] != d->nnz) return fail(MHM_EINVAL, "rowptr bounds");
+  std::vector<int> seen(d->n, 0);
+  for (int i = 0; i < d->n; ++i) {
+    if (d->perm[i] < 0 || d->perm[i] >= d->n || seen[d->perm[i]]++)
+      return fail(MHM_EINVAL, "perm must be a permutation");
+  }
+  if (d->ne > 0) {
+    if (!((d->dim == 2 && (d->nb == 3 || d->nb == 6)) || (d->dim == 3 && (d->nb == 4 || d->nb == 10))))
+      return fail(MHM_EINVAL, "assembly supports P1/P2 triangles and tetrahedra");
+    if (!d->slot_ptr || !d->slot_code || !d->load_ptr || !d->load_code || !d->grad_lambda ||
+        !d->measure || !d->qweights || !d->dtable || !d->vtable || d->nq <= 0)
+      return fail(MHM_EINVAL, "assembly arrays missing");
+    const int64_t ncode = (int64_t)d->ne * d->nb * d->nb;
+    if (d->slot_ptr[0] != 0 || d->slot_ptr[d->nnz] != ncode || d->load_ptr[0] != 0 ||
+        d->load_ptr[d->n] != (int64_t)d->ne * d->nb)
+      return fail(MHM_EINVAL, "assembly contribution lists inconsistent");
+    for (int64_t t = 0; t < ncode; ++t)
+      if (d->slot_code[t] < 0 || d->slot_code[t] >= ncode) return fail(MHM_EINVAL, "slot_code range");
+    for (int64_t t = 0; t < (int64_t)d->ne * d->nb; ++t)
+      if (d->load_code[t] < 0 || d->load_code[t] >= (int64_t)d->ne * d->nb)
+        return fail(MHM_EINVAL, "load_code range");
+  }
+  if (d->coupling_rows)
+    for (int q = 0; q < d->ncoupling_rows; ++q)
+      if (d->coupling_rows[q] < 0 || d->coupling_rows[q] >= d->n)
+        return fail(MHM_EINVAL, "coupling_rows range");
+  int ndev = 0;
+  HIP_TRY(hipGetDeviceCount(&ndev));
+  if (device < 0 || device >= ndev) return fail(MHM_EINVAL, "device outside visible devices");
+  DeviceGuard guard(device);
+  auto *L = new mhm_layout();
+  L->device = device;
+  LayoutK &k = L->k;
+  k.n = d->n;
+  k.k = d->k;
+  k.m = d->m;
+  k.nnz = d->nnz;
+  k.b = d->bandwidth;
+  k.W = d->bandwidth + 1;
+  k.pin = d->k ? d->pin : -1;
+  k.ne = d->ne;
+  k.nb = d->nb;
+  k.dim = d->dim;
+  k.nq = d->nq;
+  k.ncr = d->coupling_rows ? d->ncoupling_rows : 0;
+  int rc = 0;
+  auto cleanup = [&](int code) {
+    for (void *p : L->owned) (void)hipFree(p);
+    delete L;
+    return code;
+  };
+  const size_t n = d->n;
+  if ((rc = upload(L->owned, d->perm, n, &k.perm)) || (rc = upload(L->owned, d->rowptr, n + 1, &k.rowptr)) ||
+      (rc = upload(L->owned, d->cols, (size_t)d->nnz, &k.cols)) ||
+      (rc = upload(L->owned, diag.data(), n, &k.diagslot)) ||
+      (rc = upload(L->owned, d->coupling_rows, (size_t)k.ncr, &k.crows)) ||
+      (rc = upload(L->owned, d->coupling, (size_t)d->m * n, &k.coupling)) ||
+      (rc = upload(L->owned, d->kernel, n, &k.kernel)) ||
+      (rc = upload(L->owned, d->constraint, n, &k.constraint)))
+    return cleanup(rc);
+  if (d->ne > 0) {
+    const size_t ne = d->ne, nb = d->nb, nq = d->nq, dim = d->dim;
+    if ((rc = upload(L->owned, d->slot_ptr, (size_t)d->nnz + 1, &k.slot_ptr)) ||
+        (rc = upload(L->owned, d->slot_code, ne * nb * nb, &k.slot_code)) ||
+        (rc = upload(L->owned, d->load_ptr, n + 1, &k.load_ptr)) ||
+        (rc = upload(L->owned, d->load_code, ne * nb, &k.load_code)) ||
+        (rc = upload(L->owned, d->grad_lambda, ne * (dim + 1) * dim, &k.glam)) ||
+        (rc = upload(L->owned, d->measure, ne, &k.measure)) ||
+        (rc = upload(L->owned, d->qweights, nq, &k.qw)) ||
+        (rc = upload(L->owned, d->dtable, nq * nb * (dim + 1), &k.dtab)) ||
+        (rc = upload(L->owned, d->vtable, nq * nb, &k.vtab)))
+      return cleanup(rc);
+  }
+  *out = L;
+  return MHM_OK;
+}
+
+int mhm_layout_destroy(mhm_layout *L) {
+  if (!L) return MHM_OK;
+  DeviceGuard guard(L->device);
+  (void)hipDeviceSynchronize();
+  for (void *p : L->owned) (void)hipFree(p);
+  delete L;
+  return MHM_OK;
+}
+
+int mhm_layout_info(const mhm_layout *L, int32_t *n, int32_t *bw, int32_t *nnz) {
+  if (!L) return fail(MHM_EINVAL, "layout is NULL");
+  if (n) *n = L->k.n;
+  if (bw) *bw = L->k.b;
+  if (nnz) *nnz = L->k.nnz;
+  return MHM_OK;
+}
+
+int mhm_batch_create(const mhm_layout *L, int32_t macros, int32_t precision, int32_t max_columns,
+                     mhm_batch **out) {
+  if (!L || !out) return fail(MHM_EINVAL, "layout/out is NULL");
+  *out = nullptr;
+  if (macros <= 0 || macros > 65535) return fail(MHM_EINVAL, "macros must be in [1, 65535]");
+  if (precision != MHM_FP64 && precision != MHM_MIXED) return fail(MHM_EINVAL, "unknown precision");
+  if (max_columns < 0) return fail(MHM_EINVAL, "max_columns must be nonnegative");
+  const LayoutK &lk = L->k;
+  const size_t tsize = precision == MHM_FP64 ? 8 : 4;
+  const int dev = L->device;
+  const int limit = lds_limit(dev);
+  const int nt = lk.b >= 12 ? 256 : 64;
+  if (lk.b + 1 > nt || std::max(std::max(1 + lk.m + lk.k, 2), max_columns) > nt)
+    return fail(MHM_EINVAL, "bandwidth or column count exceeds the workgroup size");
+  if (factor_lds(lk.b, tsize) > (size_t)limit)
+    return fail(MHM_EINVAL, "bandwidth " + std::to_string(lk.b) +
+                                " too large for the LDS-resident factor window at this precision");
+  DeviceGuard guard(dev);
+  auto *b = new mhm_batch();
+  b->layout = L;
+  b->M = macros;
+  b->precision = precision;
+  b->tsize = tsize;
+  b->R = 1 + lk.m + lk.k;
+  b->N = lk.n + lk.k;
+  b->maxcols = max_columns;
+  b->ldy = std::max(std::max(b->R, 2), max_columns);
+  if (solve_lds(lk.b, b->ldy, tsize) > (size_t)limit) {
+    delete b;
+    return fail(MHM_EINVAL, "too many columns for the LDS solve window");
+  }
+  const size_t M = macros, n = lk.n, N = b->N, R = b->R, ncm = std::max(b->R, max_columns);
+  BatchK &k = b->bk;
+  k.M = macros;
+  k.R = b->R;
+  k.N = b->N;
+  k.ldy = b->ldy;
+  int rc = 0;
+  auto cleanup = [&](int code) {
+    for (void *p : b->owned) (void)hipFree(p);
+    delete b;
+    return code;
+  };
+  if ((rc = dev_alloc(b->owned, &k.aval, M * lk.nnz)) || (rc = dev_alloc(b->owned, &k.load, M * n)) ||
+      (rc = dev_alloc(b->owned, &k.az, M * n)) || (rc = dev_alloc(b->owned, &k.dscale, M * n)) ||
+      (rc = dev_alloc(b->owned, &k.scal, M * NSCAL)) || (rc = dev_alloc(b->owned, &k.uw, M * 2 * n)) ||
+      (rc = dev_alloc(b->owned, &k.status, M)) || (rc = dev_alloc(b->owned, &k.iters, M)) ||
+      (rc = dev_alloc(b->owned, &b->xstd, M * R * N)) ||
+      (rc = dev_alloc(b->owned, &b->rhsstd, M * R * N)) ||
+      (rc = dev_alloc(b->owned, &b->beta, M * ncm)) ||
+      (rc = dev_alloc(b->owned, &b->colnorm, M * ncm * 2)) ||
+      (rc = dev_alloc(b->owned, &b->colstate, M * ncm)))
+    return cleanup(rc);
+  if (max_columns > 0 &&
+      ((rc = dev_alloc(b->owned, &b->rhsu, M * max_columns * N)) ||
+       (rc = dev_alloc(b->owned, &b->xu, M * max_columns * N))))
+    return cleanup(rc);
+  {
+    unsigned char *p = nullptr;
+    if ((rc = dev_alloc(b->owned, &p, M * n * lk.W * tsize))) return cleanup(rc);
+    b->band = p;
+    if ((rc = dev_alloc(b->owned, &p, M * n * b->ldy * tsize))) return cleanup(rc);
+    b->y = p;
+  }
+  HIP_TRY(hipMemset(k.status, 0, M * sizeof(int)));
+  HIP_TRY(hipMemset(k.iters, 0, M * sizeof(int)));
+  HIP_TRY(hipMemset(k.scal, 0, M * NSCAL * sizeof(double)));
+  *out = b;
+  return MHM_OK;
+}
+
+int mhm_batch_destroy(mhm_batch *b) {
+  if (!b) return MHM_OK;
+  DeviceGuard guard(b->layout->device);
+  (void)hipDeviceSynchronize();
+  for (void *p : b->owned) (void)hipFree(p);
+  delete b;
+  return MHM_OK;
+}
+
+static dim3 grid2(size_t work, int threads, int macros) {
+  return dim3((unsigned)((work + threads - 1) / threads), (unsigned)macros);
+}
+
+template <int NB, int D>
+static int launch_assemble(const mhm_batch *b, const double *kq, const double *fq, double fc,
+                           hipStream_t st) {
+  const LayoutK &L = b->layout->k;
+  const size_t shm = ((size_t)L.nq * NB * (D + 1) + (size_t)L.nq * NB + L.nq) * sizeof(double);
+  hipLaunchKernelGGL((k_assemble<NB, D>), grid2(L.ne, 128, b->M), dim3(128), shm, st, L, kq, fq, fc,
+                     b->ework, b->fwork);
+  LAUNCH_CHECK("k_assemble");
+  return MHM_OK;
+}
+
+int mhm_batch_assemble(mhm_batch *b, const double *kq, const double *fq, double f_const, void *stream) {
+  if (!b || !kq) return fail(MHM_EINVAL, "batch/kq is NULL");
+  const LayoutK &L = b->layout->k;
+  if (L.ne <= 0) return fail(MHM_EINVAL, "layout has no assembly description");
+  DeviceGuard guard(b->layout->device);
+  hipStream_t st = (hipStream_t)stream;
+  if (!b->ework) {
+    int rc;
+    if ((rc = dev_alloc(b->owned, &b->ework, (size_t)b->M * L.nb * L.nb * L.ne)) ||
+        (rc = dev_alloc(b->owned, &b->fwork, (size_t)b->M * L.nb * L.ne)))
+      return rc;
+  }
+  int rc = MHM_OK;
+  if (L.dim == 2 && L.nb == 3)
+    rc = launch_assemble<3, 2>(b, kq, fq, f_const, st);
+  else if (L.dim == 2 && L.nb == 6)
+    rc = launch_assemble<6, 2>(b, kq, fq, f_const, st);
+  else if (L.dim == 3 && L.nb == 4)
+    rc = launch_assemble<4, 3>(b, kq, fq, f_const, st);
+  else
+    rc = launch_assemble<10, 3>(b, kq, fq, f_const, st);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_gather, grid2(L.n, 128, b->M), dim3(128), 0, st, L, b->bk, b->ework, b->fwork);
+  LAUNCH_CHECK("k_gather");
+  b->factored = b->condensed = false;
+  return MHM_OK;
+}
+
+int mhm_batch_set_operator(mhm_batch *b, const double *values, const double *loads, void *stream) {
+  if (!b || !values || !loads) return fail(MHM_EINVAL, "batch/values/loads is NULL");
+  const LayoutK &L = b->layout->k;
+  DeviceGuard guard(b->layout->device);
+  hipStream_t st = (hipStream_t)stream;
+  HIP_TRY(hipMemcpyAsync(b->bk.aval, values, (size_t)b->M * L.nnz * sizeof(double),
+                         hipMemcpyDeviceToDevice, st));
+  hipLaunchKernelGGL(k_rowsum, grid2(L.n, 128, b->M), dim3(128), 0, st, L, b->bk, loads);
+  LAUNCH_CHECK("k_rowsum");
+  b->factored = b->condensed = false;
+  return MHM_OK;
+}
+
+static int permuted_copy(mhm_batch *b, const double *src, double **dst, int inner, hipStream_t st) {
+  const LayoutK &L = b->layout->k;
+  if (!*dst) {
+    int rc = dev_alloc(b->owned, dst, (size_t)b->M * inner * L.n);
+    if (rc) return rc;
+  }
+  hipLaunchKernelGGL(k_permute_rows, grid2((size_t)L.n * inner, 256, b->M), dim3(256), 0, st, L.perm,
+                     L.n, inner, src, *dst);
+  LAUNCH_CHECK("k_permute_rows");
+  return MHM_OK;
+}
+
+int mhm_batch_set_coupling(mhm_batch *b, const int32_t *colmap, const double *colscale,
+                           const double *couplings, const double *constraints, const double *kernels,
+                           void *stream) {
+  if (!b) return fail(MHM_EINVAL, "batch is NULL");
+  const LayoutK &L = b->layout->k;
+  DeviceGuard guard(b->layout->device);
+  hipStream_t st = (hipStream_t)stream;
+  int rc;
+  const size_t mm = (size_t)b->M * std::max(L.m, 1);
+  if (colmap) {
+    if (!L.coupling) return fail(MHM_EINVAL, "a column map needs a layout coupling");
+    if (!b->colmap && (rc = dev_alloc(b->owned, &b->colmap, mm))) return rc;
+    HIP_TRY(hipMemcpyAsync(b->colmap, colmap, (size_t)b->M * L.m * sizeof(int), hipMemcpyDeviceToDevice, st));
+    b->bk.colmap = b->colmap;
+  }
+  if (colscale) {
+    if (!b->colscale && (rc = dev_alloc(b->owned, &b->colscale, mm))) return rc;
+    HIP_TRY(hipMemcpyAsync(b->colscale, colscale, (size_t)b->M * L.m * sizeof(double), hipMemcpyDeviceToDevice, st));
+    b->bk.colscale = b->colscale;
+  }
+  if (couplings) {
+    if ((rc = permuted_copy(b, couplings, &b->bcoup, L.m, st))) return rc;
+    b->bk.bcoup = b->bcoup;
+  }
+  if (constraints) {
+    if ((rc = permuted_copy(b, constraints, &b->cons, 1, st))) return rc;
+    b->bk.cons = b->cons;
+  }
+  if (kernels) {
+    if ((rc = permuted_copy(b, kernels, &b->kern, 1, st))) return rc;
+    b->bk.kern = b->kern;
+  }
+  if (!L.coupling && !b->bk.bcoup && L.m > 0)
+    return fail(MHM_EINVAL, "no coupling: the layout has none and no per-macro couplings were set");
+  if (L.k && !L.constraint && !b->bk.cons) return fail(MHM_EINVAL, "no constraint vectors");
+  return MHM_OK;
+}
+
+template <typename T>
+static int run_factor(mhm_batch *b, hipStream_t st) {
+  const mhm_layout *lay = b->layout;
+  const LayoutK &L = lay->k;
+  hipLaunchKernelGGL(k_fill_int, dim3((b->M + 255) / 256), dim3(256), 0, st, b->bk.status, 0, (size_t)b->M);
+  hipLaunchKernelGGL((k_prepare<T>), dim3(b->M), dim3(256), 0, st, L, b->bk, (T *)b->band);
+  LAUNCH_CHECK("k_prepare");
+  const size_t shm = factor_lds(L.b, sizeof(T));
+  const double tiny = (double)L.n * (sizeof(T) == 8 ? 2.220446049250313e-16 : 1.1920929e-07);
+  if (L.b >= 12) {
+    if (shm > 65536)
+      HIP_TRY(hipFuncSetAttribute((const void *)k_factor<T, 256>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+    hipLaunchKernelGGL((k_factor<T, 256>), dim3(b->M), dim3(256), shm, st, L.n, L.b, (T *)b->band,
+                       b->bk.status, tiny);
+  } else {
+    hipLaunchKernelGGL((k_factor<T, 64>), dim3(b->M), dim3(64), shm, st, L.n, L.b, (T *)b->band,
+                       b->bk.status, tiny);
+  }
+  LAUNCH_CHECK("k_factor");
+  if (L.k) {
+    hipLaunchKernelGGL((k_load_uw<T>), grid2(L.n, 256, b->M), dim3(256), 0, st, L, b->bk, (T *)b->y);
+    LAUNCH_CHECK("k_load_uw");
+    const size_t sshm = solve_lds(L.b, 2, sizeof(T));
+    if (L.b >= 12)
+      hipLaunchKernelGGL((k_solve<T, 256>), dim3(b->M), dim3(256), sshm, st, L.n, L.b, b->ldy, 2,
+                         (const T *)b->band, (T *)b->y, (const int *)nullptr, 0, (int *)nullptr);
+    else
+      hipLaunchKernelGGL((k_solve<T, 64>), dim3(b->M), dim3(64), sshm, st, L.n, L.b, b->ldy, 2,
+                         (const T *)b->band, (T *)b->y, (const int *)nullptr, 0, (int *)nullptr);
+    LAUNCH_CHECK("k_solve(uw)");
+    hipLaunchKernelGGL((k_saddle_init<T>), dim3(b->M), dim3(256), 0, st, L, b->bk, (const T *)b->y);
+    LAUNCH_CHECK("k_saddle_init");
+  }
+  return MHM_OK;
+}
+
+int mhm_batch_factor(mhm_batch *b, void *stream) {
+  if (!b) return fail(MHM_EINVAL, "batch is NULL");
+  const LayoutK &L = b->layout->k;
+  if (L.m > 0 && !L.coupling && !b->bk.bcoup)
+    return fail(MHM_EINVAL, "no coupling: set per-macro couplings before factoring");
+  if (L.k && !L.constraint && !b->bk.cons) return fail(MHM_EINVAL, "no constraint vectors");
+  DeviceGuard guard(b->layout->device);
+  hipStream_t st = (hipStream_t)stream;
+  int rc = b->precision == MHM_FP64 ? run_factor<double>(b, st) : run_factor<float>(b, st);
+  if (rc) return rc;
+  b->factored = true;
+  b->condensed = false;
+  return MHM_OK;
+}
+
+// Solve ncols binary64 columns rhs[mac][col][N] (band rows) into X, refining.
+template <typename T>
+static int run_solve(mhm_batch *b, const double *rhs, double *X, int ncols, int max_refine,
+                     double rtol, hipStream_t st) {
+  const LayoutK &L = b->layout->k;
+  const int NT = L.b >= 12 ? 256 : 64;
+  const size_t sshm = solve_lds(L.b, ncols, sizeof(T));
+  const double fwd = b->precision == MHM_MIXED ? 1.0e-15 : 0.0;
+  hipLaunchKernelGGL(k_fill_int, dim3((unsigned)(((size_t)b->M * ncols + 255) / 256)), dim3(256), 0, st,
+                     b->colstate, 0, (size_t)b->M * ncols);
+  hipLaunchKernelGGL(k_fill_int, dim3((b->M + 255) / 256), dim3(256), 0, st, b->bk.iters, 0, (size_t)b->M);
+  hipLaunchKernelGGL((k_load_y<T>), grid2((size_t)L.n * ncols, 256, b->M), dim3(256), 0, st, L.n, b->N,
+                     ncols, b->ldy, rhs, b->bk.dscale, (T *)b->y, b->beta);
+  LAUNCH_CHECK("k_load_y");
+  for (int it = 0; it <= max_refine; ++it) {
+    if (it > 0) {
+      hipLaunchKernelGGL((k_residual<T>), dim3(ncols, b->M), dim3(256), 0, st, L, b->bk, ncols, rhs,
+                         (const double *)X, rtol, fwd, it - 1, (T *)b->y, b->beta, b->colstate,
+                         (const double *)b->colnorm, 0);
+      LAUNCH_CHECK("k_residual");
+    }
+    if (NT == 256)
+      hipLaunchKernelGGL((k_solve<T, 256>), dim3(b->M), dim3(256), sshm, st, L.n, L.b, b->ldy, ncols,
+                         (const T *)b->band, (T *)b->y, (const int *)b->colstate, ncols, b->bk.iters);
+    else
+      hipLaunchKernelGGL((k_solve<T, 64>), dim3(b->M), dim3(64), sshm, st, L.n, L.b, b->ldy, ncols,
+                         (const T *)b->band, (T *)b->y, (const int *)b->colstate, ncols, b->bk.iters);
+    LAUNCH_CHECK("k_solve");
+    hipLaunchKernelGGL((k_saddle_apply<T>), dim3(ncols, b->M), dim3(256), 0, st, L, b->bk, ncols,
+                       (const T *)b->y, (const double *)b->beta, X, it > 0 ? 1 : 0,
+                       (const int *)b->colstate, b->colnorm);
+    LAUNCH_CHECK("k_saddle_apply");
+  }
+  hipLaunchKernelGGL((k_residual<T>), dim3(ncols, b->M), dim3(256), 0, st, L, b->bk, ncols, rhs,
+                     (const double *)X, rtol, 0.0, 0, (T *)b->y, b->beta, b->colstate,
+                     (const double *)b->colnorm, 1);
+  LAUNCH_CHECK("k_residual(check)");
+  return MHM_OK;
+}
+
+int mhm_batch_condense(mhm_batch *b, int32_t max_refine, double rtol, double *blocks, double *loads,
+                       void *stream) {
+  if (!b || !blocks || !loads) return fail(MHM_EINVAL, "batch/blocks/loads is NULL");
+  if (!b->factored) return fail(MHM_EINVAL, "batch is not factored");
+  if (max_refine < 0 || max_refine > 64 || !(rtol > 0.0)) return fail(MHM_EINVAL, "bad refinement arguments");
+  const LayoutK &L = b->layout->k;
+  DeviceGuard guard(b->layout->device);
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_std_rhs, grid2((size_t)b->N * b->R, 256, b->M), dim3(256), 0, st, L, b->bk,
+                     b->rhsstd);
+  LAUNCH_CHECK("k_std_rhs");
+  int rc = b->precision == MHM_FP64
+               ? run_solve<double>(b, b->rhsstd, b->xstd, b->R, max_refine, rtol, st)
+               : run_solve<float>(b, b->rhsstd, b->xstd, b->R, max_refine, rtol, st);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_condense, dim3(b->M), dim3(256), 0, st, L, b->bk, (const double *)b->xstd, blocks,
+                     loads);
+  LAUNCH_CHECK("k_condense");
+  b->condensed = true;
+  return MHM_OK;
+}
+
+int mhm_batch_responses(mhm_batch *b, double *src, double *lifts, double *ret, void *stream) {
+  if (!b) return fail(MHM_EINVAL, "batch is NULL");
+  if (!b->condensed) return fail(MHM_EINVAL, "no condensed responses");
+  const LayoutK &L = b->layout->k;
+  DeviceGuard guard(b->layout->device);
+  hipLaunchKernelGGL(k_responses, grid2(L.n, 256, b->M), dim3(256), 0, (hipStream_t)stream, L, b->bk,
+                     (const double *)b->xstd, src, lifts, ret);
+  LAUNCH_CHECK("k_responses");
+  return MHM_OK;
+}
+
+int mhm_batch_solve(mhm_batch *b, const double *rhs, double *x, int32_t ncols, int32_t max_refine,
+                    double rtol, void *stream) {
+  if (!b || !rhs || !x) return fail(MHM_EINVAL, "batch/rhs/x is NULL");
+  if (!b->factored) return fail(MHM_EINVAL, "batch is not factored");
+  if (ncols <= 0 || ncols > b->maxcols) return fail(MHM_EINVAL, "ncols outside [1, max_columns]");
+  if (max_refine < 0 || max_refine > 64 || !(rtol > 0.0)) return fail(MHM_EINVAL, "bad refinement arguments");
+  const LayoutK &L = b->layout->k;
+  DeviceGuard guard(b->layout->device);
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_permute_cols, grid2((size_t)b->N * ncols, 256, b->M), dim3(256), 0, st, L.perm, L.n,
+                     b->N, ncols, rhs, b->rhsu, 1);
+  LAUNCH_CHECK("k_permute_cols");
+  int rc = b->precision == MHM_FP64 ? run_solve<double>(b, b->rhsu, b->xu, ncols, max_refine, rtol, st)
+                                    : run_solve<float>(b, b->rhsu, b->xu, ncols, max_refine, rtol, st);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_permute_cols, grid2((size_t)b->N * ncols, 256, b->M), dim3(256), 0, st, L.perm, L.n,
+                     b->N, ncols, (const double *)b->xu, x, 0);
+  LAUNCH_CHECK("k_permute_cols");
+  return MHM_OK;
+}
+
+int mhm_batch_reconstruct(mhm_batch *b, const double *tr, const double *co, double *fields, int32_t ncases,
+                          void *stream) {
+  if (!b || !tr || !fields || ncases <= 0) return fail(MHM_EINVAL, "bad reconstruct arguments");
+  if (!b->condensed) return fail(MHM_EINVAL, "no condensed responses");
+  const LayoutK &L = b->layout->k;
+  if (L.k && !co) return fail(MHM_EINVAL, "coarse amplitudes required");
+  DeviceGuard guard(b->layout->device);
+  hipLaunchKernelGGL(k_reconstruct, grid2(L.n, 256, b->M), dim3(256), 0, (hipStream_t)stream, L, b->bk,
+                     (const double *)b->xstd, tr, co, fields, ncases);
+  LAUNCH_CHECK("k_reconstruct");
+  return MHM_OK;
+}
+
+int mhm_batch_status(mhm_batch *b, int32_t *host_status, int32_t *any, void *stream) {
+  if (!b) return fail(MHM_EINVAL, "batch is NULL");
+  DeviceGuard guard(b->layout->device);
+  std::vector<int> tmp(b->M);
+  HIP_TRY(hipMemcpyAsync(tmp.data(), b->bk.status, b->M * sizeof(int), hipMemcpyDeviceToHost,
+                         (hipStream_t)stream));
+  HIP_TRY(hipStreamSynchronize((hipStream_t)stream));
+  int acc = 0;
+  for (int i = 0; i < b->M; ++i) {
+    acc |= tmp[i];
+    if (host_status) host_status[i] = tmp[i];
+  }
+  if (any) *any = acc;
+  return MHM_OK;
+}
+
+int mhm_batch_iterations(mhm_batch *b, int32_t *host_counts, void *stream) {
+  if (!b || !host_counts) return fail(MHM_EINVAL, "batch/host_counts is NULL");
+  DeviceGuard guard(b->layout->device);
+  HIP_TRY(hipMemcpyAsync(host_counts, b->bk.iters, b->M * sizeof(int), hipMemcpyDeviceToHost,
+                         (hipStream_t)stream));
+  HIP_TRY(hipStreamSynchronize((hipStream_t)stream));
+  return MHM_OK;
+}
+
+int mhm_scatter_coordinates(const int64_t *idx, int32_t width, int32_t count, int64_t *rows, int64_t *cols,
+                            void *stream) {
+  if (!idx || !rows || !cols || width <= 0 || count < 0) return fail(MHM_EINVAL, "bad scatter arguments");
+  if (count == 0) return MHM_OK;
+  const size_t total = (size_t)width * width * count;
+  hipLaunchKernelGGL(k_scatter, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, (hipStream_t)stream, idx,
+                     width, count, rows, cols);
+  LAUNCH_CHECK("k_scatter");
+  return MHM_OK;
+}
+
+int mhm_ordered_gather(const int64_t *ptr, const int64_t *code, const double *src, int64_t count, double *out,
+                       double *aout, void *stream) {
+  if (!ptr || !code || !src || !out || count < 0) return fail(MHM_EINVAL, "bad gather arguments");
+  if (count == 0) return MHM_OK;
+  hipLaunchKernelGGL(k_ordered_gather, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, (hipStream_t)stream,
+                     ptr, code, src, count, out, aout);
+  LAUNCH_CHECK("k_ordered_gather");
+  return MHM_OK;
+}
+

@@ -1,0 +1,122 @@
+"""Ordered global scatter of condensed blocks (replaces ``core/contributions.py:77-195``).
+
+The reference expands each cell's ``(m+k)^2`` block into row-major COO
+coordinates in cell order (``core/_assembly.py:9-25``), sums duplicates in the
+COO -> CSC conversion, drops explicit zeros, and accumulates the reduced loads
+and their absolute values serially in cell order (``core/_assembly.py:28-42``).
+
+Here the integer work is fixed once per macro mesh: the COO stream, the CSC
+pattern it produces and, for every global nonzero and every global load row,
+the ordered list of stream positions that sum into it.  Values are then
+gathered on the GPU in that fixed order (``mhm_ordered_gather``), so results
+are deterministic and independent of batching.  Shared macrofaces give at most
+two contributions per entry, for which the order of an IEEE sum is immaterial,
+so the values equal the reference's bit for bit given equal blocks.
+"""
+
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import torch
+from scipy import sparse
+
+from . import _lib
+
+
+def _ptr(t):
+    return ctypes.c_void_p(0 if t is None else t.data_ptr())
+
+
+def _stream():
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _groups(keys):
+    """Stable grouping of stream positions by key -> (unique keys, ptr, positions)."""
+    order = np.argsort(keys, kind="stable")
+    uniq, counts = np.unique(keys[order], return_counts=True)
+    ptr = np.zeros(len(uniq) + 1, dtype=np.int64)
+    ptr[1:] = np.cumsum(counts)
+    return uniq, ptr, order.astype(np.int64)
+
+
+class HybridScatter:
+    """Fixed scatter plan of one macro mesh (indices ``(M, w)`` per cell in cell order)."""
+
+    def __init__(self, indices, trace_size: int, kernel_offsets, device=0):
+        idx = np.asarray(indices, dtype=np.int64)
+        if idx.ndim != 2:
+            raise ValueError("indices must be (cells, width)")
+        offs = np.asarray(kernel_offsets, dtype=np.int64)
+        self.size = int(offs[-1])
+        self.trace_size = int(trace_size)
+        M, w = idx.shape
+        if len(offs) != M + 1 or offs[0] != trace_size or np.any(np.diff(offs) < 0):
+            raise ValueError("kernel_offsets must be an ordered partition starting at trace_size")
+        if np.any(idx < 0) or np.any(idx >= self.size):
+            raise ValueError("contribution indices must be valid global integers")
+        for c in range(M):
+            row = idx[c]
+            if len(np.unique(row)) != w:
+                raise ValueError("contribution indices must be distinct")
+            if not np.array_equal(np.sort(row[row >= trace_size]), np.arange(offs[c], offs[c + 1])):
+                raise ValueError("contribution coarse indices must match its ordered cell partition")
+        if not np.array_equal(np.unique(idx), np.arange(self.size)):
+            raise ValueError("contributions must cover the contiguous global numbering")
+        self.M, self.w = M, w
+        self.indices = idx
+        self.device = torch.device("cuda", device) if torch.cuda.is_available() else None
+        rows = np.repeat(idx, w, axis=1).ravel()
+        cols = np.tile(idx, (1, w)).ravel()
+        keys, vptr, vcode = _groups(cols * self.size + rows)
+        self.csc_indices = (keys % self.size).astype(np.int32)
+        colcount = np.bincount((keys // self.size).astype(np.int64), minlength=self.size)
+        self.csc_indptr = np.zeros(self.size + 1, dtype=np.int32)
+        self.csc_indptr[1:] = np.cumsum(colcount)
+        _, lptr, lcode = _groups(idx.ravel())
+        self._host = dict(vptr=vptr, vcode=vcode, lptr=lptr, lcode=lcode)
+        if self.device is not None:
+            self.vptr = torch.as_tensor(vptr, device=self.device)
+            self.vcode = torch.as_tensor(vcode, device=self.device)
+            self.lptr = torch.as_tensor(lptr, device=self.device)
+            self.lcode = torch.as_tensor(lcode, device=self.device)
+            self.d_indices = torch.as_tensor(idx, device=self.device)
+
+    def coordinates(self):
+        """Row-major COO ``(rows, cols)`` of the whole stream on the GPU."""
+        rows = torch.empty(self.M * self.w * self.w, dtype=torch.int64, device=self.device)
+        cols = torch.empty_like(rows)
+        _lib.call("mhm_scatter_coordinates", _ptr(self.d_indices), self.w, self.M, _ptr(rows), _ptr(cols),
+                  _stream())
+        return rows, cols
+
+    def gather(self, blocks, loads):
+        """Global CSC values (pattern before zero elimination), rhs and load scale on the GPU."""
+        blocks = blocks.reshape(-1).contiguous()
+        loads = loads.reshape(-1).contiguous()
+        vals = torch.empty(len(self.csc_indices), dtype=torch.float64, device=self.device)
+        rhs = torch.empty(self.size, dtype=torch.float64, device=self.device)
+        scale = torch.empty_like(rhs)
+        _lib.call("mhm_ordered_gather", _ptr(self.vptr), _ptr(self.vcode), _ptr(blocks),
+                  int(len(self.csc_indices)), _ptr(vals), None, _stream())
+        _lib.call("mhm_ordered_gather", _ptr(self.lptr), _ptr(self.lcode), _ptr(loads), int(self.size),
+                  _ptr(rhs), _ptr(scale), _stream())
+        return vals, rhs, scale
+
+    def assemble(self, blocks, loads, boundary_load=None):
+        """``(matrix, rhs, load_scale)`` as ``assemble_hybrid_contributions`` returns them."""
+        vals, rhs, scale = self.gather(blocks, loads)
+        mat = sparse.csc_matrix((vals.cpu().numpy(), self.csc_indices.copy(), self.csc_indptr.copy()),
+                                shape=(self.size, self.size))
+        mat.eliminate_zeros()
+        rhs = rhs.cpu().numpy()
+        scale = scale.cpu().numpy()
+        if boundary_load is not None:
+            b = np.asarray(boundary_load, dtype=float)
+            if b.shape != (self.trace_size,) or not np.all(np.isfinite(b)):
+                raise ValueError("boundary_load must be finite with shape (trace_size,)")
+            rhs[:self.trace_size] -= b
+            scale[:self.trace_size] += np.abs(b)
+        return mat, rhs, scale

@@ -1,0 +1,290 @@
+"""Macro/local meshes, nodal numbering and trace coupling for the local stage.
+
+Mesh-only preparation done once per local shape on the host (the data the
+reference providers rebuild per cell in Python):
+
+* polygon macro topology and ear splitting: ``src/pymhm/meshes/polygonal.py:33-57,83-190``
+* conforming lattice refinement of the ear triangles: ``polygonal.py:217-265``
+* local triangle topology (face numbering by first encounter, orientation):
+  ``src/pymhm/meshes/triangle.py:28-84``
+* continuous P1/P2 numbering (vertices, then one node per face):
+  ``src/pymhm/fem/scalar/triangle.py:65-92``
+* trace coupling B = int_{dK} phi_i mu_j with quadrature split at fine
+  vertices and skeleton breaks: ``src/pymhm/fem/scalar/triangle.py:160-205``
+* skeleton numbering (face offsets, side order): ``src/pymhm/fem/traces/interval.py:119-161``
+
+Node order and numbering must match the reference exactly (integer maps are
+compared bit-for-bit in the tests).
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+from numpy.polynomial.legendre import leggauss
+
+from . import elements
+
+
+def _cross(a, b):
+    return a[..., 0] * b[..., 1] - a[..., 1] * b[..., 0]
+
+
+# ----------------------------------------------------------------- macro topology
+@dataclass
+class MacroMesh:
+    """Polygonal macro cells with signed faces (first-encounter face numbering)."""
+
+    points: np.ndarray
+    cells: list            # CCW vertex indices per cell
+    faces: np.ndarray      # (F, 2) as stored by the first adjacent cell
+    cell_faces: list       # per cell, face index per side
+    signs: list            # per cell, +1 first owner / -1 second
+
+    @property
+    def boundary_faces(self):
+        owners = np.zeros(len(self.faces), dtype=np.int64)
+        for cf in self.cell_faces:
+            np.add.at(owners, cf, 1)
+        return np.flatnonzero(owners == 1)
+
+
+def macro_topology(points, polygons) -> MacroMesh:
+    points = np.asarray(points, dtype=float)
+    cells, faces, cell_faces, signs = [], [], [], []
+    owner = {}
+    for poly in polygons:
+        idx = np.asarray(poly, dtype=np.int64)
+        xy = points[idx]
+        if np.sum(_cross(xy - xy[0], np.roll(xy, -1, axis=0) - xy[0])) < 0:
+            idx = idx[::-1].copy()
+        cf, sg = [], []
+        for a, b in zip(idx.tolist(), np.roll(idx, -1).tolist()):
+            key = (a, b) if a < b else (b, a)
+            face = owner.get(key)
+            if face is None:
+                owner[key] = face = len(faces)
+                faces.append((a, b))
+                sg.append(1)
+            else:
+                if faces[face] == (a, b):
+                    raise ValueError("inconsistent macro orientation")
+                sg.append(-1)
+            cf.append(face)
+        cells.append(idx)
+        cell_faces.append(np.asarray(cf, dtype=np.int64))
+        signs.append(np.asarray(sg, dtype=np.int64))
+    return MacroMesh(points, cells, np.asarray(faces, dtype=np.int64), cell_faces, signs)
+
+
+def square_grid(nx: int, ny: int | None = None) -> MacroMesh:
+    """Unit square split into ``nx * ny`` counterclockwise square macros (row-major)."""
+    ny = nx if ny is None else ny
+    xs, ys = np.meshgrid(np.arange(nx + 1) / nx, np.arange(ny + 1) / ny)
+    pts = np.stack((xs.ravel(), ys.ravel()), axis=1)
+    a = (np.arange(ny)[:, None] * (nx + 1) + np.arange(nx)[None, :]).ravel()
+    polys = np.stack((a, a + 1, a + nx + 2, a + nx + 1), axis=1)
+    return macro_topology(pts, polys)
+
+
+# ----------------------------------------------------------------- local meshes
+def ear_split(xy):
+    """Ear clipping of a simple counterclockwise polygon (first valid ear, repeatedly)."""
+    rest = list(range(len(xy)))
+    out = []
+    span = float(np.max(np.ptp(xy, axis=0)))
+    tol = 64 * np.finfo(float).eps * span * span
+    while len(rest) > 3:
+        for j in range(len(rest)):
+            a, b, c = rest[j - 1], rest[j], rest[(j + 1) % len(rest)]
+            if _cross(xy[b] - xy[a], xy[c] - xy[a]) <= tol:
+                continue
+            tri = xy[[a, b, c]]
+            others = np.asarray([v for v in rest if v not in (a, b, c)], dtype=np.int64)
+            if len(others):
+                edges = np.roll(tri, -1, axis=0) - tri
+                rel = xy[others][:, None, :] - tri[None, :, :]
+                inside = np.all(_cross(edges[None, :, :], rel) >= -tol, axis=1)
+                if inside.any():
+                    continue
+            out.append((a, b, c))
+            del rest[j]
+            break
+        else:
+            raise ValueError("polygon has no resolvable ear")
+    out.append(tuple(rest))
+    return np.asarray(out, dtype=np.int64)
+
+
+def refine_triangles(vertices, triangles, r: int):
+    """Conforming ``r``-lattice refinement, nodes shared through topological keys."""
+    keymap = {}
+    pts = []
+    cells = []
+    for tri in triangles:
+        tri = [int(v) for v in tri]
+        ids = {}
+        for i in range(r + 1):
+            for j in range(r + 1 - i):
+                w = (r - i - j, i, j)
+                key = tuple(sorted((tri[t], w[t]) for t in range(3) if w[t]))
+                node = keymap.get(key)
+                if node is None:
+                    keymap[key] = node = len(pts)
+                    pts.append(np.dot(np.asarray(w, dtype=float), vertices[tri]) / r)
+                ids[i, j] = node
+        for i in range(r):
+            for j in range(r - i):
+                cells.append((ids[i, j], ids[i + 1, j], ids[i, j + 1]))
+                if i + j < r - 1:
+                    cells.append((ids[i + 1, j], ids[i + 1, j + 1], ids[i, j + 1]))
+    return np.asarray(pts), np.asarray(cells, dtype=np.int64)
+
+
+def triangle_topology(cells):
+    """Faces by first encounter over (cell, side), side s = (v_s, v_{s+1})."""
+    faces, owner = [], {}
+    cell_faces = np.empty_like(cells)
+    count = []
+    for c, tri in enumerate(cells.tolist()):
+        for s in range(3):
+            a, b = tri[s], tri[(s + 1) % 3]
+            key = (a, b) if a < b else (b, a)
+            f = owner.get(key)
+            if f is None:
+                owner[key] = f = len(faces)
+                faces.append((a, b))
+                count.append(0)
+            count[f] += 1
+            cell_faces[c, s] = f
+    return np.asarray(faces, dtype=np.int64), cell_faces, np.asarray(count)
+
+
+@dataclass
+class LocalMesh:
+    """Local triangulation of one macro with its continuous nodal numbering."""
+
+    points: np.ndarray       # vertex coordinates (np, 2)
+    cells: np.ndarray        # (ne, 3)
+    faces: np.ndarray        # (nf, 2)
+    cell_faces: np.ndarray   # (ne, 3)
+    boundary: np.ndarray     # boundary face indices
+    degree: int
+    dofs: np.ndarray         # (ne, nb) local DOF ids
+    nodes: np.ndarray        # (n, 2) DOF coordinates
+
+    @property
+    def n(self) -> int:
+        return len(self.nodes)
+
+
+def local_mesh(points, cells, degree: int) -> LocalMesh:
+    pts = np.asarray(points, dtype=float)
+    cells = np.asarray(cells, dtype=np.int64).copy()
+    v = pts[cells]
+    det = _cross(v[:, 1] - v[:, 0], v[:, 2] - v[:, 0])
+    flip = det < 0
+    cells[flip] = cells[flip][:, [0, 2, 1]]
+    faces, cf, count = triangle_topology(cells)
+    nodes = [pts]
+    if degree == 1:
+        dofs = cells.copy()
+    elif degree == 2:
+        dofs = np.empty((len(cells), 6), dtype=np.int64)
+        dofs[:, :3] = cells
+        # one midpoint node per face, numbered after the vertices in face order
+        dofs[:, 3:] = len(pts) + cf
+        nodes.append(0.5 * (pts[faces[:, 0]] + pts[faces[:, 1]]))
+    else:
+        raise ValueError("degree must be 1 or 2")
+    return LocalMesh(pts, cells, faces, cf, np.flatnonzero(count == 1), degree, dofs,
+                     np.concatenate(nodes))
+
+
+# ----------------------------------------------------------------- trace coupling
+@dataclass(frozen=True)
+class FaceSpace:
+    """Discontinuous Legendre segments on a macroface parameterised by [0, 1]."""
+
+    breaks: tuple = (0.0, 1.0)
+    degrees: tuple = (0,)
+
+    @classmethod
+    def uniform(cls, degree: int = 0, segments: int = 1) -> "FaceSpace":
+        return cls(tuple(np.linspace(0.0, 1.0, segments + 1).tolist()), (int(degree),) * segments)
+
+    @property
+    def size(self) -> int:
+        return sum(p + 1 for p in self.degrees)
+
+    def evaluate(self, t):
+        t = np.asarray(t, dtype=float)
+        out = np.zeros((len(t), self.size))
+        col = 0
+        last = len(self.degrees) - 1
+        for s, p in enumerate(self.degrees):
+            lo, hi = self.breaks[s], self.breaks[s + 1]
+            sel = (t >= lo) & ((t < hi) | ((s == last) & (t <= hi)))
+            out[sel, col:col + p + 1] = elements.legendre_modes(2.0 * (t[sel] - lo) / (hi - lo) - 1.0, p)
+            col += p + 1
+        return out
+
+    def reversal(self):
+        """Signed column map of the same space under t -> 1 - t (uniform breaks only)."""
+        b = np.asarray(self.breaks)
+        if not np.allclose(b, 1.0 - b[::-1], rtol=0, atol=1e-15) or len(set(self.degrees)) != 1:
+            raise ValueError("orientation reversal needs symmetric breaks and equal degrees")
+        p = self.degrees[0]
+        nseg = len(self.degrees)
+        cmap = np.empty(self.size, dtype=np.int64)
+        scale = np.empty(self.size)
+        for s in range(nseg):
+            for mode in range(p + 1):
+                cmap[s * (p + 1) + mode] = (nseg - 1 - s) * (p + 1) + mode
+                scale[s * (p + 1) + mode] = -1.0 if mode % 2 else 1.0
+        return cmap, scale
+
+
+def side_coupling(mesh: LocalMesh, start, end, space: FaceSpace, sign: float = 1.0):
+    """``B[i, j] = sign * int_side phi_i mu_j`` with the side parameterised start -> end."""
+    start = np.asarray(start, dtype=float)
+    tangent = np.asarray(end, dtype=float) - start
+    length = float(np.sqrt(tangent @ tangent))
+    k = mesh.degree
+    gx, gw = leggauss(max(space.degrees) + k + 1)
+    out = np.zeros((mesh.n, space.size))
+    for f in mesh.boundary:
+        a, b = mesh.faces[f]
+        xy = mesh.points[[a, b]]
+        t = (xy - start) @ tangent / (length * length)
+        if (not np.allclose(xy, start + t[:, None] * tangent, atol=1e-12, rtol=1e-12)
+                or t.min() < -1e-12 or t.max() > 1 + 1e-12):
+            continue
+        lo, hi = np.clip(np.sort(t), 0.0, 1.0)
+        cuts = sorted({lo, hi, *(x for x in space.breaks if lo < x < hi)})
+        rows = [a, b] + ([len(mesh.points) + f] if k == 2 else [])
+        for left, right in zip(cuts[:-1], cuts[1:]):
+            par = left + (gx + 1.0) * (right - left) / 2.0
+            w = gw * (right - left) / 2.0 * length * sign
+            local = elements.interval_tables(k, (par - t[0]) / (t[1] - t[0]))
+            out[rows, :] += local.T @ (w[:, None] * space.evaluate(par))
+    return out
+
+
+def local_mass_ones(mesh: LocalMesh):
+    """``M 1`` (integral of each basis function), exact for P1/P2 on triangles."""
+    _, meas = elements.barycentric_gradients(mesh.points, mesh.cells)
+    out = np.zeros(mesh.n)
+    if mesh.degree == 1:
+        np.add.at(out, mesh.dofs.ravel(), np.repeat(meas / 3.0, 3))
+    else:
+        # P2 vertex functions integrate to 0, edge functions to |T|/3
+        np.add.at(out, mesh.dofs[:, 3:].ravel(), np.repeat(meas / 3.0, 3))
+    return out
+
+
+def quadrature_points(mesh: LocalMesh, order: int, offset=(0.0, 0.0)):
+    """Physical quadrature points ``(ne, q, 2)`` of the element rule used in assembly."""
+    bary, _ = elements.triangle_rule(order)
+    return np.einsum("qi,tij->tqj", bary, mesh.points[mesh.cells]) + np.asarray(offset)

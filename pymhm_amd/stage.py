@@ -1,0 +1,202 @@
+"""Resident batched local stage on one GPU (thin host wrapper of the C ABI).
+
+``LocalBatch`` owns one ``mhm_batch``: the band factors, the binary64
+responses and the work buffers of ``macros`` local problems of one
+``LocalLayout``.  Device arrays are PyTorch tensors used only as buffers;
+every numerical operation runs in ``libmhm_hip.so``.
+
+Mirrors ``BatchedFactorization`` (``src/pymhm/execution/cuda.py:165-288``):
+inputs are copied, the object stays on its construction device, ``close()``
+is idempotent and runs on exceptions, failures raise ``ValueError`` or
+``LinearSolveError`` and never fall back to another solver.
+"""
+
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import LinearSolveError
+
+PRECISIONS = {"fp64": _lib.MHM_FP64, "mixed": _lib.MHM_MIXED}
+DEFAULT_REFINE = {"fp64": 2, "mixed": 8}
+
+
+def _ptr(t):
+    return ctypes.c_void_p(0 if t is None else t.data_ptr())
+
+
+def _stream():
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+class LocalBatch:
+    """``macros`` local problems of one layout with resident factors."""
+
+    def __init__(self, layout, macros: int, *, precision: str = "fp64", max_columns: int = 0):
+        if precision not in PRECISIONS:
+            raise ValueError("precision must be 'fp64' or 'mixed'")
+        if not torch.cuda.is_available():
+            raise _lib.SolverUnavailableError("the HIP local stage requires a visible GPU")
+        self.layout = layout
+        self.macros = int(macros)
+        self.precision = precision
+        self.max_columns = int(max_columns)
+        self.device = torch.device("cuda", layout.device)
+        self.n, self.m, self.k = layout.n, layout.m, layout.k
+        self.N = self.n + self.k
+        self.R = 1 + self.m + self.k
+        self.width = self.m + self.k
+        handle = ctypes.c_void_p()
+        _lib.call("mhm_batch_create", layout.handle, self.macros, PRECISIONS[precision],
+                  self.max_columns, ctypes.byref(handle))
+        self.handle = handle
+        self._keep = []
+        self.blocks = None
+        self.loads = None
+
+    # -------------------------------------------------------------- inputs
+    def _dev(self, a, dtype):
+        t = torch.as_tensor(a, dtype=dtype, device=self.device).contiguous()
+        return t
+
+    def assemble(self, kq, fq=None, f_const: float = 0.0):
+        """Assemble from diffusion (and source) values at quadrature points ``(M, ne, nq)``."""
+        kq = self._dev(kq, torch.float64)
+        shape = (self.macros, self.layout.ne, self.layout.nq)
+        if tuple(kq.shape) != shape:
+            raise ValueError(f"kq must have shape {shape}")
+        if fq is not None:
+            fq = self._dev(fq, torch.float64)
+            if tuple(fq.shape) != shape:
+                raise ValueError(f"fq must have shape {shape}")
+        self._keep = [kq, fq]
+        _lib.call("mhm_batch_assemble", self.handle, _ptr(kq), _ptr(fq), float(f_const), _stream())
+
+    def set_operator(self, values, loads):
+        """Assembled operators: CSR values in layout slot order ``(M, nnz)``, loads ``(M, n)``."""
+        values = self._dev(values, torch.float64)
+        loads = self._dev(loads, torch.float64)
+        if tuple(values.shape) != (self.macros, self.layout.nnz) or tuple(loads.shape) != (self.macros, self.n):
+            raise ValueError("values/loads have the wrong shape")
+        self._keep = [values, loads]
+        _lib.call("mhm_batch_set_operator", self.handle, _ptr(values), _ptr(loads), _stream())
+
+    def set_coupling(self, colmap=None, colscale=None, couplings=None, constraints=None, kernels=None):
+        """Per-macro coupling column maps/scales or full couplings, constraints, kernels."""
+        M = self.macros
+        cm = None if colmap is None else self._dev(np.asarray(colmap).reshape(M, self.m), torch.int32)
+        cs = None if colscale is None else self._dev(np.asarray(colscale).reshape(M, self.m), torch.float64)
+        cp = None if couplings is None else self._dev(couplings, torch.float64).reshape(M, self.m, self.n)
+        cn = None if constraints is None else self._dev(constraints, torch.float64).reshape(M, self.n)
+        kn = None if kernels is None else self._dev(kernels, torch.float64).reshape(M, self.n)
+        if cm is not None and (int(cm.min()) < 0 or int(cm.max()) >= self.m):
+            raise ValueError("colmap entries outside the coupling columns")
+        _lib.call("mhm_batch_set_coupling", self.handle, _ptr(cm), _ptr(cs), _ptr(cp), _ptr(cn),
+                  _ptr(kn), _stream())
+        torch.cuda.current_stream().synchronize()
+
+    # -------------------------------------------------------------- numerics
+    def factor(self, check: bool = True):
+        _lib.call("mhm_batch_factor", self.handle, _stream())
+        if check:
+            self.raise_on_failure("factor")
+
+    def condense(self, max_refine: int | None = None, rtol: float = 1e-10, check: bool = True):
+        """Responses to [f, B, A Z] and condensed blocks ``(M, w, w)`` / loads ``(M, w)``."""
+        if max_refine is None:
+            max_refine = DEFAULT_REFINE[self.precision]
+        if self.blocks is None:
+            self.blocks = torch.empty((self.macros, self.width, self.width), dtype=torch.float64,
+                                      device=self.device)
+            self.loads = torch.empty((self.macros, self.width), dtype=torch.float64, device=self.device)
+        _lib.call("mhm_batch_condense", self.handle, int(max_refine), float(rtol), _ptr(self.blocks),
+                  _ptr(self.loads), _stream())
+        if check:
+            self.raise_on_failure("condense")
+        return self.blocks, self.loads
+
+    def responses(self):
+        """(source ``(M,n)``, lifts ``(M,n,m)``, retained ``(M,n)``) in original rows."""
+        src = torch.empty((self.macros, self.n), dtype=torch.float64, device=self.device)
+        lifts = torch.empty((self.macros, self.n, self.m), dtype=torch.float64, device=self.device)
+        ret = torch.empty((self.macros, self.n), dtype=torch.float64, device=self.device) if self.k else None
+        _lib.call("mhm_batch_responses", self.handle, _ptr(src), _ptr(lifts), _ptr(ret), _stream())
+        return src, lifts, ret
+
+    def solve(self, rhs, max_refine: int | None = None, rtol: float = 1e-10, check: bool = True):
+        """Solve ``(M, ncols, N)`` right-hand sides of the augmented system (original rows)."""
+        if max_refine is None:
+            max_refine = DEFAULT_REFINE[self.precision]
+        rhs = self._dev(rhs, torch.float64)
+        if rhs.dim() != 3 or rhs.shape[0] != self.macros or rhs.shape[2] != self.N:
+            raise ValueError(f"rhs must have shape (macros, ncols, {self.N})")
+        x = torch.empty_like(rhs)
+        _lib.call("mhm_batch_solve", self.handle, _ptr(rhs), _ptr(x), int(rhs.shape[1]), int(max_refine),
+                  float(rtol), _stream())
+        if check:
+            self.raise_on_failure("solve")
+        return x
+
+    def reconstruct(self, trace, coarse=None):
+        """Fields ``(M, cases, n)`` from local traces ``(M, cases, m)`` and amplitudes ``(M, cases, k)``."""
+        trace = self._dev(trace, torch.float64)
+        if trace.dim() == 2:
+            trace = trace[:, None, :]
+        cases = trace.shape[1]
+        co = None
+        if self.k:
+            co = self._dev(coarse, torch.float64).reshape(self.macros, cases, self.k)
+        fields = torch.empty((self.macros, cases, self.n), dtype=torch.float64, device=self.device)
+        _lib.call("mhm_batch_reconstruct", self.handle, _ptr(trace.contiguous()), _ptr(co), _ptr(fields),
+                  int(cases), _stream())
+        return fields
+
+    # -------------------------------------------------------------- status
+    def status(self):
+        out = np.zeros(self.macros, dtype=np.int32)
+        anyv = ctypes.c_int32()
+        _lib.call("mhm_batch_status", self.handle, out.ctypes.data_as(ctypes.c_void_p), ctypes.byref(anyv),
+                  _stream())
+        return out
+
+    def iterations(self):
+        out = np.zeros(self.macros, dtype=np.int32)
+        _lib.call("mhm_batch_iterations", self.handle, out.ctypes.data_as(ctypes.c_void_p), _stream())
+        return out
+
+    def raise_on_failure(self, what: str):
+        st = self.status()
+        if st.any():
+            bad = np.flatnonzero(st)
+            kinds = []
+            if np.any(st & _lib.STATUS_PIVOT):
+                kinds.append("singular or rank-deficient local operator")
+            if np.any(st & _lib.STATUS_NONFINITE):
+                kinds.append("nonfinite values")
+            if np.any(st & _lib.STATUS_RESIDUAL):
+                kinds.append("residual criterion failed")
+            raise LinearSolveError(f"{what}: {', '.join(kinds)} in macros {bad[:8].tolist()}"
+                                   + ("..." if len(bad) > 8 else ""))
+
+    def close(self):
+        if getattr(self, "handle", None) is not None and self.handle.value:
+            torch.cuda.synchronize(self.device)
+            _lib.load().mhm_batch_destroy(self.handle)
+            self.handle = ctypes.c_void_p()
+            self._keep = []
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,86 @@
+"""Synthetic workloads of the local stage and a one-call runner.
+
+* ``config1``: 8x8 unit-square macros, each ear-split and refined to 4x4x2 P1
+  triangles (25 local DOFs), piecewise-constant multipliers on 2 segments per
+  macroface (m = 8), K = 1, f = 2 pi^2 sin(pi x) sin(pi y), homogeneous
+  Dirichlet data (zero weak boundary load), quadrature order 4.
+* ``config2``: 64x64 macros of 16x16x2 P2 triangles (1089 DOFs), discontinuous
+  P1 multipliers on 4 segments per macroface (m = 32), log K a Gaussian random
+  field with exponential covariance (correlation length 0.05, variance 4,
+  fixed seed) sampled at the 16 quadrature points of every triangle, f = 1.
+
+Fields are generated here (seeded) because the reference holds no random
+field generator for these cases.
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import meshing
+from .problems import SquareGridProblem
+
+
+@dataclass
+class Case:
+    name: str
+    problem: SquareGridProblem
+    kq: np.ndarray            # (M, ne, nq)
+    fq: np.ndarray | None     # (M, ne, nq) or None
+    f_const: float = 0.0
+
+
+def gaussian_field(points, corr_len: float, variance: float, seed: int, grid: int = 1024):
+    """Stationary Gaussian field with covariance ``variance * exp(-r / corr_len)``.
+
+    Sampled exactly on a periodic ``2 grid x 2 grid`` lattice over [0, 2)^2 by
+    circulant embedding (negative embedding eigenvalues clipped to zero), then
+    bilinearly interpolated at ``points`` in [0, 1]^2.
+    """
+    n2 = 2 * grid
+    h = 1.0 / grid
+    d = np.minimum(np.arange(n2), n2 - np.arange(n2)) * h
+    r = np.sqrt(d[:, None] ** 2 + d[None, :] ** 2)
+    lam = np.fft.fft2(variance * np.exp(-r / corr_len)).real
+    lam = np.maximum(lam, 0.0)
+    rng = np.random.default_rng(seed)
+    xi = rng.standard_normal((n2, n2)) + 1j * rng.standard_normal((n2, n2))
+    field = np.fft.fft2(np.sqrt(lam / (n2 * n2)) * xi).real[: grid + 1, : grid + 1]
+    p = np.asarray(points, dtype=float).reshape(-1, 2)
+    x = np.clip(p[:, 0] / h, 0, grid - 1e-9)
+    y = np.clip(p[:, 1] / h, 0, grid - 1e-9)
+    i, j = np.floor(x).astype(np.int64), np.floor(y).astype(np.int64)
+    fx, fy = x - i, y - j
+    val = ((1 - fx) * (1 - fy) * field[i, j] + fx * (1 - fy) * field[i + 1, j]
+           + (1 - fx) * fy * field[i, j + 1] + fx * fy * field[i + 1, j + 1])
+    return val.reshape(np.shape(points)[:-1])
+
+
+def config1(nx: int = 8, device: int = 0) -> Case:
+    prob = SquareGridProblem(nx, 4, 1, meshing.FaceSpace.uniform(0, 2), order=4, device=device)
+    q = prob.quadrature_points()
+    kq = np.ones(q.shape[:3])
+    fq = 2 * np.pi ** 2 * np.sin(np.pi * q[..., 0]) * np.sin(np.pi * q[..., 1])
+    return Case("config1", prob, kq, fq)
+
+
+def config2(nx: int = 64, seed: int = 20261203, device: int = 0, refinement: int = 16) -> Case:
+    prob = SquareGridProblem(nx, refinement, 2, meshing.FaceSpace.uniform(1, 4), order=4, device=device)
+    q = prob.quadrature_points()
+    logk = gaussian_field(q, 0.05, 4.0, seed)
+    return Case("config2", prob, np.exp(logk), None, 1.0)
+
+
+def run_stage(case: Case, precision: str = "fp64", macros: slice | None = None):
+    """Assemble, factor, solve and condense on the GPU; return host arrays."""
+    sl = slice(None) if macros is None else macros
+    prob = case.problem
+    with prob.batch(precision, macros=sl) as batch:
+        batch.assemble(case.kq[sl], None if case.fq is None else case.fq[sl], case.f_const)
+        batch.factor()
+        blocks, loads = batch.condense()
+        src, lifts, ret = batch.responses()
+        return dict(blocks=blocks.cpu().numpy(), loads=loads.cpu().numpy(), source=src.cpu().numpy(),
+                    lifts=lifts.cpu().numpy(), retained=ret.cpu().numpy(), iterations=batch.iterations())
