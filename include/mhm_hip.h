@@ -204,6 +204,23 @@ int mhm_scatter_coordinates(const int64_t *d_indices, int32_t width, int32_t cou
 int mhm_ordered_gather(const int64_t *d_ptr, const int64_t *d_code, const double *d_src,
                        int64_t count, double *d_out, double *d_abs_out, void *stream);
 
+/* Kernel timing with HIP events recorded on the launch stream (no
+ * counterpart in the reference, whose benchmarks time phases with
+ * perf_counter wall clocks, benchmarks/execution_modes.py:227-258).
+ * Kernel ids: MHM_TIMER_ASSEMBLE, _FACTOR, _SOLVE, _RESIDUAL, _CONDENSE.
+ * mhm_timing_read synchronises on the recorded events. */
+enum {
+  MHM_TIMER_ASSEMBLE = 0,
+  MHM_TIMER_FACTOR = 1,
+  MHM_TIMER_SOLVE = 2,
+  MHM_TIMER_RESIDUAL = 3,
+  MHM_TIMER_CONDENSE = 4,
+  MHM_TIMER_COUNT = 5
+};
+int mhm_timing_enable(int32_t enable);
+int mhm_timing_reset(void);
+int mhm_timing_read(int32_t kernel, double *total_ms, int64_t *launches);
+
 #ifdef __cplusplus
 }
 #endif

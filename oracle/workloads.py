@@ -54,3 +54,44 @@ def oracle_config(case, macros=None):
     return dict(blocks=np.array(blocks), loads=np.array(loads), source=np.array(src),
                 lifts=np.array(lifts), retained=np.array(ret), correct_kernel=np.array(corr),
                 trace_size=trace_size, problems=problems)
+
+
+_MACRO_CACHE = {}
+
+
+def grid_spec(case, macros):
+    """Picklable description of selected macros of a square-grid case."""
+    prob = case.problem
+    macros = list(macros)
+    return dict(nx=prob.nx, refinement=prob.refinement, degree=prob.degree, order=prob.order,
+                breaks=tuple(prob.space.breaks), degrees=tuple(prob.space.degrees),
+                macros=macros, kq=case.kq[macros],
+                fq=None if case.fq is None else case.fq[macros], f_const=case.f_const)
+
+
+def condense_spec(spec):
+    """Assemble and condense the macros of ``grid_spec`` (one worker's share).
+
+    Same scope as the GPU stage: local mesh, operators, trace coupling,
+    augmented SuperLU solve and condensed block/load per macro.
+    """
+    key = (spec["nx"], spec["breaks"], spec["degrees"])
+    if key not in _MACRO_CACHE:   # macro mesh and skeleton are mesh-only: built once per worker
+        pm = PolygonMesh.unit_square_grid(spec["nx"])
+        space = FaceSpace(spec["breaks"], spec["degrees"])
+        _MACRO_CACHE[key] = (pm, SkeletonSpace(pm, [space] * len(pm.faces)))
+    pm, skel = _MACRO_CACHE[key]
+    blocks, loads = [], []
+    for r, c in enumerate(spec["macros"]):
+        fine = pm.submesh(c, spec["refinement"])
+        fq = None if spec["fq"] is None else spec["fq"][r]
+        A, M, f = scalar_operators(fine, spec["degree"], diffusion_q=spec["kq"][r], source_q=fq,
+                                   source=spec["f_const"], order=spec["order"])
+        B = trace_coupling(pm, c, fine, skel, spec["degree"])
+        Z = np.ones((A.shape[0], 1))
+        mz = M @ Z
+        p = LocalProblem(A, B, f, skel.cell_dofs(c), Z, mz / (Z.T @ mz).item())
+        _, blk, ld = p.condense().global_contribution([skel.size + c])
+        blocks.append(blk)
+        loads.append(ld)
+    return spec["macros"], np.array(blocks), np.array(loads)

@@ -25,7 +25,9 @@ EXPORTS = (
     "mhm_batch_set_coupling", "mhm_batch_factor", "mhm_batch_condense", "mhm_batch_responses",
     "mhm_batch_solve", "mhm_batch_reconstruct", "mhm_batch_status", "mhm_batch_iterations",
     "mhm_scatter_coordinates", "mhm_ordered_gather",
+    "mhm_timing_enable", "mhm_timing_reset", "mhm_timing_read",
 )
+TIMERS = {"assemble": 0, "factor": 1, "solve": 2, "residual": 3, "condense": 4}
 
 
 class LinearSolveError(RuntimeError):
@@ -88,6 +90,9 @@ def load(path: str | os.PathLike | None = None):
         "mhm_batch_iterations": (P, P, P),
         "mhm_scatter_coordinates": (P, I32, I32, P, P, P),
         "mhm_ordered_gather": (P, P, P, I64, P, P, P),
+        "mhm_timing_enable": (I32,),
+        "mhm_timing_reset": (),
+        "mhm_timing_read": (I32, P, P),
     }
     for name in EXPORTS:
         fn = getattr(lib, name)
@@ -120,3 +125,13 @@ def check(rc: int, what: str) -> None:
 
 def call(name: str, *args) -> None:
     check(getattr(load(), name)(*args), name)
+
+
+def kernel_times():
+    """``{name: (total_ms, launches)}`` of the event-timed kernels since the last reset."""
+    out = {}
+    for name, kid in TIMERS.items():
+        ms, n = ctypes.c_double(), ctypes.c_int64()
+        call("mhm_timing_read", kid, ctypes.byref(ms), ctypes.byref(n))
+        out[name] = (ms.value, n.value)
+    return out

@@ -118,6 +118,42 @@ __device__ double block_max(double v, double *sh) {
   return t;
 }
 
+// ----------------------------------------------------------------- event timing
+struct TimerState {
+  bool enabled = false;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> spans[MHM_TIMER_COUNT];
+  std::vector<hipEvent_t> pool;
+};
+static TimerState g_timer;
+
+static hipEvent_t timer_event() {
+  if (!g_timer.pool.empty()) {
+    hipEvent_t e = g_timer.pool.back();
+    g_timer.pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+// RAII span: records start/end events around the launches in its scope.
+struct TimerSpan {
+  int id;
+  hipStream_t st;
+  hipEvent_t a = nullptr;
+  TimerSpan(int kernel, hipStream_t stream) : id(kernel), st(stream) {
+    if (g_timer.enabled && (a = timer_event())) (void)hipEventRecord(a, st);
+  }
+  ~TimerSpan() {
+    if (!a) return;
+    hipEvent_t b = timer_event();
+    if (!b) return;
+    (void)hipEventRecord(b, st);
+    g_timer.spans[id].push_back({a, b});
+  }
+};
+
 // ----------------------------------------------------------------- kernel arguments
 struct LayoutK {
   int n, k, m, nnz, b, W, pin, ne, nb, dim, nq, ncr;
@@ -606,13 +642,20 @@ __global__ void __launch_bounds__(256)
       x[n] = mu;
     }
   }
+  // colnorm[3*col]: ||delta||^2 of the last correction, ||x||^2, previous ||delta||^2
   if (mode) {
     dn = block_sum<256>(dn, red);
     xn = block_sum<256>(xn, red);
     if (threadIdx.x == 0) {
-      colnorm[((size_t)mac * ncols + c) * 2] = dn;
-      colnorm[((size_t)mac * ncols + c) * 2 + 1] = xn;
+      double *cn = colnorm + ((size_t)mac * ncols + c) * 3;
+      cn[2] = cn[0];
+      cn[0] = dn;
+      cn[1] = xn;
     }
+  } else if (threadIdx.x == 0) {
+    double *cn = colnorm + ((size_t)mac * ncols + c) * 3;
+    cn[0] = cn[2] = INFINITY;
+    cn[1] = 0.0;
   }
 }
 
@@ -680,13 +723,10 @@ __global__ void __launch_bounds__(256)
   if (threadIdx.x == 0) {
     bool ok = sqrt(rr) <= rtol * sqrt(bb) && !anybad;
     if (ok && fwd_tol > 0.0) {
-      if (iteration == 0) {
-        ok = false;
-      } else {
-        const double dn = colnorm[((size_t)mac * ncols + c) * 2];
-        const double xn = colnorm[((size_t)mac * ncols + c) * 2 + 1];
-        ok = dn <= fwd_tol * fwd_tol * xn;
-      }
+      // forward convergence: the last correction is below fwd_tol ||x||, or it
+      // no longer halves (refinement has reached the binary64 noise floor)
+      const double *cn = colnorm + ((size_t)mac * ncols + c) * 3;
+      ok = cn[0] <= fwd_tol * fwd_tol * cn[1] || cn[0] >= 0.25 * cn[2];
     }
     if (check_only) {
       if (anybad) atomicOr(&B.status[mac], MHM_STATUS_NONFINITE);
@@ -1128,7 +1168,7 @@ int mhm_batch_create(const mhm_layout *L, int32_t macros, int32_t precision, int
       (rc = dev_alloc(b->owned, &b->xstd, M * R * N)) ||
       (rc = dev_alloc(b->owned, &b->rhsstd, M * R * N)) ||
       (rc = dev_alloc(b->owned, &b->beta, M * ncm)) ||
-      (rc = dev_alloc(b->owned, &b->colnorm, M * ncm * 2)) ||
+      (rc = dev_alloc(b->owned, &b->colnorm, M * ncm * 3)) ||
       (rc = dev_alloc(b->owned, &b->colstate, M * ncm)))
     return cleanup(rc);
   if (max_columns > 0 &&
@@ -1185,6 +1225,7 @@ int mhm_batch_assemble(mhm_batch *b, const double *kq, const double *fq, double 
         (rc = dev_alloc(b->owned, &b->fwork, (size_t)b->M * L.nb * L.ne)))
       return rc;
   }
+  TimerSpan span(MHM_TIMER_ASSEMBLE, st);
   int rc = MHM_OK;
   if (L.dim == 2 && L.nb == 3)
     rc = launch_assemble<3, 2>(b, kq, fq, f_const, st);
@@ -1273,6 +1314,8 @@ static int run_factor(mhm_batch *b, hipStream_t st) {
   LAUNCH_CHECK("k_prepare");
   const size_t shm = factor_lds(L.b, sizeof(T));
   const double tiny = (double)L.n * (sizeof(T) == 8 ? 2.220446049250313e-16 : 1.1920929e-07);
+  {
+  TimerSpan span(MHM_TIMER_FACTOR, st);
   if (L.b >= 12) {
     if (shm > 65536)
       HIP_TRY(hipFuncSetAttribute((const void *)k_factor<T, 256>,
@@ -1284,6 +1327,7 @@ static int run_factor(mhm_batch *b, hipStream_t st) {
                        b->bk.status, tiny);
   }
   LAUNCH_CHECK("k_factor");
+  }
   if (L.k) {
     hipLaunchKernelGGL((k_load_uw<T>), grid2(L.n, 256, b->M), dim3(256), 0, st, L, b->bk, (T *)b->y);
     LAUNCH_CHECK("k_load_uw");
@@ -1323,7 +1367,7 @@ static int run_solve(mhm_batch *b, const double *rhs, double *X, int ncols, int 
   const LayoutK &L = b->layout->k;
   const int NT = L.b >= 12 ? 256 : 64;
   const size_t sshm = solve_lds(L.b, ncols, sizeof(T));
-  const double fwd = b->precision == MHM_MIXED ? 1.0e-15 : 0.0;
+  const double fwd = b->precision == MHM_MIXED ? 1.0e-14 : 0.0;
   hipLaunchKernelGGL(k_fill_int, dim3((unsigned)(((size_t)b->M * ncols + 255) / 256)), dim3(256), 0, st,
                      b->colstate, 0, (size_t)b->M * ncols);
   hipLaunchKernelGGL(k_fill_int, dim3((b->M + 255) / 256), dim3(256), 0, st, b->bk.iters, 0, (size_t)b->M);
@@ -1332,18 +1376,22 @@ static int run_solve(mhm_batch *b, const double *rhs, double *X, int ncols, int 
   LAUNCH_CHECK("k_load_y");
   for (int it = 0; it <= max_refine; ++it) {
     if (it > 0) {
+      TimerSpan span(MHM_TIMER_RESIDUAL, st);
       hipLaunchKernelGGL((k_residual<T>), dim3(ncols, b->M), dim3(256), 0, st, L, b->bk, ncols, rhs,
                          (const double *)X, rtol, fwd, it - 1, (T *)b->y, b->beta, b->colstate,
                          (const double *)b->colnorm, 0);
       LAUNCH_CHECK("k_residual");
     }
-    if (NT == 256)
-      hipLaunchKernelGGL((k_solve<T, 256>), dim3(b->M), dim3(256), sshm, st, L.n, L.b, b->ldy, ncols,
-                         (const T *)b->band, (T *)b->y, (const int *)b->colstate, ncols, b->bk.iters);
-    else
-      hipLaunchKernelGGL((k_solve<T, 64>), dim3(b->M), dim3(64), sshm, st, L.n, L.b, b->ldy, ncols,
-                         (const T *)b->band, (T *)b->y, (const int *)b->colstate, ncols, b->bk.iters);
-    LAUNCH_CHECK("k_solve");
+    {
+      TimerSpan span(MHM_TIMER_SOLVE, st);
+      if (NT == 256)
+        hipLaunchKernelGGL((k_solve<T, 256>), dim3(b->M), dim3(256), sshm, st, L.n, L.b, b->ldy, ncols,
+                           (const T *)b->band, (T *)b->y, (const int *)b->colstate, ncols, b->bk.iters);
+      else
+        hipLaunchKernelGGL((k_solve<T, 64>), dim3(b->M), dim3(64), sshm, st, L.n, L.b, b->ldy, ncols,
+                           (const T *)b->band, (T *)b->y, (const int *)b->colstate, ncols, b->bk.iters);
+      LAUNCH_CHECK("k_solve");
+    }
     hipLaunchKernelGGL((k_saddle_apply<T>), dim3(ncols, b->M), dim3(256), 0, st, L, b->bk, ncols,
                        (const T *)b->y, (const double *)b->beta, X, it > 0 ? 1 : 0,
                        (const int *)b->colstate, b->colnorm);
@@ -1371,6 +1419,7 @@ int mhm_batch_condense(mhm_batch *b, int32_t max_refine, double rtol, double *bl
                ? run_solve<double>(b, b->rhsstd, b->xstd, b->R, max_refine, rtol, st)
                : run_solve<float>(b, b->rhsstd, b->xstd, b->R, max_refine, rtol, st);
   if (rc) return rc;
+  TimerSpan span(MHM_TIMER_CONDENSE, st);
   hipLaunchKernelGGL(k_condense, dim3(b->M), dim3(256), 0, st, L, b->bk, (const double *)b->xstd, blocks,
                      loads);
   LAUNCH_CHECK("k_condense");
@@ -1469,3 +1518,34 @@ int mhm_ordered_gather(const int64_t *ptr, const int64_t *code, const double *sr
   return MHM_OK;
 }
 
+
+int mhm_timing_enable(int32_t enable) {
+  g_timer.enabled = enable != 0;
+  return MHM_OK;
+}
+
+int mhm_timing_reset(void) {
+  for (auto &v : g_timer.spans) {
+    for (auto &p : v) {
+      g_timer.pool.push_back(p.first);
+      g_timer.pool.push_back(p.second);
+    }
+    v.clear();
+  }
+  return MHM_OK;
+}
+
+int mhm_timing_read(int32_t kernel, double *total_ms, int64_t *launches) {
+  if (kernel < 0 || kernel >= MHM_TIMER_COUNT || !total_ms || !launches)
+    return fail(MHM_EINVAL, "bad timing arguments");
+  double total = 0.0;
+  for (auto &p : g_timer.spans[kernel]) {
+    HIP_TRY(hipEventSynchronize(p.second));
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, p.first, p.second));
+    total += ms;
+  }
+  *total_ms = total;
+  *launches = (int64_t)g_timer.spans[kernel].size();
+  return MHM_OK;
+}
