@@ -130,6 +130,13 @@ int mhm_batch_assemble(mhm_batch *batch, const double *d_kq, const double *d_fq,
 int mhm_batch_set_operator(mhm_batch *batch, const double *d_values, const double *d_loads,
                            void *stream);
 
+/* Copy the assembled operators out: d_values[macro][nnz] (layout slot
+ * order), d_loads[macro][n] and d_az[macro][n] (A Z), ORIGINAL rows; any may
+ * be NULL.  Mirror of the (matrix, load, _retained_action) fields of
+ * LocalProblem (core/contracts.py:92-211) for inspection and tests. */
+int mhm_batch_operator(mhm_batch *batch, double *d_values, double *d_loads, double *d_az,
+                       void *stream);
+
 /* Per-macro coupling data (all optional, NULL keeps the layout default):
  * d_colmap[macro][m] (int32) and d_colscale[macro][m] give column a of the
  * macro's coupling as colscale[a] * (layout coupling column colmap[a]) - a

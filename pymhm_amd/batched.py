@@ -70,7 +70,9 @@ def _pattern(p):
     a = sparse.csr_matrix(p.matrix, dtype=float, copy=True)
     a.sum_duplicates()
     a.sort_indices()
-    sym = (a + a.T).tocsr()       # structurally symmetric pattern
+    ones = a.copy()
+    ones.data[:] = 1.0            # stored zeros stay in the pattern
+    sym = (ones + ones.T).tocsr()  # structurally symmetric pattern
     sym.sort_indices()
     return a, sym.indptr.astype(np.int64), sym.indices.astype(np.int64)
 

@@ -193,13 +193,31 @@ __device__ __forceinline__ double kernel_at(const LayoutK &L, const BatchK &B, i
 }
 
 // ----------------------------------------------------------------- assembly
-// One thread per element: element stiffness (upper triangle in registers) and
-// load from K and f at the element's quadrature points.  Element matrices are
+// Neumaier step of fem/scalar/_integration.py:32-40 (magnitude-aware correction).
+__device__ __forceinline__ void neumaier(double &total, double &corr, double value) {
+#pragma clang fp contract(off)
+  const double combined = total + value;
+  if (fabs(total) >= fabs(value))
+    corr += (total - combined) + value;
+  else
+    corr += (value - combined) + total;
+  total = combined;
+}
+
+// One thread per (element, basis row i): row i of the element stiffness and
+// load entry i.  The arithmetic repeats the reference Gram kernel operation
+// for operation - physical gradients as the sequential sum
+// sum_i dphi/dlambda_i grad(lambda_i), K grad(phi_j) and the Cartesian
+// contraction compensated, ((c + corr) * w_q) * |T|, compensated quadrature
+// sum (fem/scalar/_integration.py:43-88) - with floating contraction off, so
+// the element matrices match a binary64 CPU evaluation bit for bit.  The load
+// is sum_q ((w_q phi_i) f_q) |T| in quadrature order.  Element matrices are
 // stored code-major, ework[mac][i*NB+j][e], so the stores coalesce.
 template <int NB, int D>
 __global__ void __launch_bounds__(128)
     k_assemble(LayoutK L, const double *__restrict__ kq, const double *__restrict__ fq,
                double fconst, double *__restrict__ ework, double *__restrict__ fwork) {
+#pragma clang fp contract(off)
   extern __shared__ double sh[];
   const int nq = L.nq;
   double *sd = sh;
@@ -209,21 +227,20 @@ __global__ void __launch_bounds__(128)
   for (int t = threadIdx.x; t < nq * NB; t += blockDim.x) sv[t] = L.vtab[t];
   for (int t = threadIdx.x; t < nq; t += blockDim.x) sw[t] = L.qw[t];
   __syncthreads();
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  const int tid = blockIdx.x * blockDim.x + threadIdx.x;
   const int mac = blockIdx.y;
-  if (e >= L.ne) return;
+  if (tid >= L.ne * NB) return;
+  const int e = tid / NB, i = tid - e * NB;
   double gl[D + 1][D];
 #pragma unroll
-  for (int i = 0; i <= D; ++i)
+  for (int l = 0; l <= D; ++l)
 #pragma unroll
-    for (int a = 0; a < D; ++a) gl[i][a] = L.glam[((size_t)e * (D + 1) + i) * D + a];
+    for (int a = 0; a < D; ++a) gl[l][a] = L.glam[((size_t)e * (D + 1) + l) * D + a];
   const double meas = L.measure[e];
-  double K[NB * (NB + 1) / 2];
-  double F[NB];
+  double tot[NB], cor[NB];
 #pragma unroll
-  for (int t = 0; t < NB * (NB + 1) / 2; ++t) K[t] = 0.0;
-#pragma unroll
-  for (int t = 0; t < NB; ++t) F[t] = 0.0;
+  for (int j = 0; j < NB; ++j) tot[j] = cor[j] = 0.0;
+  double F = 0.0;
   const double *kp = kq + ((size_t)mac * L.ne + e) * nq;
   const double *fp = fq ? fq + ((size_t)mac * L.ne + e) * nq : nullptr;
   for (int q = 0; q < nq; ++q) {
@@ -232,39 +249,28 @@ __global__ void __launch_bounds__(128)
     for (int b = 0; b < NB; ++b)
 #pragma unroll
       for (int a = 0; a < D; ++a) {
-        double s = 0.0;
+        double acc = sd[(q * NB + b) * (D + 1)] * gl[0][a];
 #pragma unroll
-        for (int i = 0; i <= D; ++i) s += sd[(q * NB + b) * (D + 1) + i] * gl[i][a];
-        g[b][a] = s;
+        for (int l = 1; l <= D; ++l) acc = acc + sd[(q * NB + b) * (D + 1) + l] * gl[l][a];
+        g[b][a] = acc;
       }
-    const double kw = sw[q] * kp[q] * meas;
-    int t = 0;
+    const double k = kp[q];
+    const double w = sw[q];
 #pragma unroll
-    for (int i = 0; i < NB; ++i)
+    for (int j = 0; j < NB; ++j) {
+      // K grad(phi_j) for K = k I: the compensated sum over b of K[a][b] g[j][b]
+      // reduces to the single rounded product k * g[j][a]
+      double ct = 0.0, cc = 0.0;
 #pragma unroll
-      for (int j = i; j < NB; ++j) {
-        double c = 0.0;
-#pragma unroll
-        for (int a = 0; a < D; ++a) c += g[i][a] * g[j][a];
-        K[t++] += kw * c;
-      }
-    const double fw = sw[q] * (fp ? fp[q] : fconst) * meas;
-#pragma unroll
-    for (int b = 0; b < NB; ++b) F[b] += fw * sv[q * NB + b];
+      for (int a = 0; a < D; ++a) neumaier(ct, cc, g[i][a] * (k * g[j][a]));
+      neumaier(tot[j], cor[j], ((ct + cc) * w) * meas);
+    }
+    F = F + ((w * sv[q * NB + i]) * (fp ? fp[q] : fconst)) * meas;
   }
   const size_t base = (size_t)mac * NB * NB * L.ne;
-  int t = 0;
 #pragma unroll
-  for (int i = 0; i < NB; ++i)
-#pragma unroll
-    for (int j = i; j < NB; ++j) {
-      const double v = K[t++];
-      ework[base + (size_t)(i * NB + j) * L.ne + e] = v;
-      if (j != i) ework[base + (size_t)(j * NB + i) * L.ne + e] = v;
-    }
-  const size_t fb = (size_t)mac * NB * L.ne;
-#pragma unroll
-  for (int b = 0; b < NB; ++b) fwork[fb + (size_t)b * L.ne + e] = F[b];
+  for (int j = 0; j < NB; ++j) ework[base + (size_t)(i * NB + j) * L.ne + e] = tot[j] + cor[j];
+  fwork[(size_t)mac * NB * L.ne + (size_t)i * L.ne + e] = F;
 }
 
 // One thread per band row: ordered gather of the element contributions of each
@@ -279,8 +285,11 @@ __global__ void __launch_bounds__(128)
   const double *ew = ework + (size_t)mac * L.nb * L.nb * L.ne;
   double hi = 0.0, lo = 0.0;
   for (int s = L.rowptr[i]; s < L.rowptr[i + 1]; ++s) {
+    // stored order = the reference's duplicate-summation order (first value, then +=)
     double v = 0.0;
-    for (int t = L.slot_ptr[s]; t < L.slot_ptr[s + 1]; ++t) v += ew[L.slot_code[t]];
+    const int t0 = L.slot_ptr[s], t1 = L.slot_ptr[s + 1];
+    if (t1 > t0) v = ew[L.slot_code[t0]];
+    for (int t = t0 + 1; t < t1; ++t) v += ew[L.slot_code[t]];
     B.aval[(size_t)mac * L.nnz + s] = v;
     dd_fma(hi, lo, v, kernel_at(L, B, mac, L.cols[s]));
   }
@@ -1009,8 +1018,8 @@ int mhm_layout_create(int32_t device, const mhm_layout_desc *d, mhm_layout **out
   if (d->n <= 0 || d->m < 0 || d->k < 0 || d->k > 1 || d->nnz <= 0 || d->bandwidth < 0 ||
       d->bandwidth >= d->n + 1 || !d->perm || !d->rowptr || !d->cols)
     return fail(MHM_EINVAL, "layout sizes or pattern arrays are invalid");
-  if (d->k && (d->pin < 0 || d->pin >= d->n || !d->constraint))
-    return fail(MHM_EINVAL, "k == 1 requires a pin row and a constraint vector");
+  if (d->k && (d->pin < 0 || d->pin >= d->n))
+    return fail(MHM_EINVAL, "k == 1 requires a pin row");
   if (d->bandwidth >= 65535) return fail(MHM_EINVAL, "bandwidth exceeds 65534");
   // host validation of the pattern: sorted rows, in range, symmetric bandwidth
   std::vector<int> diag(d->n, -1);
@@ -1207,7 +1216,7 @@ static int launch_assemble(const mhm_batch *b, const double *kq, const double *f
                            hipStream_t st) {
   const LayoutK &L = b->layout->k;
   const size_t shm = ((size_t)L.nq * NB * (D + 1) + (size_t)L.nq * NB + L.nq) * sizeof(double);
-  hipLaunchKernelGGL((k_assemble<NB, D>), grid2(L.ne, 128, b->M), dim3(128), shm, st, L, kq, fq, fc,
+  hipLaunchKernelGGL((k_assemble<NB, D>), grid2((size_t)L.ne * NB, 128, b->M), dim3(128), shm, st, L, kq, fq, fc,
                      b->ework, b->fwork);
   LAUNCH_CHECK("k_assemble");
   return MHM_OK;
@@ -1252,6 +1261,36 @@ int mhm_batch_set_operator(mhm_batch *b, const double *values, const double *loa
   hipLaunchKernelGGL(k_rowsum, grid2(L.n, 128, b->M), dim3(128), 0, st, L, b->bk, loads);
   LAUNCH_CHECK("k_rowsum");
   b->factored = b->condensed = false;
+  return MHM_OK;
+}
+
+// Band rows -> original rows of a per-macro [macro][n] vector.
+__global__ void __launch_bounds__(256)
+    k_unpermute_vec(const int *__restrict__ perm, int n, const double *__restrict__ in,
+                    double *__restrict__ out) {
+  const int mac = blockIdx.y;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  out[(size_t)mac * n + perm[i]] = in[(size_t)mac * n + i];
+}
+
+int mhm_batch_operator(mhm_batch *b, double *values, double *loads, double *az, void *stream) {
+  if (!b) return fail(MHM_EINVAL, "batch is NULL");
+  const LayoutK &L = b->layout->k;
+  DeviceGuard guard(b->layout->device);
+  hipStream_t st = (hipStream_t)stream;
+  if (values)
+    HIP_TRY(hipMemcpyAsync(values, b->bk.aval, (size_t)b->M * L.nnz * sizeof(double), hipMemcpyDeviceToDevice, st));
+  if (loads) {
+    hipLaunchKernelGGL(k_unpermute_vec, grid2(L.n, 256, b->M), dim3(256), 0, st, L.perm, L.n,
+                       (const double *)b->bk.load, loads);
+    LAUNCH_CHECK("k_unpermute_vec");
+  }
+  if (az) {
+    hipLaunchKernelGGL(k_unpermute_vec, grid2(L.n, 256, b->M), dim3(256), 0, st, L.perm, L.n,
+                       (const double *)b->bk.az, az);
+    LAUNCH_CHECK("k_unpermute_vec");
+  }
   return MHM_OK;
 }
 

@@ -18,6 +18,7 @@ import numpy as np
 from numpy.polynomial.legendre import leggauss
 
 TRIANGLE_EDGES = ((0, 1), (1, 2), (2, 0))
+TETRA_EDGES = ((0, 1), (1, 2), (2, 0), (0, 3), (1, 3), (2, 3))  # placeholder order, fixed below
 
 
 def triangle_rule(order: int):
@@ -32,24 +33,71 @@ def triangle_rule(order: int):
     return bary, weights
 
 
-def triangle_tables(degree: int, bary):
-    """P1/P2 values ``(q, nb)`` and ``dphi/dlambda`` ``(q, nb, 3)``."""
+def lattice_nodes(degree: int, dim: int = 2):
+    """Barycentric multi-indices: vertices, then edge nodes along each oriented edge.
+
+    Triangle edges (0,1), (1,2), (2,0) as ``src/pymhm/fem/scalar/triangle.py:14-24``;
+    interior nodes (degree >= 3) are not needed for P1/P2.
+    """
+    k = int(degree)
+    edges = TRIANGLE_EDGES if dim == 2 else TETRA_EDGES
+    nodes = []
+    for v in range(dim + 1):
+        a = [0] * (dim + 1)
+        a[v] = k
+        nodes.append(tuple(a))
+    for a_, b_ in edges:
+        for j in range(1, k):
+            a = [0] * (dim + 1)
+            a[a_], a[b_] = k - j, j
+            nodes.append(tuple(a))
+    if k > 2:
+        raise ValueError("tables support degree 1 or 2")
+    return np.asarray(nodes, dtype=np.int64)
+
+
+def lagrange_tables(degree: int, bary, nodes):
+    """Equispaced Lagrange values ``(q, nb)`` and ``dphi/dlambda`` ``(q, nb, d+1)``.
+
+    Textbook product form ``phi_a = prod_i prod_{m < a_i} (k lambda_i - m) / (m + 1)``
+    with the product rule, evaluated factor by factor (the same sequence of
+    operations as the oracle's tabulation, so the tables agree bit for bit).
+    """
+    k = int(degree)
     lam = np.asarray(bary, dtype=float)
-    q = len(lam)
-    if degree == 1:
-        return lam.copy(), np.broadcast_to(np.eye(3), (q, 3, 3)).copy()
-    if degree != 2:
+    q, d1 = lam.shape
+    nb = len(nodes)
+    parts = np.empty((d1, q, nb))
+    dparts = np.empty((d1, q, nb))
+    for i in range(d1):
+        kl = k * lam[:, i]
+        for b, a in enumerate(nodes):
+            val = np.ones(q)
+            der = np.zeros(q)
+            for m in range(int(a[i])):
+                f = (kl - m) / (m + 1)
+                der = der * f + val / (m + 1)
+                val = val * f
+            parts[i, :, b] = val
+            dparts[i, :, b] = der * k
+    vals = np.ones((q, nb))
+    for i in range(d1):
+        vals = vals * parts[i]
+    ders = np.empty((q, nb, d1))
+    for i in range(d1):
+        prod = dparts[i].copy()
+        for j in range(d1):
+            if j != i:
+                prod = prod * parts[j]
+        ders[:, :, i] = prod
+    return vals, ders
+
+
+def triangle_tables(degree: int, bary):
+    """P1/P2 triangle values ``(q, nb)`` and ``dphi/dlambda`` ``(q, nb, 3)``."""
+    if degree not in (1, 2):
         raise ValueError("triangle tables support degree 1 or 2")
-    vals = np.empty((q, 6))
-    der = np.zeros((q, 6, 3))
-    for i in range(3):
-        vals[:, i] = lam[:, i] * (2.0 * lam[:, i] - 1.0)
-        der[:, i, i] = 4.0 * lam[:, i] - 1.0
-    for e, (a, b) in enumerate(TRIANGLE_EDGES):
-        vals[:, 3 + e] = 4.0 * lam[:, a] * lam[:, b]
-        der[:, 3 + e, a] = 4.0 * lam[:, b]
-        der[:, 3 + e, b] = 4.0 * lam[:, a]
-    return vals, der
+    return lagrange_tables(degree, bary, lattice_nodes(degree, 2))
 
 
 def interval_tables(degree: int, s):
@@ -74,14 +122,20 @@ def legendre_modes(x, degree: int):
 
 
 def barycentric_gradients(points, cells):
-    """Physical gradients of the barycentric coordinates ``(ne, d+1, d)`` and measures."""
+    """Physical gradients of the barycentric coordinates ``(ne, d+1, d)`` and measures.
+
+    Gradients are rows 1.. of ``inv([1 x])`` per simplex and triangle areas the
+    halved cross product, the operations of ``fem/scalar/operators.py:72-77`` and
+    ``meshes/triangle.py:115-120``, so affine geometry enters assembly with the
+    reference's rounding.
+    """
     v = np.asarray(points, dtype=float)[cells]
     d = v.shape[2]
-    jac = (v[:, 1:, :] - v[:, :1, :]).transpose(0, 2, 1)  # columns are edge vectors
-    inv = np.linalg.inv(jac)  # rows: gradients of lambda_1..lambda_d
-    grads = np.empty((len(v), d + 1, d))
-    grads[:, 1:, :] = inv
-    grads[:, 0, :] = -inv.sum(axis=1)
-    det = np.linalg.det(jac)
-    measure = np.abs(det) / (2.0 if d == 2 else 6.0)
+    m = np.concatenate((np.ones(v.shape[:2] + (1,)), v), axis=2)
+    grads = np.linalg.inv(m)[:, 1:, :].swapaxes(1, 2)
+    if d == 2:
+        a, b = v[:, 1] - v[:, 0], v[:, 2] - v[:, 0]
+        measure = (a[:, 0] * b[:, 1] - a[:, 1] * b[:, 0]) / 2
+    else:
+        measure = np.abs(np.linalg.det(v[:, 1:] - v[:, :1])) / 6.0
     return grads, measure

@@ -37,6 +37,31 @@ def element_pattern(n: int, edofs):
     return mat.indptr.astype(np.int64), mat.indices.astype(np.int64)
 
 
+def summation_order(n: int, edofs):
+    """Order in which SciPy's COO -> CSC conversion sums duplicate element entries.
+
+    The reference assembles ``coo_matrix((blocks.ravel(), (repeat(dofs), tile(dofs)))).tocsc()``
+    (``fem/scalar/triangle.py:250-254``): entries are placed column by column in
+    stream order, each column's row indices are sorted (an unstable sort whose
+    result depends only on the index sequence) and equal neighbours are summed
+    left to right.  Running SciPy's own conversion on entry ids recovers that
+    order, so the GPU gather adds every slot's contributions in the same order.
+    Returns the COO stream positions (``e*nb*nb + i*nb + j``) in summation order.
+    """
+    edofs = np.asarray(edofs, dtype=np.int64)
+    nb = edofs.shape[1]
+    rows = np.repeat(edofs, nb, axis=1).ravel()
+    cols = np.tile(edofs, (1, nb)).ravel()
+    first = np.argsort(cols, kind="stable")          # column placement keeps stream order
+    counts = np.bincount(cols, minlength=n)
+    indptr = np.r_[0, np.cumsum(counts)].astype(np.int64)
+    ids = first.astype(np.float64)                   # exact below 2**53
+    csc = sparse.csc_matrix((ids, rows[first].copy(), indptr), shape=(n, n))
+    csc.has_sorted_indices = False
+    csc.sort_indices()
+    return csc.data.astype(np.int64)
+
+
 def bandwidth(perm, indptr, indices) -> int:
     inv = np.empty_like(perm)
     inv[perm] = np.arange(len(perm))
@@ -172,7 +197,13 @@ class LocalLayout:
         ij = np.arange(nb * nb)
         code = (ij[None, :] * ne + np.arange(ne)[:, None]).ravel()   # (i*nb+j)*ne + e
         flat = slot.ravel()
-        order = np.lexsort((np.repeat(np.arange(ne), nb * nb), flat))  # by slot, then element
+        order = summation_order(self.n, edofs)
+        flat_sorted = flat[order]
+        # contributions of one slot must be consecutive in the summation order
+        starts = np.r_[True, flat_sorted[1:] != flat_sorted[:-1]]
+        if len(np.unique(flat_sorted[starts])) != starts.sum():
+            raise ValueError("summation order does not group the slots")
+        order = order[np.argsort(flat_sorted, kind="stable")]
         slot_ptr = np.zeros(self.nnz + 1, dtype=np.int64)
         np.add.at(slot_ptr, flat + 1, 1)
         slot_ptr = np.cumsum(slot_ptr)

@@ -85,6 +85,14 @@ class LocalBatch:
         self._keep = [values, loads]
         _lib.call("mhm_batch_set_operator", self.handle, _ptr(values), _ptr(loads), _stream())
 
+    def operator(self):
+        """Assembled ``(values (M, nnz) slot order, loads (M, n), A Z (M, n))``."""
+        vals = torch.empty((self.macros, self.layout.nnz), dtype=torch.float64, device=self.device)
+        loads = torch.empty((self.macros, self.n), dtype=torch.float64, device=self.device)
+        az = torch.empty((self.macros, self.n), dtype=torch.float64, device=self.device)
+        _lib.call("mhm_batch_operator", self.handle, _ptr(vals), _ptr(loads), _ptr(az), _stream())
+        return vals, loads, az
+
     def set_coupling(self, colmap=None, colscale=None, couplings=None, constraints=None, kernels=None):
         """Per-macro coupling column maps/scales or full couplings, constraints, kernels."""
         M = self.macros
