@@ -213,6 +213,7 @@ def main(argv=None):
         "rhs_solves_per_s": value * (1 + prob.m + 1),
         "refinement_solves_per_macro": {p: float(res[p]["iters"].mean()) for p in res},
         "kernel_ms_per_step": {k: v[0] / args.steps for k, v in times.items()},
+        "fp64_kernel_ms_per_step": {k: v[0] / args.steps for k, v in res["fp64"]["times"].items()},
         "roofline": {
             "kernel": "k_factor<float>" if dominant == "factor" else "k_solve<float>",
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -118,6 +118,16 @@ __device__ double block_max(double v, double *sh) {
   return t;
 }
 
+// Workgroup barrier that orders LDS only.  __syncthreads() also releases and
+// acquires global memory, which waits for every outstanding vector-memory
+// operation - including prefetch loads meant to stay in flight across rows.
+// Used inside the band kernels' row loops, where no thread reads global data
+// written by another thread of the block; full __syncthreads() remains
+// wherever it does.
+__device__ __forceinline__ void lds_barrier() {
+  __asm__ volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
+
 // ----------------------------------------------------------------- event timing
 struct TimerState {
   bool enabled = false;
@@ -218,7 +228,7 @@ __global__ void __launch_bounds__(128)
     k_assemble(LayoutK L, const double *__restrict__ kq, const double *__restrict__ fq,
                double fconst, double *__restrict__ ework, double *__restrict__ fwork) {
 #pragma clang fp contract(off)
-  extern __shared__ double sh[];
+  extern __shared__ __attribute__((aligned(16))) double sh[];
   const int nq = L.nq;
   double *sd = sh;
   double *sv = sd + nq * NB * (D + 1);
@@ -370,24 +380,36 @@ __global__ void __launch_bounds__(256) k_prepare(LayoutK L, BatchK B, T *__restr
 
 // ----------------------------------------------------------------- band Cholesky
 // Block per macro.  The lower band of column j is band[j*W + r] = L(j+r, j).
-// Window slot (col % W) holds the b+1 band entries of one active column; the
-// new column j+W enters the slot freed by column j.  Two barriers per column.
-// pairs[t] = (cc << 16) | r enumerates the rank-1 update triangle.
+// Window slot (col % W) holds the b+1 band entries of one active column.  All
+// global traffic happens at chunk boundaries (FACTOR_CHUNK columns): the
+// columns entering during chunk q+1 are loaded into registers at the start of
+// chunk q and staged in a double-buffered LDS area at its end; a column's
+// stage slot is reused for its factor column, and the chunk's factor columns
+// are written out (coalesced) at the start of the next chunk.  Inside a chunk
+// each column costs two LDS-only barriers.  pairs[t] = (cc << 16) | r
+// enumerates the rank-1 update triangle.
+#define FACTOR_CHUNK 8
+
 template <typename T, int NT>
 __global__ void __launch_bounds__(NT)
     k_factor(int n, int b, T *__restrict__ band, int *__restrict__ status, double tiny) {
-  extern __shared__ unsigned char smem_raw[];
+  constexpr int CH = FACTOR_CHUNK;
+  constexpr int MAXR = 8;  // per <= NT * MAXR because W <= NT
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
   const int W = b + 1;
   const int npairs = b * (b + 1) / 2;
+  const int per = CH * W;
+  const int nreg = (per + NT - 1) / NT;
   T *win = reinterpret_cast<T *>(smem_raw);
   T *lc = win + (size_t)W * W;
-  unsigned *pairs = reinterpret_cast<unsigned *>(lc + W);
+  T *stage = lc + W;  // 2 * per
+  unsigned *pairs = reinterpret_cast<unsigned *>(stage + 2 * per);
+  __shared__ int bad;
   const int mac = blockIdx.x;
   const int tid = threadIdx.x;
   T *Lg = band + (size_t)mac * n * W;
   for (int t = tid; t < npairs; t += NT) {
-    // decode t -> (cc, r): cc = 1..b, r = 0..b-cc, column-offset major
-    int cc = 1, rem = t;
+    int cc = 1, rem = t;  // t -> (cc = 1..b, r = 0..b-cc), column-offset major
     while (rem >= b - cc + 1) {
       rem -= b - cc + 1;
       ++cc;
@@ -398,158 +420,283 @@ __global__ void __launch_bounds__(NT)
     const int c = t / W, r = t - c * W;
     win[t] = c < n ? Lg[(size_t)c * W + r] : T(0);
   }
-  __shared__ int bad;
+  for (int e = tid; e < per; e += NT) {  // stage chunk 0: columns W .. W+CH-1
+    const int col = W + e / W;
+    stage[e] = col < n ? Lg[(size_t)col * W + (e % W)] : T(0);
+  }
   if (tid == 0) bad = 0;
+  T pf[MAXR];
   __syncthreads();
   int s = 0;  // slot of column j
-  for (int j = 0; j < n; ++j) {
-    if (tid < W) {
-      T d = win[s * W];
-      if (!(d > T(tiny)) || !isfinite((double)d)) {
-        if (tid == 0) bad = MHM_STATUS_PIVOT;
-        d = T(1);
+  const int nchunks = (n + CH - 1) / CH;
+  for (int q = 0; q < nchunks; ++q) {
+    T *sb = stage + (q & 1) * per;
+    if (q > 0) {
+      // factor columns of chunk q-1 (held in the other stage buffer) -> HBM
+      const T *src = stage + ((q - 1) & 1) * per;
+      const size_t base = (size_t)(q - 1) * CH * W;
+      for (int e = tid; e < per; e += NT) Lg[base + e] = src[e];
+    }
+    // columns entering during chunk q+1 (j + W for j in chunk q+1) -> registers
+#pragma unroll
+    for (int v = 0; v < MAXR; ++v) {
+      const int e = tid + v * NT;
+      const int col = (q + 1) * CH + W + e / W;
+      pf[v] = (v < nreg && e < per && col < n) ? Lg[(size_t)col * W + (e % W)] : T(0);
+    }
+    const int jend = min(n, (q + 1) * CH);
+    for (int j = q * CH; j < jend; ++j) {
+      const int jj = j - q * CH;
+      if (tid < W) {
+        T d = win[s * W];
+        if (!(d > T(tiny)) || !isfinite((double)d)) {
+          if (tid == 0) bad = MHM_STATUS_PIVOT;
+          d = T(1);
+        }
+        const T root = sqrt(d);
+        lc[tid] = tid == 0 ? root : win[s * W + tid] / root;
       }
-      const T root = sqrt(d);
-      lc[tid] = tid == 0 ? root : win[s * W + tid] / root;
+      lds_barrier();
+      if (tid < W) {
+        const T in = sb[jj * W + tid];
+        win[s * W + tid] = in;       // column j+W enters the freed slot
+        sb[jj * W + tid] = lc[tid];  // the stage slot now holds factor column j
+      }
+      for (int t = tid; t < npairs; t += NT) {
+        const unsigned p = pairs[t];
+        const int cc = (int)(p >> 16), r = (int)(p & 0xffffu);
+        int slot = s + cc;
+        if (slot >= W) slot -= W;
+        win[slot * W + r] -= lc[cc + r] * lc[cc];
+      }
+      lds_barrier();
+      s = s + 1 == W ? 0 : s + 1;
     }
-    __syncthreads();
-    if (tid < W) {
-      Lg[(size_t)j * W + tid] = lc[tid];
-      const int cn = j + W;
-      win[s * W + tid] = cn < n ? Lg[(size_t)cn * W + tid] : T(0);
+    if (q + 1 < nchunks) {
+      T *dst = stage + ((q + 1) & 1) * per;
+#pragma unroll
+      for (int v = 0; v < MAXR; ++v) {
+        const int e = tid + v * NT;
+        if (v < nreg && e < per) dst[e] = pf[v];
+      }
+      lds_barrier();
     }
-    for (int t = tid; t < npairs; t += NT) {
-      const unsigned p = pairs[t];
-      const int cc = (int)(p >> 16), r = (int)(p & 0xffffu);
-      int slot = s + cc;
-      if (slot >= W) slot -= W;
-      win[slot * W + r] -= lc[cc + r] * lc[cc];
-    }
-    __syncthreads();
-    s = s + 1 == W ? 0 : s + 1;
+  }
+  {  // factor columns of the last chunk
+    const int q = nchunks - 1;
+    const T *src = stage + (q & 1) * per;
+    const size_t base = (size_t)q * CH * W;
+    const int count = (n - q * CH) * W;
+    for (int e = tid; e < count; e += NT) Lg[base + e] = src[e];
   }
   if (tid == 0 && bad) atomicOr(&status[mac], bad);
 }
 
 // ----------------------------------------------------------------- band solves
-// Block per macro: forward L z = y then backward L^T x = z for ncols columns of
-// y[mac][row][col] (leading dimension ldy), each sweep through a (b+1)-row
-// LDS window.  Skips macros whose every column has converged.
+// Block per macro: forward L z = y (axpy form) then backward L^T x = z (dot
+// form, x_j = (z_j - L(j+1..j+b, j)^T x) / L_jj) for ncols columns of the
+// row-major y[mac][row][ldy].  The (b+1)-row window lives in registers:
+// thread t owns column t % ncols and window slots g, g + G, ... (g = t /
+// ncols, G = NT / ncols); a slot holds one row (row % (b+1)).  One LDS-only
+// barrier per row.  All global traffic happens at chunk boundaries
+// (SOLVE_CHUNK rows): band columns and incoming rows are loaded one chunk
+// ahead into registers and staged in double-buffered LDS; finished rows are
+// collected in an LDS buffer and written out (coalesced) during the next
+// chunk.  Skips macros whose every column has converged.
+#define SOLVE_CHUNK 16
+
+template <typename T, int NT, int MAXR>
+__device__ __forceinline__ void stage_issue(T (&regs)[MAXR], int nreg, const T *Lg, const T *Y, int W,
+                                            int ldy, int ncols, int first_col, int first_row, int dir, int n) {
+  // stage element e < CH*W: band column first_col + dir*(e / W), entry e % W;
+  // element CH*W + e': row first_row + dir*(e' / ncols), column e' % ncols
+  const int ncolvals = SOLVE_CHUNK * W;
+  const int total = ncolvals + SOLVE_CHUNK * ncols;
+#pragma unroll
+  for (int q = 0; q < MAXR; ++q) {
+    const int e = threadIdx.x + q * NT;
+    T v = T(0);
+    if (q < nreg) {
+      if (e < ncolvals) {
+        const int jj = e / W, r = e - jj * W;
+        const int col = first_col + dir * jj;
+        if (col >= 0 && col < n) v = Lg[(size_t)col * W + r];
+      } else if (e < total) {
+        const int e2 = e - ncolvals;
+        const int jj = e2 / ncols, c = e2 - jj * ncols;
+        const int row = first_row + dir * jj;
+        if (row >= 0 && row < n) v = Y[(size_t)row * ldy + c];
+      }
+    }
+    regs[q] = v;
+  }
+}
+
+template <typename T, int NT, int MAXR>
+__device__ __forceinline__ void stage_store(const T (&regs)[MAXR], int nreg, T *stage, int W, int ncols) {
+  const int total = SOLVE_CHUNK * (W + ncols);
+#pragma unroll
+  for (int q = 0; q < MAXR; ++q) {
+    const int e = threadIdx.x + q * NT;
+    if (q < nreg && e < total) stage[e] = regs[q];
+  }
+}
+
+// Rows first_row + dir*jj (jj < count) of the chunk output buffer -> Y.
 template <typename T, int NT>
+__device__ __forceinline__ void rows_out(const T *buf, T *Y, int ldy, int ncols, int first_row, int dir,
+                                         int count) {
+  for (int e = threadIdx.x; e < count * ncols; e += NT) {
+    const int jj = e / ncols, c = e - jj * ncols;
+    Y[(size_t)(first_row + dir * jj) * ldy + c] = buf[e];
+  }
+}
+
+template <typename T, int NT, int MAXS>
 __global__ void __launch_bounds__(NT)
     k_solve(int n, int b, int ldy, int ncols, const T *__restrict__ band, T *__restrict__ y,
-            const int *__restrict__ colstate, int cs_ld, int *__restrict__ iters) {
-  extern __shared__ unsigned char smem_raw[];
+            const int *__restrict__ colstate, int *__restrict__ iters) {
+  constexpr int CH = SOLVE_CHUNK;
+  constexpr int MAXR = 16;  // staged values per thread: CH*(W+ncols) <= NT*MAXR
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
   const int mac = blockIdx.x;
   const int tid = threadIdx.x;
   __shared__ int active;
   if (colstate) {
     if (tid == 0) active = 0;
     __syncthreads();
-    if (tid < ncols && colstate[(size_t)mac * cs_ld + tid] == 0) active = 1;
+    if (tid < ncols && colstate[(size_t)mac * ncols + tid] == 0) active = 1;
     __syncthreads();
     if (!active) return;
   }
   if (iters && tid == 0) iters[mac] += 1;
   const int W = b + 1;
-  T *win = reinterpret_cast<T *>(smem_raw);
-  T *piv = win + (size_t)W * ncols;
+  const int G = NT / ncols;
+  const int g = tid / ncols, c = tid - g * ncols;
+  const bool on = g < G;
+  const int cnt = on ? (W - g + G - 1) / G : 0;  // owned slots: g + G*u, u < cnt
+  const int per = CH * (W + ncols);               // staged values per chunk
+  const int nreg = (per + NT - 1) / NT;
+  T *stage = reinterpret_cast<T *>(smem_raw);     // 2 * per: [CH][W] columns, then [CH][ncols] rows
+  T *outb = stage + 2 * per;                      // 2 * CH * ncols finished rows
+  T *part = outb + 2 * CH * ncols;                // 2 * G * ncols
   const T *Lg = band + (size_t)mac * n * W;
   T *Y = y + (size_t)mac * n * ldy;
-  const int nupd = b * ncols;
-  // forward
-  for (int t = tid; t < W * ncols; t += NT) {
-    const int r = t / ncols, c = t - r * ncols;
-    win[t] = r < n ? Y[(size_t)r * ldy + c] : T(0);
+  T reg[MAXS];
+  T pf[MAXR];
+  const int nchunks = (n + CH - 1) / CH;
+  // ---------------- forward: chunk q covers rows q*CH .. q*CH+CH-1
+#pragma unroll
+  for (int u = 0; u < MAXS; ++u) {
+    const int row = g + G * u;
+    reg[u] = (u < cnt && row < n) ? Y[(size_t)row * ldy + c] : T(0);
   }
+  stage_issue<T, NT, MAXR>(pf, nreg, Lg, Y, W, ldy, ncols, 0, W, 1, n);
+  stage_store<T, NT, MAXR>(pf, nreg, stage, W, ncols);
   __syncthreads();
   int s = 0;
-  for (int j = 0; j < n; ++j) {
-    if (tid < ncols) {
-      const T z = win[s * ncols + tid] / Lg[(size_t)j * W];
-      piv[tid] = z;
-      Y[(size_t)j * ldy + tid] = z;
+  for (int q = 0; q < nchunks; ++q) {
+    const T *sb = stage + (q & 1) * per;
+    T *ob = outb + (q & 1) * CH * ncols;
+    if (q > 0) rows_out<T, NT>(outb + ((q - 1) & 1) * CH * ncols, Y, ldy, ncols, (q - 1) * CH, 1, CH);
+    stage_issue<T, NT, MAXR>(pf, nreg, Lg, Y, W, ldy, ncols, (q + 1) * CH, (q + 1) * CH + W, 1, n);
+    const int jend = min(n, (q + 1) * CH);
+    for (int j = q * CH; j < jend; ++j) {
+      const int jj = j - q * CH;
+      const T *col = sb + jj * W;
+      // phase 1: owner of (slot s, column c) finishes row j; row j+W enters the slot
+      if (on && (s % G) == g) {
+        const int u = s / G;
+        const T in = sb[CH * W + jj * ncols + c];
+        T z = T(0);
+#pragma unroll
+        for (int v = 0; v < MAXS; ++v)
+          if (v == u) z = reg[v];
+        z = z / col[0];
+        ob[jj * ncols + c] = z;
+#pragma unroll
+        for (int v = 0; v < MAXS; ++v)
+          if (v == u) reg[v] = in;
+      }
+      lds_barrier();
+      // phase 2: every other slot holds rows j+1..j+b: subtract L(row, j) z_j
+      if (on) {
+        const T p = ob[jj * ncols + c];
+        T coef[MAXS];
+#pragma unroll
+        for (int u = 0; u < MAXS; ++u) {
+          int r = g + G * u - s;
+          r = r < 0 ? r + W : r;
+          r = u < cnt ? r : 0;
+          coef[u] = r != 0 ? col[r] : T(0);
+        }
+#pragma unroll
+        for (int u = 0; u < MAXS; ++u) reg[u] -= coef[u] * p;
+      }
+      s = s + 1 == W ? 0 : s + 1;
     }
-    __syncthreads();
-    const T *col = Lg + (size_t)j * W;
-    for (int t = tid; t < nupd; t += NT) {
-      const int r = 1 + t / ncols, c = t - (r - 1) * ncols;
-      int slot = s + r;
-      if (slot >= W) slot -= W;
-      win[slot * ncols + c] -= col[r] * piv[c];
-    }
-    if (tid < ncols) {
-      const int rn = j + W;
-      win[s * ncols + tid] = rn < n ? Y[(size_t)rn * ldy + tid] : T(0);
-    }
-    __syncthreads();
-    s = s + 1 == W ? 0 : s + 1;
+    stage_store<T, NT, MAXR>(pf, nreg, stage + ((q + 1) & 1) * per, W, ncols);
+    lds_barrier();
   }
-  // backward: window holds rows j-b..j, slot = row % W
-  for (int t = tid; t < W * ncols; t += NT) {
-    const int rr = t / ncols, c = t - rr * ncols;
-    const int row = n - W + rr;
-    if (row >= 0) win[(row % W) * ncols + c] = Y[(size_t)row * ldy + c];
+  {
+    const int q = nchunks - 1;
+    rows_out<T, NT>(outb + (q & 1) * CH * ncols, Y, ldy, ncols, q * CH, 1, n - q * CH);
   }
   __syncthreads();
-  s = (n - 1) % W;
-  for (int j = n - 1; j >= 0; --j) {
-    if (tid < ncols) {
-      const T x = win[s * ncols + tid] / Lg[(size_t)j * W];
-      piv[tid] = x;
-      Y[(size_t)j * ldy + tid] = x;
-    }
-    __syncthreads();
-    for (int t = tid; t < nupd; t += NT) {
-      const int r = 1 + t / ncols, c = t - (r - 1) * ncols;
-      if (j - r >= 0) {
-        int slot = s - r;
-        if (slot < 0) slot += W;
-        win[slot * ncols + c] -= Lg[(size_t)(j - r) * W + r] * piv[c];
+  // ---------------- backward: step k handles row j = n-1-k; registers hold finished x rows
+#pragma unroll
+  for (int u = 0; u < MAXS; ++u) reg[u] = T(0);
+  stage_issue<T, NT, MAXR>(pf, nreg, Lg, Y, W, ldy, ncols, n - 1, n - 1, -1, n);
+  stage_store<T, NT, MAXR>(pf, nreg, stage, W, ncols);
+  __syncthreads();
+  int sj = (n - 1) % W;
+  for (int q = 0; q < nchunks; ++q) {
+    const T *sb = stage + (q & 1) * per;
+    T *ob = outb + (q & 1) * CH * ncols;
+    if (q > 0)
+      rows_out<T, NT>(outb + ((q - 1) & 1) * CH * ncols, Y, ldy, ncols, n - 1 - (q - 1) * CH, -1, CH);
+    stage_issue<T, NT, MAXR>(pf, nreg, Lg, Y, W, ldy, ncols, n - 1 - (q + 1) * CH, n - 1 - (q + 1) * CH, -1, n);
+    const int kend = min(n, (q + 1) * CH);
+    for (int k = q * CH; k < kend; ++k) {
+      const int j = n - 1 - k;
+      const int kk = k - q * CH;
+      const T *col = sb + kk * W;
+      // phase 1: partial dot over owned finished rows j+1..j+b
+      if (on) {
+        T coef[MAXS];
+#pragma unroll
+        for (int u = 0; u < MAXS; ++u) {
+          int r = g + G * u - sj;
+          r = r < 0 ? r + W : r;
+          r = u < cnt ? r : 0;
+          coef[u] = r != 0 ? col[r] : T(0);
+        }
+        T acc = T(0);
+#pragma unroll
+        for (int u = 0; u < MAXS; ++u) acc += coef[u] * reg[u];
+        part[((k & 1) * G + g) * ncols + c] = acc;
       }
+      lds_barrier();
+      // phase 2: owner of (slot sj, column c) completes x_j
+      if (on && (sj % G) == g) {
+        T sum = T(0);
+        for (int l = 0; l < G; ++l) sum += part[((k & 1) * G + l) * ncols + c];
+        const T x = (sb[CH * W + kk * ncols + c] - sum) / col[0];
+        const int u = sj / G;
+#pragma unroll
+        for (int v = 0; v < MAXS; ++v)
+          if (v == u) reg[v] = x;
+        ob[kk * ncols + c] = x;
+      }
+      sj = sj == 0 ? W - 1 : sj - 1;
     }
-    if (tid < ncols) {
-      const int rn = j - W;
-      if (rn >= 0) win[s * ncols + tid] = Y[(size_t)rn * ldy + tid];
-    }
-    __syncthreads();
-    s = s == 0 ? W - 1 : s - 1;
+    stage_store<T, NT, MAXR>(pf, nreg, stage + ((q + 1) & 1) * per, W, ncols);
+    lds_barrier();
   }
-}
-
-// ----------------------------------------------------------------- right-hand sides
-// Standard columns [f, B, A Z] (and, for the factor step, e_p and c) into the
-// binary64 rhs buffer rhs[mac][col][N] (band rows; row n = constraint row = 0).
-__global__ void __launch_bounds__(256) k_std_rhs(LayoutK L, BatchK B, double *__restrict__ rhs) {
-  const int mac = blockIdx.y;
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  const int N = B.N;
-  if (t >= N * B.R) return;
-  const int c = t / N, i = t - c * N;
-  double v = 0.0;
-  if (i < L.n) {
-    if (c == 0)
-      v = B.load[(size_t)mac * L.n + i];
-    else if (c <= L.m)
-      v = coupling_at(L, B, mac, c - 1, i);
-    else
-      v = B.az[(size_t)mac * L.n + i];
+  {
+    const int q = nchunks - 1;
+    rows_out<T, NT>(outb + (q & 1) * CH * ncols, Y, ldy, ncols, n - 1 - q * CH, -1, n - q * CH);
   }
-  rhs[((size_t)mac * B.R + c) * N + i] = v;
-}
-
-// Scaled copy of binary64 columns into the factor-precision workspace.
-template <typename T>
-__global__ void __launch_bounds__(256)
-    k_load_y(int n, int N, int ncols, int ldy, const double *__restrict__ rhs,
-             const double *__restrict__ dscale, T *__restrict__ y, double *__restrict__ beta) {
-  const int mac = blockIdx.y;
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n * ncols) return;
-  const int i = t / ncols, c = t - i * ncols;
-  const double *r = rhs + ((size_t)mac * ncols + c) * N;
-  y[((size_t)mac * n + i) * ldy + c] = T(dscale[(size_t)mac * n + i] * r[i]);
-  if (i == 0 && beta) beta[(size_t)mac * ncols + c] = N > n ? r[n] : 0.0;
 }
 
 // D e_p and D c into columns 0, 1 of the workspace (factor step).
@@ -600,150 +747,199 @@ __global__ void __launch_bounds__(256) k_saddle_init(LayoutK L, BatchK B, const 
   }
 }
 
-// Block per (column, macro): x = D y (+ bordered correction when k == 1);
-// mode 0 sets X, mode 1 adds the correction and records ||delta||, ||X||.
-template <typename T>
-__global__ void __launch_bounds__(256)
-    k_saddle_apply(LayoutK L, BatchK B, int ncols, const T *__restrict__ y,
-                   const double *__restrict__ beta, double *__restrict__ X, int mode,
-                   const int *__restrict__ colstate, double *__restrict__ colnorm) {
-  __shared__ double red[4];
-  const int c = blockIdx.x, mac = blockIdx.y;
-  if (mode && colstate[(size_t)mac * ncols + c]) return;
-  const int n = L.n, N = B.N;
-  double *x = X + ((size_t)mac * ncols + c) * N;
-  const double *ds = B.dscale + (size_t)mac * n;
-  const T *Y = y + (size_t)mac * n * B.ldy + c;
-  double xp = 0.0, mu = 0.0;
-  const double *sc = B.scal + (size_t)mac * NSCAL;
-  const double *u = B.uw + (size_t)mac * 2 * n;
-  const double *w = u + n;
-  if (L.k) {
-    const double cs = sc[S_CS];
-    double cy = 0.0;
-    for (int i = threadIdx.x; i < n; i += 256)
-      cy += cs * constraint_at(L, B, mac, i) * (ds[i] * (double)Y[(size_t)i * B.ldy]);
-    cy = block_sum<256>(cy, red);
-    const double yp = ds[L.pin] * (double)Y[(size_t)L.pin * B.ldy];
-    const double r1 = yp, r2 = cy - beta[(size_t)mac * ncols + c];
-    const double det = sc[S_DET];
-    xp = (r1 * sc[S_A22] - sc[S_A12] * r2) / det;
-    mu = (sc[S_A11] * r2 - sc[S_A21] * r1) / det;
-  }
-  const double sigma = sc[S_SIGMA];
-  double dn = 0.0, xn = 0.0;
-  for (int i = threadIdx.x; i < n; i += 256) {
-    double v = ds[i] * (double)Y[(size_t)i * B.ldy];
-    if (L.k) v += sigma * xp * u[i] - mu * w[i];
-    if (mode) {
-      const double nv = x[i] + v;
-      x[i] = nv;
-      dn += v * v;
-      xn += nv * nv;
-    } else {
-      x[i] = v;
-    }
-  }
-  if (L.k && threadIdx.x == 0) {
-    if (mode) {
-      x[n] += mu;
-    } else {
-      x[n] = mu;
-    }
-  }
-  // colnorm[3*col]: ||delta||^2 of the last correction, ||x||^2, previous ||delta||^2
-  if (mode) {
-    dn = block_sum<256>(dn, red);
-    xn = block_sum<256>(xn, red);
-    if (threadIdx.x == 0) {
-      double *cn = colnorm + ((size_t)mac * ncols + c) * 3;
-      cn[2] = cn[0];
-      cn[0] = dn;
-      cn[1] = xn;
-    }
-  } else if (threadIdx.x == 0) {
-    double *cn = colnorm + ((size_t)mac * ncols + c) * 3;
-    cn[0] = cn[2] = INFINITY;
-    cn[1] = 0.0;
-  }
+// ----------------------------------------------------------------- refinement
+// Solutions are row-major per macro: X[mac][row][col], N rows (row n = the
+// constraint multiplier when k == 1).  Both refinement kernels run one block
+// per macro on a (row lane x column) grid: thread t owns column t % ncols and
+// rows lane, lane + lanes, ...; per-column sums are reduced over lanes in a
+// fixed order.  ncols <= 64.
+
+// Right-hand side entry: the standard columns [f, B, A Z] (rhs == NULL) or a
+// caller's row-major band-order rhs.
+__device__ __forceinline__ double rhs_at(const LayoutK &L, const BatchK &B, const double *rhs, int mac,
+                                         int ncols, int i, int c) {
+  if (rhs) return rhs[((size_t)mac * B.N + i) * ncols + c];
+  if (i >= L.n) return 0.0;
+  if (c == 0) return B.load[(size_t)mac * L.n + i];
+  if (c <= L.m) return coupling_at(L, B, mac, c - 1, i);
+  return B.az[(size_t)mac * L.n + i];
 }
 
-// Block per (column, macro): r = b - S x accumulated in double-double, the
-// per-column acceptance ||r/s|| <= rtol ||b/s|| (s = max|b|) of
-// linalg/linear.py:132-152, and the scaled residual as the next correction
-// right-hand side.  fwd_tol > 0 additionally requires the last correction to
-// satisfy ||delta|| <= fwd_tol ||x|| (forward convergence of refinement).
-// check_only: set MHM_STATUS_RESIDUAL for unconverged columns.
+// r = b - S x in double-double, per-column acceptance ||r|| <= rtol ||b||
+// (linalg/linear.py:132-152; the max|b| column scaling of the reference
+// cancels from both sides), scaled correction rhs D r into y, constraint-row
+// residual into beta.  it == 0: x = 0 (initial solve).  fwd_tol > 0 also
+// requires forward convergence of the refinement.  check_only: set
+// MHM_STATUS_RESIDUAL for columns that miss the criterion.
 template <typename T>
 __global__ void __launch_bounds__(256)
     k_residual(LayoutK L, BatchK B, int ncols, const double *__restrict__ rhs,
-               const double *__restrict__ X, double rtol, double fwd_tol, int iteration,
-               T *__restrict__ y, double *__restrict__ beta, int *__restrict__ colstate,
-               const double *__restrict__ colnorm, int check_only) {
-  __shared__ double red[4];
-  const int c = blockIdx.x, mac = blockIdx.y;
-  int *state = colstate + (size_t)mac * ncols + c;
-  if (!check_only && *state) return;
+               const double *__restrict__ X, double rtol, double fwd_tol, int it, T *__restrict__ y,
+               double *__restrict__ beta, int *__restrict__ colstate, const double *__restrict__ colnorm,
+               int check_only) {
+  __shared__ double sred[5][256];
+  const int mac = blockIdx.x;
+  const int tid = threadIdx.x;
+  const int lanes = 256 / ncols;
+  const int lane = tid / ncols, c = tid - lane * ncols;
+  const bool on = lane < lanes;
   const int n = L.n, N = B.N;
-  const double *x = X + ((size_t)mac * ncols + c) * N;
-  const double *bv = rhs + ((size_t)mac * ncols + c) * N;
+  int *state = colstate + (size_t)mac * ncols;
+  // converged columns were accepted by the residual of their current X
+  const bool skip = !on || state[c];
+  const double *x = X + (size_t)mac * N * ncols;
   const double *a = B.aval + (size_t)mac * L.nnz;
   const double cs = B.scal[(size_t)mac * NSCAL + S_CS];
-  const double mu = L.k ? x[n] : 0.0;
-  double bmax = 0.0;
-  for (int i = threadIdx.x; i < N; i += 256) bmax = fmax(bmax, fabs(bv[i]));
-  bmax = block_max<256>(bmax, red);
-  const double scale = bmax > 0.0 ? bmax : 1.0;
+  const double *ds = B.dscale + (size_t)mac * n;
+  const double mu = (L.k && it > 0) ? x[(size_t)n * ncols + c] : 0.0;
+  T *Y = y + (size_t)mac * n * B.ldy;
   double rr = 0.0, bb = 0.0, chi = 0.0, clo = 0.0;
   bool finite = true;
-  T *Y = y + (size_t)mac * n * B.ldy + c;
-  const double *ds = B.dscale + (size_t)mac * n;
-  for (int i = threadIdx.x; i < n; i += 256) {
-    double hi = bv[i], lo = 0.0;
-    for (int s = L.rowptr[i]; s < L.rowptr[i + 1]; ++s) dd_fma(hi, lo, -a[s], x[L.cols[s]]);
-    double ci = 0.0;
+  if (!skip) {
+    for (int i = lane; i < n; i += lanes) {
+      const double bi = rhs_at(L, B, rhs, mac, ncols, i, c);
+      double hi = bi, lo = 0.0;
+      if (it > 0) {
+        for (int s = L.rowptr[i]; s < L.rowptr[i + 1]; ++s)
+          dd_fma(hi, lo, -a[s], x[(size_t)L.cols[s] * ncols + c]);
+        if (L.k) {
+          const double ci = cs * constraint_at(L, B, mac, i);
+          dd_fma(hi, lo, -ci, mu);
+          dd_fma(chi, clo, ci, x[(size_t)i * ncols + c]);
+        }
+      }
+      const double r = hi + lo;
+      finite &= isfinite(r);
+      rr += r * r;
+      bb += bi * bi;
+      if (!check_only) Y[(size_t)i * B.ldy + c] = T(ds[i] * r);
+    }
+  }
+  sred[0][tid] = rr;
+  sred[1][tid] = bb;
+  sred[2][tid] = chi;
+  sred[3][tid] = clo;
+  sred[4][tid] = finite ? 0.0 : 1.0;
+  __syncthreads();
+  if (tid < ncols && !skip) {
+    double R2 = 0.0, B2 = 0.0, CH = 0.0, CL = 0.0, BAD = 0.0;
+    for (int l = 0; l < lanes; ++l) {
+      const int t = l * ncols + tid;
+      R2 += sred[0][t];
+      B2 += sred[1][t];
+      dd_add(CH, CL, sred[2][t]);
+      CL += sred[3][t];
+      BAD += sred[4][t];
+    }
+    double rn = 0.0;
     if (L.k) {
-      ci = cs * constraint_at(L, B, mac, i);
-      dd_fma(hi, lo, -ci, mu);
-      dd_fma(chi, clo, ci, x[i]);
+      const double bn = rhs ? rhs[((size_t)mac * N + n) * ncols + tid] : 0.0;
+      rn = bn - (CH + CL);
+      R2 += rn * rn;
+      B2 += bn * bn;
     }
-    const double r = hi + lo;
-    finite &= isfinite(r);
-    rr += (r / scale) * (r / scale);
-    bb += (bv[i] / scale) * (bv[i] / scale);
-    if (!check_only) Y[(size_t)i * B.ldy] = T(ds[i] * r);
-  }
-  double rn = 0.0;
-  if (L.k) {
-    const double cx = block_sum<256>(chi, red) + block_sum<256>(clo, red);
-    rn = bv[n] - cx;
-    if (threadIdx.x == 0) {
-      rr += (rn / scale) * (rn / scale);
-      bb += (bv[n] / scale) * (bv[n] / scale);
-    }
-  }
-  rr = block_sum<256>(rr, red);
-  bb = block_sum<256>(bb, red);
-  __shared__ int anybad;
-  if (threadIdx.x == 0) anybad = 0;
-  __syncthreads();
-  if (!finite) anybad = 1;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    bool ok = sqrt(rr) <= rtol * sqrt(bb) && !anybad;
+    const bool resid_ok = sqrt(R2) <= rtol * sqrt(B2) && BAD == 0.0;
+    bool ok = resid_ok && it > 0;  // the initial solve always runs
     if (ok && fwd_tol > 0.0) {
       // forward convergence: the last correction is below fwd_tol ||x||, or it
       // no longer halves (refinement has reached the binary64 noise floor)
-      const double *cn = colnorm + ((size_t)mac * ncols + c) * 3;
+      const double *cn = colnorm + ((size_t)mac * ncols + tid) * 3;
       ok = cn[0] <= fwd_tol * fwd_tol * cn[1] || cn[0] >= 0.25 * cn[2];
     }
+    if (BAD != 0.0) atomicOr(&B.status[mac], MHM_STATUS_NONFINITE);
     if (check_only) {
-      if (anybad) atomicOr(&B.status[mac], MHM_STATUS_NONFINITE);
-      if (!(sqrt(rr) <= rtol * sqrt(bb))) atomicOr(&B.status[mac], MHM_STATUS_RESIDUAL);
+      if (!resid_ok) atomicOr(&B.status[mac], MHM_STATUS_RESIDUAL);
     } else {
-      *state = ok ? 1 : 0;
-      beta[(size_t)mac * ncols + c] = rn;
-      if (anybad) atomicOr(&B.status[mac], MHM_STATUS_NONFINITE);
+      state[tid] = ok ? 1 : 0;
+      beta[(size_t)mac * ncols + tid] = rn;
+    }
+  }
+}
+
+// x = D y (+ bordered correction when k == 1); it == 0 sets X, later
+// iterations add the correction and record ||delta||^2, ||x||^2 and the
+// previous ||delta||^2 in colnorm[3 * col].
+template <typename T>
+__global__ void __launch_bounds__(256)
+    k_saddle_apply(LayoutK L, BatchK B, int ncols, const T *__restrict__ y, const double *__restrict__ beta,
+                   double *__restrict__ X, int it, const int *__restrict__ colstate,
+                   double *__restrict__ colnorm) {
+  __shared__ double sred[2][256];
+  __shared__ double sxp[64], smu[64];
+  const int mac = blockIdx.x;
+  const int tid = threadIdx.x;
+  const int lanes = 256 / ncols;
+  const int lane = tid / ncols, c = tid - lane * ncols;
+  const bool on = lane < lanes;
+  const int n = L.n, N = B.N;
+  const bool skip = !on || (it > 0 && colstate[(size_t)mac * ncols + c]);
+  double *x = X + (size_t)mac * N * ncols;
+  const double *ds = B.dscale + (size_t)mac * n;
+  const T *Y = y + (size_t)mac * n * B.ldy;
+  const double *sc = B.scal + (size_t)mac * NSCAL;
+  const double *u = B.uw + (size_t)mac * 2 * n;
+  const double *w = u + n;
+  const double sigma = sc[S_SIGMA];
+  if (L.k) {
+    const double cs = sc[S_CS];
+    double cy = 0.0;
+    if (!skip)
+      for (int i = lane; i < n; i += lanes)
+        cy += cs * constraint_at(L, B, mac, i) * (ds[i] * (double)Y[(size_t)i * B.ldy + c]);
+    sred[0][tid] = cy;
+    __syncthreads();
+    if (tid < ncols) {
+      double s = 0.0;
+      for (int l = 0; l < lanes; ++l) s += sred[0][l * ncols + tid];
+      const double yp = ds[L.pin] * (double)Y[(size_t)L.pin * B.ldy + tid];
+      const double r1 = yp, r2 = s - beta[(size_t)mac * ncols + tid];
+      const double det = sc[S_DET];
+      sxp[tid] = (r1 * sc[S_A22] - sc[S_A12] * r2) / det;
+      smu[tid] = (sc[S_A11] * r2 - sc[S_A21] * r1) / det;
+    }
+    __syncthreads();
+  }
+  double dn = 0.0, xn = 0.0;
+  if (!skip) {
+    const double xp = L.k ? sxp[c] : 0.0, mu = L.k ? smu[c] : 0.0;
+    for (int i = lane; i < n; i += lanes) {
+      double v = ds[i] * (double)Y[(size_t)i * B.ldy + c];
+      if (L.k) v += sigma * xp * u[i] - mu * w[i];
+      double *xi = x + (size_t)i * ncols + c;
+      if (it > 0) {
+        const double nv = *xi + v;
+        *xi = nv;
+        dn += v * v;
+        xn += nv * nv;
+      } else {
+        *xi = v;
+      }
+    }
+    if (L.k && lane == 0) {
+      double *xm = x + (size_t)n * ncols + c;
+      if (it > 0)
+        *xm += mu;
+      else
+        *xm = mu;
+    }
+  }
+  sred[0][tid] = dn;
+  sred[1][tid] = xn;
+  __syncthreads();
+  if (tid < ncols && !(skip && on)) {
+    double D2 = 0.0, X2 = 0.0;
+    for (int l = 0; l < lanes; ++l) {
+      D2 += sred[0][l * ncols + tid];
+      X2 += sred[1][l * ncols + tid];
+    }
+    double *cn = colnorm + ((size_t)mac * ncols + tid) * 3;
+    if (it > 0) {
+      cn[2] = cn[0];
+      cn[0] = D2;
+      cn[1] = X2;
+    } else {
+      cn[0] = cn[2] = INFINITY;
+      cn[1] = 0.0;
     }
   }
 }
@@ -756,59 +952,62 @@ __global__ void k_fill_int(int *p, int v, size_t count) {
 // ----------------------------------------------------------------- condensation
 // Block per macro: the condensed block of core/contributions.py:40-47 in the
 // general (corrected retained) form and the reduced load of
-// core/reconstruction.py:122-127, from s = X0, L = X1..m, E = Z - X(m+1).
+// core/reconstruction.py:122-127, from s = X[:,0], L = X[:,1..m],
+// E = Z - X[:,m+1] (X row-major, R columns).
 __global__ void __launch_bounds__(256)
     k_condense(LayoutK L, BatchK B, const double *__restrict__ X, double *__restrict__ blocks,
                double *__restrict__ loads) {
+  __shared__ double sal[64];
   const int mac = blockIdx.x;
-  const int n = L.n, m = L.m, k = L.k, N = B.N;
+  const int n = L.n, m = L.m, k = L.k, N = B.N, R = B.R;
   const int w = m + k;
-  const double *Xm = X + (size_t)mac * B.R * N;
-  const double *s = Xm;
+  const double *Xm = X + (size_t)mac * N * R;
   const double *az = B.az + (size_t)mac * n;
   const double *f = B.load + (size_t)mac * n;
   double *blk = blocks + (size_t)mac * w * w;
   double *ld = loads + (size_t)mac * w;
   const int nrows = L.crows ? L.ncr : n;
   for (int t = threadIdx.x; t < m * m; t += 256) {
-    const int a = t / m, b = t - a * m;
-    const double *lb = Xm + (size_t)(1 + b) * N;
+    const int a = t / m, bcol = t - a * m;
     double acc = 0.0;
     for (int q = 0; q < nrows; ++q) {
       const int i = L.crows ? L.crows[q] : q;
-      acc += coupling_at(L, B, mac, a, i) * lb[i];
+      acc += coupling_at(L, B, mac, a, i) * Xm[(size_t)i * R + 1 + bcol];
     }
-    blk[a * w + b] = acc;
+    blk[a * w + bcol] = acc;
   }
   for (int a = threadIdx.x; a < m; a += 256) {
     double acc = 0.0;
     for (int q = 0; q < nrows; ++q) {
       const int i = L.crows ? L.crows[q] : q;
-      acc += coupling_at(L, B, mac, a, i) * s[i];
+      acc += coupling_at(L, B, mac, a, i) * Xm[(size_t)i * R];
     }
     ld[a] = acc;
   }
   if (!k) return;
-  const double *xaz = Xm + (size_t)(1 + m) * N;
+  // (A Z)^T L by column, lanes over rows
   for (int a = threadIdx.x; a < m; a += 256) {
-    const double *la = Xm + (size_t)(1 + a) * N;
     double g = 0.0, zb = 0.0;
     for (int q = 0; q < nrows; ++q) {
       const int i = L.crows ? L.crows[q] : q;
       const double bi = coupling_at(L, B, mac, a, i);
-      g += bi * (kernel_at(L, B, mac, i) - xaz[i]);
+      g += bi * (kernel_at(L, B, mac, i) - Xm[(size_t)i * R + 1 + m]);
       zb += kernel_at(L, B, mac, i) * bi;
     }
-    double al = 0.0;
-    for (int i = 0; i < n; ++i) al += az[i] * la[i];
     blk[a * w + m] = -g;
-    blk[m * w + a] = -(zb - al);
+    sal[a] = zb;
+  }
+  __syncthreads();
+  for (int a = threadIdx.x; a < m; a += 256) {
+    double al = 0.0;
+    for (int i = 0; i < n; ++i) al += az[i] * Xm[(size_t)i * R + 1 + a];
+    blk[m * w + a] = -(sal[a] - al);
   }
   if (threadIdx.x == 0) {
     double cm = 0.0, as = 0.0, zf = 0.0;
     for (int i = 0; i < n; ++i) {
-      cm += az[i] * (kernel_at(L, B, mac, i) - xaz[i]);
-      as += az[i] * s[i];
+      cm += az[i] * (kernel_at(L, B, mac, i) - Xm[(size_t)i * R + 1 + m]);
+      as += az[i] * Xm[(size_t)i * R];
       zf += kernel_at(L, B, mac, i) * f[i];
     }
     blk[m * w + m] = -cm;
@@ -823,13 +1022,13 @@ __global__ void __launch_bounds__(256)
   const int mac = blockIdx.y;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= L.n) return;
-  const int N = B.N, n = L.n, m = L.m;
-  const double *Xm = X + (size_t)mac * B.R * N;
+  const int N = B.N, R = B.R, n = L.n, m = L.m;
+  const double *xr = X + ((size_t)mac * N + i) * R;
   const int o = L.perm[i];
-  if (src) src[(size_t)mac * n + o] = Xm[i];
+  if (src) src[(size_t)mac * n + o] = xr[0];
   if (lifts)
-    for (int a = 0; a < m; ++a) lifts[((size_t)mac * n + o) * m + a] = Xm[(size_t)(1 + a) * N + i];
-  if (ret && L.k) ret[(size_t)mac * n + o] = kernel_at(L, B, mac, i) - Xm[(size_t)(1 + m) * N + i];
+    for (int a = 0; a < m; ++a) lifts[((size_t)mac * n + o) * m + a] = xr[1 + a];
+  if (ret && L.k) ret[(size_t)mac * n + o] = kernel_at(L, B, mac, i) - xr[1 + m];
 }
 
 // u = s - L lambda + E c, original rows.
@@ -839,34 +1038,36 @@ __global__ void __launch_bounds__(256)
   const int mac = blockIdx.y;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= L.n) return;
-  const int N = B.N, n = L.n, m = L.m, k = L.k;
-  const double *Xm = X + (size_t)mac * B.R * N;
+  const int N = B.N, R = B.R, n = L.n, m = L.m, k = L.k;
+  const double *xr = X + ((size_t)mac * N + i) * R;
   const int o = L.perm[i];
-  const double e = k ? kernel_at(L, B, mac, i) - Xm[(size_t)(1 + m) * N + i] : 0.0;
+  const double e = k ? kernel_at(L, B, mac, i) - xr[1 + m] : 0.0;
   for (int c = 0; c < ncases; ++c) {
     const double *lam = tr + ((size_t)mac * ncases + c) * m;
     double t = 0.0;
-    for (int a = 0; a < m; ++a) t += Xm[(size_t)(1 + a) * N + i] * lam[a];
-    double u = Xm[i] - t;
+    for (int a = 0; a < m; ++a) t += xr[1 + a] * lam[a];
+    double u = xr[0] - t;
     if (k) u += e * co[((size_t)mac * ncases + c) * k];
     fields[((size_t)mac * ncases + c) * n + o] = u;
   }
 }
 
-// Permute user columns between original and band rows.
+// Caller columns [mac][col][N] (original rows) <-> internal row-major
+// [mac][row][col] (band rows); row n (constraint) keeps its position.
 __global__ void __launch_bounds__(256)
     k_permute_cols(const int *__restrict__ perm, int n, int N, int ncols, const double *__restrict__ in,
                    double *__restrict__ out, int to_band) {
   const int mac = blockIdx.y;
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= N * ncols) return;
-  const int c = t / N, i = t - c * N;
-  const size_t base = ((size_t)mac * ncols + c) * N;
+  const int i = t / ncols, c = t - i * ncols;  // internal (row, col)
   const int o = i < n ? perm[i] : i;
+  const size_t ext = ((size_t)mac * ncols + c) * N + o;
+  const size_t inr = ((size_t)mac * N + i) * ncols + c;
   if (to_band)
-    out[base + i] = in[base + o];
+    out[inr] = in[ext];
   else
-    out[base + o] = in[base + i];
+    out[ext] = in[inr];
 }
 
 // Copy a per-macro [macro][rows..] double array from original to band rows.
@@ -923,7 +1124,7 @@ struct mhm_batch {
   size_t tsize = 8;
   BatchK bk{};
   void *band = nullptr, *y = nullptr;
-  double *xstd = nullptr, *rhsstd = nullptr, *beta = nullptr, *colnorm = nullptr;
+  double *xstd = nullptr, *beta = nullptr, *colnorm = nullptr;
   double *rhsu = nullptr, *xu = nullptr;
   int *colstate = nullptr;
   double *ework = nullptr, *fwork = nullptr;
@@ -973,11 +1174,7 @@ struct DeviceGuard {
 
 static size_t factor_lds(int b, size_t tsize) {
   const size_t W = (size_t)b + 1;
-  return W * W * tsize + W * tsize + (size_t)(b * (b + 1) / 2) * 4 + 16;
-}
-
-static size_t solve_lds(int b, int ncols, size_t tsize) {
-  return ((size_t)b + 1) * ncols * tsize + (size_t)ncols * tsize + 16;
+  return (W * W + W + 2 * FACTOR_CHUNK * W) * tsize + (size_t)(b * (b + 1) / 2) * 4 + 16;
 }
 
 static int lds_limit(int device) {
@@ -987,6 +1184,67 @@ static int lds_limit(int device) {
     v = 65536;
   return v;
 }
+
+static int block_threads(int b) { return b >= 12 ? 256 : 64; }
+
+static size_t solve_lds(int b, int ncols, size_t tsize) {
+  const size_t W = (size_t)b + 1;
+  return (2 * SOLVE_CHUNK * (W + ncols) + 2 * SOLVE_CHUNK * (size_t)ncols + 2 * (size_t)block_threads(b)) *
+             tsize + 16;
+}
+
+static bool solve_stage_fits(int b, int ncols) {
+  return (size_t)SOLVE_CHUNK * (b + 1 + ncols) <= (size_t)block_threads(b) * 16;
+}
+
+// Register slots per thread of the solve window.
+static int solve_slots(int b, int ncols) {
+  const int G = block_threads(b) / ncols;
+  return (b + 1 + G - 1) / G;
+}
+
+template <typename T>
+static int launch_solve(const mhm_batch *b, int ncols, const int *colstate, int *iters, hipStream_t st) {
+  const LayoutK &L = b->layout->k;
+  const size_t shm = solve_lds(L.b, ncols, sizeof(T));
+  const int slots = solve_slots(L.b, ncols);
+  const dim3 grid(b->M);
+#define MHM_SOLVE(NT, MS)                                                                              \
+  hipLaunchKernelGGL((k_solve<T, NT, MS>), grid, dim3(NT), shm, st, L.n, L.b, b->ldy, ncols,        \
+                     (const T *)b->band, (T *)b->y, colstate, iters)
+  if (!solve_stage_fits(L.b, ncols) || shm > (size_t)lds_limit(b->layout->device))
+    return fail(MHM_EINVAL, "solve stage exceeds the register or LDS budget");
+  if (block_threads(L.b) == 64) {
+    if (slots <= 4)
+      MHM_SOLVE(64, 4);
+    else if (slots <= 8)
+      MHM_SOLVE(64, 8);
+    else if (slots <= 16)
+      MHM_SOLVE(64, 16);
+    else
+      return fail(MHM_EINVAL, "solve window exceeds the register budget");
+  } else if (slots <= 4) {
+    MHM_SOLVE(256, 4);
+  } else if (slots <= 8) {
+    MHM_SOLVE(256, 8);
+  } else if (slots <= 12) {
+    MHM_SOLVE(256, 12);
+  } else if (slots <= 16) {
+    MHM_SOLVE(256, 16);
+  } else if (slots <= 24) {
+    MHM_SOLVE(256, 24);
+  } else if (slots <= 32) {
+    MHM_SOLVE(256, 32);
+  } else if (slots <= 64) {
+    MHM_SOLVE(256, 64);
+  } else {
+    return fail(MHM_EINVAL, "solve window exceeds the register budget");
+  }
+#undef MHM_SOLVE
+  LAUNCH_CHECK("k_solve");
+  return MHM_OK;
+}
+
 
 // Exported entry points: C linkage comes from their declarations in mhm_hip.h.
 
@@ -1138,9 +1396,14 @@ int mhm_batch_create(const mhm_layout *L, int32_t macros, int32_t precision, int
   const size_t tsize = precision == MHM_FP64 ? 8 : 4;
   const int dev = L->device;
   const int limit = lds_limit(dev);
-  const int nt = lk.b >= 12 ? 256 : 64;
-  if (lk.b + 1 > nt || std::max(std::max(1 + lk.m + lk.k, 2), max_columns) > nt)
-    return fail(MHM_EINVAL, "bandwidth or column count exceeds the workgroup size");
+  const int nt = block_threads(lk.b);
+  if (lk.b + 1 > nt || std::max(std::max(1 + lk.m + lk.k, 2), max_columns) > std::min(nt, 64))
+    return fail(MHM_EINVAL, "bandwidth or column count exceeds the workgroup size (at most 64 columns)");
+  if (solve_slots(lk.b, std::max(std::max(1 + lk.m + lk.k, 2), max_columns)) > (nt == 64 ? 16 : 64) ||
+      solve_slots(lk.b, 2) > (nt == 64 ? 16 : 64))
+    return fail(MHM_EINVAL, "solve window exceeds the register budget");
+  if (!solve_stage_fits(lk.b, std::max(std::max(1 + lk.m + lk.k, 2), max_columns)))
+    return fail(MHM_EINVAL, "solve stage exceeds the register budget");
   if (factor_lds(lk.b, tsize) > (size_t)limit)
     return fail(MHM_EINVAL, "bandwidth " + std::to_string(lk.b) +
                                 " too large for the LDS-resident factor window at this precision");
@@ -1175,7 +1438,6 @@ int mhm_batch_create(const mhm_layout *L, int32_t macros, int32_t precision, int
       (rc = dev_alloc(b->owned, &k.scal, M * NSCAL)) || (rc = dev_alloc(b->owned, &k.uw, M * 2 * n)) ||
       (rc = dev_alloc(b->owned, &k.status, M)) || (rc = dev_alloc(b->owned, &k.iters, M)) ||
       (rc = dev_alloc(b->owned, &b->xstd, M * R * N)) ||
-      (rc = dev_alloc(b->owned, &b->rhsstd, M * R * N)) ||
       (rc = dev_alloc(b->owned, &b->beta, M * ncm)) ||
       (rc = dev_alloc(b->owned, &b->colnorm, M * ncm * 3)) ||
       (rc = dev_alloc(b->owned, &b->colstate, M * ncm)))
@@ -1370,14 +1632,7 @@ static int run_factor(mhm_batch *b, hipStream_t st) {
   if (L.k) {
     hipLaunchKernelGGL((k_load_uw<T>), grid2(L.n, 256, b->M), dim3(256), 0, st, L, b->bk, (T *)b->y);
     LAUNCH_CHECK("k_load_uw");
-    const size_t sshm = solve_lds(L.b, 2, sizeof(T));
-    if (L.b >= 12)
-      hipLaunchKernelGGL((k_solve<T, 256>), dim3(b->M), dim3(256), sshm, st, L.n, L.b, b->ldy, 2,
-                         (const T *)b->band, (T *)b->y, (const int *)nullptr, 0, (int *)nullptr);
-    else
-      hipLaunchKernelGGL((k_solve<T, 64>), dim3(b->M), dim3(64), sshm, st, L.n, L.b, b->ldy, 2,
-                         (const T *)b->band, (T *)b->y, (const int *)nullptr, 0, (int *)nullptr);
-    LAUNCH_CHECK("k_solve(uw)");
+    if (int rc = launch_solve<T>(b, 2, nullptr, nullptr, st)) return rc;
     hipLaunchKernelGGL((k_saddle_init<T>), dim3(b->M), dim3(256), 0, st, L, b->bk, (const T *)b->y);
     LAUNCH_CHECK("k_saddle_init");
   }
@@ -1399,46 +1654,34 @@ int mhm_batch_factor(mhm_batch *b, void *stream) {
   return MHM_OK;
 }
 
-// Solve ncols binary64 columns rhs[mac][col][N] (band rows) into X, refining.
+// Solve ncols columns into the row-major X[mac][row][col] (band rows) with
+// binary64 refinement; rhs == NULL selects the standard columns [f, B, A Z].
 template <typename T>
 static int run_solve(mhm_batch *b, const double *rhs, double *X, int ncols, int max_refine,
                      double rtol, hipStream_t st) {
   const LayoutK &L = b->layout->k;
-  const int NT = L.b >= 12 ? 256 : 64;
-  const size_t sshm = solve_lds(L.b, ncols, sizeof(T));
-  const double fwd = b->precision == MHM_MIXED ? 1.0e-14 : 0.0;
+  const double fwd = b->precision == MHM_MIXED ? 1.0e-12 : 0.0;
   hipLaunchKernelGGL(k_fill_int, dim3((unsigned)(((size_t)b->M * ncols + 255) / 256)), dim3(256), 0, st,
                      b->colstate, 0, (size_t)b->M * ncols);
   hipLaunchKernelGGL(k_fill_int, dim3((b->M + 255) / 256), dim3(256), 0, st, b->bk.iters, 0, (size_t)b->M);
-  hipLaunchKernelGGL((k_load_y<T>), grid2((size_t)L.n * ncols, 256, b->M), dim3(256), 0, st, L.n, b->N,
-                     ncols, b->ldy, rhs, b->bk.dscale, (T *)b->y, b->beta);
-  LAUNCH_CHECK("k_load_y");
   for (int it = 0; it <= max_refine; ++it) {
-    if (it > 0) {
+    {
       TimerSpan span(MHM_TIMER_RESIDUAL, st);
-      hipLaunchKernelGGL((k_residual<T>), dim3(ncols, b->M), dim3(256), 0, st, L, b->bk, ncols, rhs,
-                         (const double *)X, rtol, fwd, it - 1, (T *)b->y, b->beta, b->colstate,
+      hipLaunchKernelGGL((k_residual<T>), dim3(b->M), dim3(256), 0, st, L, b->bk, ncols, rhs,
+                         (const double *)X, rtol, fwd, it, (T *)b->y, b->beta, b->colstate,
                          (const double *)b->colnorm, 0);
       LAUNCH_CHECK("k_residual");
     }
     {
       TimerSpan span(MHM_TIMER_SOLVE, st);
-      if (NT == 256)
-        hipLaunchKernelGGL((k_solve<T, 256>), dim3(b->M), dim3(256), sshm, st, L.n, L.b, b->ldy, ncols,
-                           (const T *)b->band, (T *)b->y, (const int *)b->colstate, ncols, b->bk.iters);
-      else
-        hipLaunchKernelGGL((k_solve<T, 64>), dim3(b->M), dim3(64), sshm, st, L.n, L.b, b->ldy, ncols,
-                           (const T *)b->band, (T *)b->y, (const int *)b->colstate, ncols, b->bk.iters);
-      LAUNCH_CHECK("k_solve");
+      if (int rc = launch_solve<T>(b, ncols, b->colstate, b->bk.iters, st)) return rc;
     }
-    hipLaunchKernelGGL((k_saddle_apply<T>), dim3(ncols, b->M), dim3(256), 0, st, L, b->bk, ncols,
-                       (const T *)b->y, (const double *)b->beta, X, it > 0 ? 1 : 0,
-                       (const int *)b->colstate, b->colnorm);
+    hipLaunchKernelGGL((k_saddle_apply<T>), dim3(b->M), dim3(256), 0, st, L, b->bk, ncols, (const T *)b->y,
+                       (const double *)b->beta, X, it, (const int *)b->colstate, b->colnorm);
     LAUNCH_CHECK("k_saddle_apply");
   }
-  hipLaunchKernelGGL((k_residual<T>), dim3(ncols, b->M), dim3(256), 0, st, L, b->bk, ncols, rhs,
-                     (const double *)X, rtol, 0.0, 0, (T *)b->y, b->beta, b->colstate,
-                     (const double *)b->colnorm, 1);
+  hipLaunchKernelGGL((k_residual<T>), dim3(b->M), dim3(256), 0, st, L, b->bk, ncols, rhs, (const double *)X,
+                     rtol, 0.0, 1, (T *)b->y, b->beta, b->colstate, (const double *)b->colnorm, 1);
   LAUNCH_CHECK("k_residual(check)");
   return MHM_OK;
 }
@@ -1451,12 +1694,8 @@ int mhm_batch_condense(mhm_batch *b, int32_t max_refine, double rtol, double *bl
   const LayoutK &L = b->layout->k;
   DeviceGuard guard(b->layout->device);
   hipStream_t st = (hipStream_t)stream;
-  hipLaunchKernelGGL(k_std_rhs, grid2((size_t)b->N * b->R, 256, b->M), dim3(256), 0, st, L, b->bk,
-                     b->rhsstd);
-  LAUNCH_CHECK("k_std_rhs");
-  int rc = b->precision == MHM_FP64
-               ? run_solve<double>(b, b->rhsstd, b->xstd, b->R, max_refine, rtol, st)
-               : run_solve<float>(b, b->rhsstd, b->xstd, b->R, max_refine, rtol, st);
+  int rc = b->precision == MHM_FP64 ? run_solve<double>(b, nullptr, b->xstd, b->R, max_refine, rtol, st)
+                                    : run_solve<float>(b, nullptr, b->xstd, b->R, max_refine, rtol, st);
   if (rc) return rc;
   TimerSpan span(MHM_TIMER_CONDENSE, st);
   hipLaunchKernelGGL(k_condense, dim3(b->M), dim3(256), 0, st, L, b->bk, (const double *)b->xstd, blocks,
