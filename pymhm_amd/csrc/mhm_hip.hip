@@ -699,6 +699,224 @@ __global__ void __launch_bounds__(NT)
   }
 }
 
+// Row-major copy of the band factor for the backward sweep:
+// rband[j*W + r] = L(j, j-r) (zero for j - r < 0).  Block per (row chunk of
+// BAND_ROWS rows, macro): the columns the chunk needs are read contiguously
+// into LDS, the rows are written contiguously.
+#define BAND_ROWS 32
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+    k_band_rows(int n, int b, const T *__restrict__ band, T *__restrict__ rband) {
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
+  T *cols = reinterpret_cast<T *>(smem_raw);  // columns r0-b .. r0+BAND_ROWS-1
+  const int W = b + 1;
+  const int mac = blockIdx.y;
+  const int r0 = blockIdx.x * BAND_ROWS;
+  const T *Lg = band + (size_t)mac * n * W;
+  T *Rg = rband + (size_t)mac * n * W;
+  const int c0 = r0 - b;
+  const int count = (BAND_ROWS + b) * W;
+  for (int e = threadIdx.x; e < count; e += blockDim.x) {
+    const int col = c0 + e / W;
+    cols[e] = (col >= 0 && col < n) ? Lg[(ptrdiff_t)c0 * W + e] : T(0);
+  }
+  __syncthreads();
+  const int rows = min(BAND_ROWS, n - r0);
+  for (int e = threadIdx.x; e < rows * W; e += blockDim.x) {
+    const int jj = e / W, r = e - jj * W;
+    // L(j, j-r) is entry r of column j - r, local column jj + b - r
+    Rg[(size_t)r0 * W + e] = cols[(jj + b - r) * W + r];
+  }
+}
+
+// Band solves, wavefront per macro (SOLVE_WAVES macros per block): lane c owns
+// right-hand-side column c (ncols <= 64).  Both sweeps are right-looking:
+// forward L z = y visits rows 0..n-1 and uses column j of L (band), backward
+// L^T x = z visits rows n-1..0 and uses row j of L (rband); a step finishes
+// the pending row at the head of the window (v = w_0 / L_jj) and updates the
+// next b rows (w_r -= L_r v).  The window of pending rows is a shift register
+// of MAXW >= b+1 values per lane in VGPRs (compile-time width, so every
+// register index is static): a step is one division and MAXW fused
+// multiply-adds against the band values, broadcast from LDS, with no
+// cross-lane traffic and no barrier.  Band values are read from LDS in small
+// groups, one group ahead of the arithmetic; MEM_FENCE keeps the
+// compiler from hoisting more reads and pin() from sinking a step's updates
+// past the next step (either would hold all staged band values in registers).
+// The band rows of a chunk of steps are loaded one chunk ahead (coalesced,
+// into registers) and staged in a double-buffered per-wave LDS area whose
+// entries r >= b+1 stay zero; incoming and finished rows move between HBM and
+// registers once per chunk.
+#define SOLVE_WAVES 4
+
+// Band values per LDS read group (two groups are in flight).
+template <typename T, int MAXW>
+constexpr int wave_group() {
+  return sizeof(T) * MAXW > 256 ? 4 : 8;
+}
+
+// Steps per chunk: the chunk's incoming, next and finished rows (3 * CH values
+// per lane) share the register file with the MAXW-value window.
+template <typename T, int MAXW>
+constexpr int wave_chunk() {
+  return sizeof(T) * MAXW > 256 ? 4 : 8;
+}
+
+#define MEM_FENCE() asm volatile("" ::: "memory")
+
+template <typename T, int N>
+__device__ __forceinline__ void pin(T (&a)[N]) {
+#pragma unroll
+  for (int k = 0; k < N; ++k) asm volatile("" : "+v"(a[k]));
+}
+
+// A wave-uniform pointer held in SGPRs.  Passing the row base through
+// readfirstlane hides its affine form from loop strength reduction, so an access
+// base[lane] becomes one saddr load/store instead of a 64-bit VGPR pointer per
+// unrolled access.
+template <typename P>
+__device__ __forceinline__ __attribute__((address_space(1))) P *uniform_ptr(P *p) {
+  const unsigned long long v = (unsigned long long)p;
+  const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)v);
+  const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(v >> 32));
+  return (__attribute__((address_space(1))) P *)(((unsigned long long)hi << 32) | lo);
+}
+
+// One sweep over the n rows of Y (ldy columns per row); step k works on row
+// k (DIR > 0) or n-1-k (DIR < 0) with band values Lg[row*W + r].
+template <typename T, int MAXW, int DIR>
+__device__ __forceinline__ void band_sweep(int n_in, int W, int ldy, bool on, unsigned cl, int lane, const T *Lg,
+                                           T *Y, T *stage) {
+  // a laundered copy of n: the two sweeps' row-range tests must not be shared
+  // (a shared test result would stay live, spilled, through the first sweep)
+  int n = n_in;
+  asm volatile("" : "+s"(n));
+  constexpr int CH = wave_chunk<T, MAXW>();
+  constexpr int G = wave_group<T, MAXW>();
+  constexpr int NG = (MAXW + G - 1) / G;       // band-value groups per step
+  constexpr int NPF = (CH * MAXW + 63) / 64;  // staged band values per lane and chunk
+  const int nchunks = (n + CH - 1) / CH;
+  auto row_of = [&](int k) { return DIR > 0 ? k : n - 1 - k; };
+  T pf[NPF];
+  // stage position of the lane's element e = lane + 64 v of a chunk (e = jj*W + r
+  // for the band row of step jj), or -1 past the chunk
+  int spos[NPF];
+#pragma unroll
+  for (int v = 0; v < NPF; ++v) {
+    const int e = lane + 64 * v;
+    const int jj = e / W;
+    spos[v] = e < CH * W ? (DIR > 0 ? jj : CH - 1 - jj) * MAXW + (e - jj * W) : -1;
+  }
+  // band rows of steps q*CH .. q*CH+CH-1 -> registers; they are contiguous in
+  // HBM from row lo, and rows outside [0, n) are the elements outside [elo, ehi)
+  auto issue = [&](int q) {
+    const int lo = DIR > 0 ? q * CH : n - (q + 1) * CH;
+    const int elo = max(0, -lo) * W;
+    const int ehi = min(CH, n - lo) * W;
+#pragma unroll
+    for (int v = 0; v < NPF; ++v) {
+      const int e = lane + 64 * v;
+      auto base = uniform_ptr(Lg + (ptrdiff_t)lo * W + 64 * v);
+      pf[v] = (e >= elo && e < ehi) ? base[lane] : T(0);
+    }
+  };
+  // registers -> stage slots (entries r < W)
+  auto commit = [&](int buf) {
+    T *sb = stage + buf * CH * MAXW;
+#pragma unroll
+    for (int v = 0; v < NPF; ++v)
+      if (spos[v] >= 0) sb[spos[v]] = pf[v];
+  };
+  auto fetch = [&](T(&dst)[G], const T *col, int g) {
+#pragma unroll
+    for (int k = 0; k < G; ++k) dst[k] = (g * G + k < MAXW) ? col[g * G + k] : T(0);
+  };
+  auto load_row = [&](int k) {  // row of step k, zero past the end
+    return k < n ? uniform_ptr(Y + (size_t)row_of(k) * ldy)[cl] : T(0);
+  };
+  T win[MAXW];
+#pragma unroll
+  for (int r = 0; r < MAXW; ++r) {
+    if (r % 8 == 0) MEM_FENCE();  // at most 8 row pointers live at once
+    win[r] = load_row(r);
+  }
+  T rin[CH], rnext[CH], done[CH];
+#pragma unroll
+  for (int i = 0; i < CH; ++i) rin[i] = load_row(MAXW + i);
+  T cb0[G], cb1[G];
+  issue(0);
+  commit(0);
+  for (int q = 0; q < nchunks; ++q) {
+    const int k0 = q * CH;
+    issue(q + 1);
+#pragma unroll
+    for (int i = 0; i < CH; ++i) rnext[i] = load_row(k0 + CH + MAXW + i);
+    const T *sb = stage + (q & 1) * CH * MAXW;
+    fetch(cb0, sb, 0);
+#pragma unroll
+    for (int i = 0; i < CH; ++i) {
+      T v = T(0);
+#pragma unroll
+      for (int g = 0; g < NG; ++g) {
+        T(&cur)[G] = ((i * NG + g) & 1) ? cb1 : cb0;
+        T(&nxt)[G] = ((i * NG + g) & 1) ? cb0 : cb1;
+        MEM_FENCE();
+        if (g + 1 < NG)
+          fetch(nxt, sb + i * MAXW, g + 1);
+        else if (i + 1 < CH)
+          fetch(nxt, sb + (i + 1) * MAXW, 0);
+        MEM_FENCE();
+        if (g == 0) v = win[0] / cur[0];
+#pragma unroll
+        for (int k = 0; k < G; ++k) {
+          const int e = g * G + k;  // w_{e-1} <- w_e - L_e v
+          if (e >= 1 && e < MAXW) win[e - 1] = win[e] - cur[k] * v;
+        }
+      }
+      win[MAXW - 1] = rin[i];
+      done[i] = v;
+      pin(win);
+    }
+    // finished rows -> HBM after the chunk, so the step loop stays one basic block
+#pragma unroll
+    for (int i = 0; i < CH; ++i)
+      if (k0 + i < n) {
+        auto dst = uniform_ptr(Y + (size_t)row_of(k0 + i) * ldy);
+        if (on) dst[cl] = done[i];
+      }
+    commit((q + 1) & 1);
+#pragma unroll
+    for (int i = 0; i < CH; ++i) rin[i] = rnext[i];
+  }
+}
+
+template <typename T, int MAXW>
+__global__ void __launch_bounds__(64 * SOLVE_WAVES) __attribute__((amdgpu_waves_per_eu(2)))
+    k_solve_wave(int n, int b, int ldy, int ncols, int macros, const T *__restrict__ band,
+                 const T *__restrict__ rband, T *__restrict__ y, const int *__restrict__ colstate,
+                 int *__restrict__ iters) {
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
+  constexpr int CH = wave_chunk<T, MAXW>();
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int mac = __builtin_amdgcn_readfirstlane(blockIdx.x * SOLVE_WAVES + wave);
+  if (mac >= macros) return;
+  const int W = b + 1;
+  const bool on = lane < ncols;
+  if (colstate) {
+    const bool pending = on && colstate[(size_t)mac * ncols + lane] == 0;
+    if (!__any(pending)) return;
+  }
+  if (iters && lane == 0) iters[mac] += 1;
+  T *stage = reinterpret_cast<T *>(smem_raw) + (size_t)wave * 2 * CH * MAXW;  // [2][CH][MAXW]
+  for (int e = lane; e < 2 * CH * MAXW; e += 64) stage[e] = T(0);
+  // lanes past ncols read column 0 (in bounds) and store nothing
+  const unsigned cl = on ? (unsigned)lane : 0u;
+  T *Y = y + (size_t)mac * n * ldy;
+  band_sweep<T, MAXW, 1>(n, W, ldy, on, cl, lane, band + (size_t)mac * n * W, Y, stage);
+  // a lane reads back only the rows it wrote itself
+  band_sweep<T, MAXW, -1>(n, W, ldy, on, cl, lane, rband + (size_t)mac * n * W, Y, stage);
+}
+
 // D e_p and D c into columns 0, 1 of the workspace (factor step).
 template <typename T>
 __global__ void __launch_bounds__(256) k_load_uw(LayoutK L, BatchK B, T *__restrict__ y) {
@@ -1123,7 +1341,7 @@ struct mhm_batch {
   int M = 0, precision = MHM_FP64, maxcols = 0, R = 0, N = 0, ldy = 0;
   size_t tsize = 8;
   BatchK bk{};
-  void *band = nullptr, *y = nullptr;
+  void *band = nullptr, *rband = nullptr, *y = nullptr;  // rband: row-major band (wavefront solve)
   double *xstd = nullptr, *beta = nullptr, *colnorm = nullptr;
   double *rhsu = nullptr, *xu = nullptr;
   int *colstate = nullptr;
@@ -1203,9 +1421,48 @@ static int solve_slots(int b, int ncols) {
   return (b + 1 + G - 1) / G;
 }
 
+// Register window widths of the wavefront solve (the smallest >= b+1 is used).
+static const int kWaveWidths[] = {8, 16, 24, 32, 48, 64, 68, 80, 96};
+
+static int wave_width(int b, int ncols) {
+  if (ncols > 64) return 0;
+  for (int w : kWaveWidths)
+    if (b + 1 <= w) return w;
+  return 0;
+}
+
+// Either the wavefront kernel or the block kernel can run a solve of ncols columns.
+static bool solve_supported(int b, int ncols, size_t tsize, int limit) {
+  if (wave_width(b, ncols) > 0) return true;
+  const int cap = block_threads(b) == 64 ? 16 : 64;
+  return solve_slots(b, ncols) <= cap && solve_stage_fits(b, ncols) &&
+         solve_lds(b, ncols, tsize) <= (size_t)limit;
+}
+
+template <typename T, int MAXW>
+static void launch_wave(const mhm_batch *b, int ncols, const int *colstate, int *iters, hipStream_t st) {
+  const LayoutK &L = b->layout->k;
+  const size_t shm = (size_t)SOLVE_WAVES * 2 * wave_chunk<T, MAXW>() * MAXW * sizeof(T);
+  const dim3 grid((unsigned)((b->M + SOLVE_WAVES - 1) / SOLVE_WAVES));
+  hipLaunchKernelGGL((k_solve_wave<T, MAXW>), grid, dim3(64 * SOLVE_WAVES), shm, st, L.n, L.b, b->ldy, ncols,
+                     b->M, (const T *)b->band, (const T *)b->rband, (T *)b->y, colstate, iters);
+}
+
 template <typename T>
 static int launch_solve(const mhm_batch *b, int ncols, const int *colstate, int *iters, hipStream_t st) {
   const LayoutK &L = b->layout->k;
+  switch (b->rband ? wave_width(L.b, ncols) : 0) {
+#define MHM_WAVE(w)                                \
+  case w:                                          \
+    launch_wave<T, w>(b, ncols, colstate, iters, st); \
+    LAUNCH_CHECK("k_solve_wave");                  \
+    return MHM_OK;
+    MHM_WAVE(8) MHM_WAVE(16) MHM_WAVE(24) MHM_WAVE(32) MHM_WAVE(48) MHM_WAVE(64) MHM_WAVE(68) MHM_WAVE(80)
+    MHM_WAVE(96)
+#undef MHM_WAVE
+    default:
+      break;
+  }
   const size_t shm = solve_lds(L.b, ncols, sizeof(T));
   const int slots = solve_slots(L.b, ncols);
   const dim3 grid(b->M);
@@ -1399,11 +1656,9 @@ int mhm_batch_create(const mhm_layout *L, int32_t macros, int32_t precision, int
   const int nt = block_threads(lk.b);
   if (lk.b + 1 > nt || std::max(std::max(1 + lk.m + lk.k, 2), max_columns) > std::min(nt, 64))
     return fail(MHM_EINVAL, "bandwidth or column count exceeds the workgroup size (at most 64 columns)");
-  if (solve_slots(lk.b, std::max(std::max(1 + lk.m + lk.k, 2), max_columns)) > (nt == 64 ? 16 : 64) ||
-      solve_slots(lk.b, 2) > (nt == 64 ? 16 : 64))
-    return fail(MHM_EINVAL, "solve window exceeds the register budget");
-  if (!solve_stage_fits(lk.b, std::max(std::max(1 + lk.m + lk.k, 2), max_columns)))
-    return fail(MHM_EINVAL, "solve stage exceeds the register budget");
+  const int widest = std::max(std::max(1 + lk.m + lk.k, 2), max_columns);
+  if (!solve_supported(lk.b, widest, tsize, limit) || !solve_supported(lk.b, 2, tsize, limit))
+    return fail(MHM_EINVAL, "solve window exceeds the register or LDS budget");
   if (factor_lds(lk.b, tsize) > (size_t)limit)
     return fail(MHM_EINVAL, "bandwidth " + std::to_string(lk.b) +
                                 " too large for the LDS-resident factor window at this precision");
@@ -1417,10 +1672,6 @@ int mhm_batch_create(const mhm_layout *L, int32_t macros, int32_t precision, int
   b->N = lk.n + lk.k;
   b->maxcols = max_columns;
   b->ldy = std::max(std::max(b->R, 2), max_columns);
-  if (solve_lds(lk.b, b->ldy, tsize) > (size_t)limit) {
-    delete b;
-    return fail(MHM_EINVAL, "too many columns for the LDS solve window");
-  }
   const size_t M = macros, n = lk.n, N = b->N, R = b->R, ncm = std::max(b->R, max_columns);
   BatchK &k = b->bk;
   k.M = macros;
@@ -1450,6 +1701,10 @@ int mhm_batch_create(const mhm_layout *L, int32_t macros, int32_t precision, int
     unsigned char *p = nullptr;
     if ((rc = dev_alloc(b->owned, &p, M * n * lk.W * tsize))) return cleanup(rc);
     b->band = p;
+    if (wave_width(lk.b, widest) > 0 || wave_width(lk.b, 2) > 0) {
+      if ((rc = dev_alloc(b->owned, &p, M * n * lk.W * tsize))) return cleanup(rc);
+      b->rband = p;
+    }
     if ((rc = dev_alloc(b->owned, &p, M * n * b->ldy * tsize))) return cleanup(rc);
     b->y = p;
   }
@@ -1628,6 +1883,15 @@ static int run_factor(mhm_batch *b, hipStream_t st) {
                        b->bk.status, tiny);
   }
   LAUNCH_CHECK("k_factor");
+  if (b->rband) {
+    const size_t rshm = (size_t)(BAND_ROWS + L.b) * L.W * sizeof(T);
+    if (rshm > 65536)
+      HIP_TRY(hipFuncSetAttribute((const void *)k_band_rows<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)rshm));
+    hipLaunchKernelGGL((k_band_rows<T>), dim3((L.n + BAND_ROWS - 1) / BAND_ROWS, b->M), dim3(256), rshm, st,
+                       L.n, L.b, (const T *)b->band, (T *)b->rband);
+    LAUNCH_CHECK("k_band_rows");
+  }
   }
   if (L.k) {
     hipLaunchKernelGGL((k_load_uw<T>), grid2(L.n, 256, b->M), dim3(256), 0, st, L, b->bk, (T *)b->y);
