@@ -392,7 +392,7 @@ __global__ void __launch_bounds__(256) k_prepare(LayoutK L, BatchK B, T *__restr
 
 template <typename T, int NT>
 __global__ void __launch_bounds__(NT)
-    k_factor(int n, int b, T *__restrict__ band, int *__restrict__ status, double tiny) {
+    k_factor(int n, int b, T *__restrict__ band, T *__restrict__ rband, int *__restrict__ status, double tiny) {
   constexpr int CH = FACTOR_CHUNK;
   constexpr int MAXR = 8;  // per <= NT * MAXR because W <= NT
   extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
@@ -408,6 +408,14 @@ __global__ void __launch_bounds__(NT)
   const int mac = blockIdx.x;
   const int tid = threadIdx.x;
   T *Lg = band + (size_t)mac * n * W;
+  // row-major copy for the backward solve: rband[j*W + r] = L(j, j-r), written
+  // as each column finishes; entries with j - r < 0 are zero
+  T *Rg = rband ? rband + (size_t)mac * n * W : nullptr;
+  if (Rg)
+    for (int e = tid; e < min(b, n) * W; e += NT) {
+      const int j = e / W, r = e - j * W;
+      if (r > j) Rg[e] = T(0);
+    }
   for (int t = tid; t < npairs; t += NT) {
     int cc = 1, rem = t;  // t -> (cc = 1..b, r = 0..b-cc), column-offset major
     while (rem >= b - cc + 1) {
@@ -461,6 +469,7 @@ __global__ void __launch_bounds__(NT)
         const T in = sb[jj * W + tid];
         win[s * W + tid] = in;       // column j+W enters the freed slot
         sb[jj * W + tid] = lc[tid];  // the stage slot now holds factor column j
+        if (Rg && j + tid < n) Rg[(size_t)(j + tid) * W + tid] = lc[tid];
       }
       for (int t = tid; t < npairs; t += NT) {
         const unsigned p = pairs[t];
@@ -699,37 +708,6 @@ __global__ void __launch_bounds__(NT)
   }
 }
 
-// Row-major copy of the band factor for the backward sweep:
-// rband[j*W + r] = L(j, j-r) (zero for j - r < 0).  Block per (row chunk of
-// BAND_ROWS rows, macro): the columns the chunk needs are read contiguously
-// into LDS, the rows are written contiguously.
-#define BAND_ROWS 32
-
-template <typename T>
-__global__ void __launch_bounds__(256)
-    k_band_rows(int n, int b, const T *__restrict__ band, T *__restrict__ rband) {
-  extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
-  T *cols = reinterpret_cast<T *>(smem_raw);  // columns r0-b .. r0+BAND_ROWS-1
-  const int W = b + 1;
-  const int mac = blockIdx.y;
-  const int r0 = blockIdx.x * BAND_ROWS;
-  const T *Lg = band + (size_t)mac * n * W;
-  T *Rg = rband + (size_t)mac * n * W;
-  const int c0 = r0 - b;
-  const int count = (BAND_ROWS + b) * W;
-  for (int e = threadIdx.x; e < count; e += blockDim.x) {
-    const int col = c0 + e / W;
-    cols[e] = (col >= 0 && col < n) ? Lg[(ptrdiff_t)c0 * W + e] : T(0);
-  }
-  __syncthreads();
-  const int rows = min(BAND_ROWS, n - r0);
-  for (int e = threadIdx.x; e < rows * W; e += blockDim.x) {
-    const int jj = e / W, r = e - jj * W;
-    // L(j, j-r) is entry r of column j - r, local column jj + b - r
-    Rg[(size_t)r0 * W + e] = cols[(jj + b - r) * W + r];
-  }
-}
-
 // Band solves, wavefront per macro (SOLVE_WAVES macros per block): lane c owns
 // right-hand-side column c (ncols <= 64).  Both sweeps are right-looking:
 // forward L z = y visits rows 0..n-1 and uses column j of L (band), backward
@@ -917,22 +895,22 @@ __global__ void __launch_bounds__(64 * SOLVE_WAVES) __attribute__((amdgpu_waves_
   band_sweep<T, MAXW, -1>(n, W, ldy, on, cl, lane, rband + (size_t)mac * n * W, Y, stage);
 }
 
-// D e_p and D c into columns 0, 1 of the workspace (factor step).
+// D e_p and D c into columns c0, c0+1 of the workspace.
 template <typename T>
-__global__ void __launch_bounds__(256) k_load_uw(LayoutK L, BatchK B, T *__restrict__ y) {
+__global__ void __launch_bounds__(256) k_load_uw(LayoutK L, BatchK B, T *__restrict__ y, int c0) {
   const int mac = blockIdx.y;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= L.n) return;
   const double d = B.dscale[(size_t)mac * L.n + i];
   const double cs = B.scal[(size_t)mac * NSCAL + S_CS];
-  T *row = y + ((size_t)mac * L.n + i) * B.ldy;
+  T *row = y + ((size_t)mac * L.n + i) * B.ldy + c0;
   row[0] = T(i == L.pin ? d : 0.0);
   row[1] = T(d * cs * constraint_at(L, B, mac, i));
 }
 
-// u = D y0, w = D y1 and the 2x2 bordered matrix per macro.
+// u = D y_c0, w = D y_c0+1 and the 2x2 bordered matrix per macro.
 template <typename T>
-__global__ void __launch_bounds__(256) k_saddle_init(LayoutK L, BatchK B, const T *__restrict__ y) {
+__global__ void __launch_bounds__(256) k_saddle_init(LayoutK L, BatchK B, const T *__restrict__ y, int c0) {
   __shared__ double red[4];
   const int mac = blockIdx.x;
   double *u = B.uw + (size_t)mac * 2 * L.n;
@@ -942,7 +920,7 @@ __global__ void __launch_bounds__(256) k_saddle_init(LayoutK L, BatchK B, const 
   double cu = 0.0, cw = 0.0;
   for (int i = threadIdx.x; i < L.n; i += 256) {
     const double d = B.dscale[(size_t)mac * L.n + i];
-    const T *row = y + ((size_t)mac * L.n + i) * B.ldy;
+    const T *row = y + ((size_t)mac * L.n + i) * B.ldy + c0;
     const double ui = d * (double)row[0], wi = d * (double)row[1];
     u[i] = ui;
     w[i] = wi;
@@ -989,88 +967,133 @@ __device__ __forceinline__ double rhs_at(const LayoutK &L, const BatchK &B, cons
 // residual into beta.  it == 0: x = 0 (initial solve).  fwd_tol > 0 also
 // requires forward convergence of the refinement.  check_only: set
 // MHM_STATUS_RESIDUAL for columns that miss the criterion.
+// Block per macro, RES_WAVES wavefronts each owning a contiguous quarter of
+// the rows; lane c = column c.  Rows are visited in order, so the CSR
+// structure and values are wave-uniform scalar loads (SGPR operands of the
+// FMAs) and x row j is one coalesced saddr load of ncols values.  Two rows go
+// together: their CSR slots are contiguous, so one chunked slot loop (8 slots:
+// all loads first, then the arithmetic in slot order) feeds two independent
+// double-double accumulations.  Per-lane norms are combined across the
+// wavefronts in LDS.
+#define RES_WAVES 4
+
 template <typename T>
-__global__ void __launch_bounds__(256)
-    k_residual(LayoutK L, BatchK B, int ncols, const double *__restrict__ rhs,
-               const double *__restrict__ X, double rtol, double fwd_tol, int it, T *__restrict__ y,
-               double *__restrict__ beta, int *__restrict__ colstate, const double *__restrict__ colnorm,
-               int check_only) {
-  __shared__ double sred[5][256];
+__global__ void __launch_bounds__(64 * RES_WAVES)
+    k_residual(LayoutK L, BatchK B, int ncols, const double *__restrict__ rhs, const double *__restrict__ X,
+               double rtol, double fwd_tol, int it, T *__restrict__ y, double *__restrict__ beta,
+               int *__restrict__ colstate, const double *__restrict__ colnorm, int check_only,
+               const int *__restrict__ rowptr, const int *__restrict__ cols, const double *__restrict__ aval,
+               const double *__restrict__ dscale, const double *__restrict__ coupling_rm) {
+  __shared__ double part[RES_WAVES][5][64];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int mac = blockIdx.x;
-  const int tid = threadIdx.x;
-  const int lanes = 256 / ncols;
-  const int lane = tid / ncols, c = tid - lane * ncols;
-  const bool on = lane < lanes;
-  const int n = L.n, N = B.N;
+  const int n = L.n, N = B.N, m = L.m;
+  const bool on = lane < ncols;
+  const int c = on ? lane : 0;
   int *state = colstate + (size_t)mac * ncols;
   // converged columns were accepted by the residual of their current X
   const bool skip = !on || state[c];
+  if (!__any(!skip)) return;  // same answer in every wavefront of the block
   const double *x = X + (size_t)mac * N * ncols;
-  const double *a = B.aval + (size_t)mac * L.nnz;
+  const double *a = aval + (size_t)mac * L.nnz;
+  const double *ds = dscale + (size_t)mac * n;
   const double cs = B.scal[(size_t)mac * NSCAL + S_CS];
-  const double *ds = B.dscale + (size_t)mac * n;
   const double mu = (L.k && it > 0) ? x[(size_t)n * ncols + c] : 0.0;
   T *Y = y + (size_t)mac * n * B.ldy;
+  // right-hand side column c: explicit rhs, or [f, B, A Z] (B through the
+  // per-macro column map and scale, or per-macro couplings)
+  const int kind = rhs ? 3 : (c == 0 ? 0 : (c <= m ? 1 : 2));
+  const int acol = kind == 1 ? c - 1 : 0;
+  const int src = (kind == 1 && B.colmap) ? B.colmap[(size_t)mac * m + acol] : acol;
+  const double cscale = (kind == 1 && B.colscale) ? B.colscale[(size_t)mac * m + acol] : 1.0;
+  auto rhs_value = [&](int i) -> double {
+    if (kind == 3) return rhs[((size_t)mac * N + i) * ncols + c];
+    if (kind == 0) return B.load[(size_t)mac * n + i];
+    if (kind == 2) return B.az[(size_t)mac * n + i];
+    if (B.bcoup) return B.bcoup[((size_t)mac * m + acol) * n + i];
+    return B.colscale ? coupling_rm[(size_t)i * m + src] * cscale : coupling_rm[(size_t)i * m + src];
+  };
   double rr = 0.0, bb = 0.0, chi = 0.0, clo = 0.0;
   bool finite = true;
-  if (!skip) {
-    for (int i = lane; i < n; i += lanes) {
-      const double bi = rhs_at(L, B, rhs, mac, ncols, i, c);
-      double hi = bi, lo = 0.0;
-      if (it > 0) {
-        for (int s = L.rowptr[i]; s < L.rowptr[i + 1]; ++s)
-          dd_fma(hi, lo, -a[s], x[(size_t)L.cols[s] * ncols + c]);
-        if (L.k) {
-          const double ci = cs * constraint_at(L, B, mac, i);
-          dd_fma(hi, lo, -ci, mu);
-          dd_fma(chi, clo, ci, x[(size_t)i * ncols + c]);
+  auto finish_row = [&](int i, double bi, double hi, double lo) {
+    if (it > 0 && L.k) {
+      const double ci = cs * constraint_at(L, B, mac, i);
+      dd_fma(hi, lo, -ci, mu);
+      dd_fma(chi, clo, ci, x[(size_t)i * ncols + c]);
+    }
+    const double r = hi + lo;
+    finite &= isfinite(r);
+    rr += r * r;
+    bb += bi * bi;
+    if (!check_only && on) Y[(size_t)i * B.ldy + c] = T(ds[i] * r);
+  };
+  const int per = (n + RES_WAVES - 1) / RES_WAVES;
+  const int i0 = min(n, wave * per), i1 = min(n, i0 + per);
+  for (int i = i0; i < i1; i += 2) {
+    const bool two = i + 1 < i1;
+    const double b0 = rhs_value(i);
+    const double b1 = two ? rhs_value(i + 1) : 0.0;
+    double h0 = b0, l0 = 0.0, h1 = b1, l1 = 0.0;
+    if (it > 0) {
+      const int mid = rowptr[i + 1];
+      const int s1 = two ? rowptr[i + 2] : mid;
+      for (int sb = rowptr[i]; sb < s1; sb += 8) {
+        double av[8], xv[8];
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+          const bool v = sb + t < s1;
+          const int col = v ? cols[sb + t] : i;
+          av[t] = v ? a[sb + t] : 0.0;
+          xv[t] = uniform_ptr(x + (size_t)col * ncols)[c];
+        }
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+          if (sb + t < mid)
+            dd_fma(h0, l0, -av[t], xv[t]);
+          else if (sb + t < s1)
+            dd_fma(h1, l1, -av[t], xv[t]);
         }
       }
-      const double r = hi + lo;
-      finite &= isfinite(r);
-      rr += r * r;
-      bb += bi * bi;
-      if (!check_only) Y[(size_t)i * B.ldy + c] = T(ds[i] * r);
     }
+    finish_row(i, b0, h0, l0);
+    if (two) finish_row(i + 1, b1, h1, l1);
   }
-  sred[0][tid] = rr;
-  sred[1][tid] = bb;
-  sred[2][tid] = chi;
-  sred[3][tid] = clo;
-  sred[4][tid] = finite ? 0.0 : 1.0;
+  part[wave][0][lane] = rr;
+  part[wave][1][lane] = bb;
+  part[wave][2][lane] = chi;
+  part[wave][3][lane] = clo;
+  part[wave][4][lane] = finite ? 0.0 : 1.0;
   __syncthreads();
-  if (tid < ncols && !skip) {
-    double R2 = 0.0, B2 = 0.0, CH = 0.0, CL = 0.0, BAD = 0.0;
-    for (int l = 0; l < lanes; ++l) {
-      const int t = l * ncols + tid;
-      R2 += sred[0][t];
-      B2 += sred[1][t];
-      dd_add(CH, CL, sred[2][t]);
-      CL += sred[3][t];
-      BAD += sred[4][t];
-    }
-    double rn = 0.0;
-    if (L.k) {
-      const double bn = rhs ? rhs[((size_t)mac * N + n) * ncols + tid] : 0.0;
-      rn = bn - (CH + CL);
-      R2 += rn * rn;
-      B2 += bn * bn;
-    }
-    const bool resid_ok = sqrt(R2) <= rtol * sqrt(B2) && BAD == 0.0;
-    bool ok = resid_ok && it > 0;  // the initial solve always runs
-    if (ok && fwd_tol > 0.0) {
-      // forward convergence: the last correction is below fwd_tol ||x||, or it
-      // no longer halves (refinement has reached the binary64 noise floor)
-      const double *cn = colnorm + ((size_t)mac * ncols + tid) * 3;
-      ok = cn[0] <= fwd_tol * fwd_tol * cn[1] || cn[0] >= 0.25 * cn[2];
-    }
-    if (BAD != 0.0) atomicOr(&B.status[mac], MHM_STATUS_NONFINITE);
-    if (check_only) {
-      if (!resid_ok) atomicOr(&B.status[mac], MHM_STATUS_RESIDUAL);
-    } else {
-      state[tid] = ok ? 1 : 0;
-      beta[(size_t)mac * ncols + tid] = rn;
-    }
+  if (wave != 0 || skip) return;
+  double R2 = 0.0, B2 = 0.0, CH = 0.0, CL = 0.0, BAD = 0.0;
+  for (int w = 0; w < RES_WAVES; ++w) {
+    R2 += part[w][0][lane];
+    B2 += part[w][1][lane];
+    dd_add(CH, CL, part[w][2][lane]);
+    CL += part[w][3][lane];
+    BAD += part[w][4][lane];
+  }
+  double rn = 0.0;
+  if (L.k) {
+    const double bn = rhs ? rhs[((size_t)mac * N + n) * ncols + c] : 0.0;
+    rn = bn - (CH + CL);
+    R2 += rn * rn;
+    B2 += bn * bn;
+  }
+  const bool resid_ok = sqrt(R2) <= rtol * sqrt(B2) && BAD == 0.0;
+  bool ok = resid_ok && it > 0;  // the initial solve always runs
+  if (ok && fwd_tol > 0.0) {
+    // forward convergence: the last correction is below fwd_tol ||x||, or it
+    // no longer halves (refinement has reached the binary64 noise floor)
+    const double *cn = colnorm + ((size_t)mac * ncols + c) * 3;
+    ok = cn[0] <= fwd_tol * fwd_tol * cn[1] || cn[0] >= 0.25 * cn[2];
+  }
+  if (BAD != 0.0) atomicOr(&B.status[mac], MHM_STATUS_NONFINITE);
+  if (check_only) {
+    if (!resid_ok) atomicOr(&B.status[mac], MHM_STATUS_RESIDUAL);
+  } else {
+    state[c] = ok ? 1 : 0;
+    beta[(size_t)mac * ncols + c] = rn;
   }
 }
 
@@ -1172,65 +1195,111 @@ __global__ void k_fill_int(int *p, int v, size_t count) {
 // general (corrected retained) form and the reduced load of
 // core/reconstruction.py:122-127, from s = X[:,0], L = X[:,1..m],
 // E = Z - X[:,m+1] (X row-major, R columns).
+// Phase 1 (coupled rows, in chunks of COND_ROWS staged in LDS): the products
+// B^T [L | s | E | Z], one output per (a, col), accumulated in LDS.
+// Phase 2 (all rows): (A Z)^T [L | E | s] and Z^T f, thread (group, col) with
+// coalesced row reads, then a reduction over the groups.
+#define COND_ROWS 32
+
 __global__ void __launch_bounds__(256)
     k_condense(LayoutK L, BatchK B, const double *__restrict__ X, double *__restrict__ blocks,
                double *__restrict__ loads) {
-  __shared__ double sal[64];
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
   const int mac = blockIdx.x;
+  const int tid = threadIdx.x;
   const int n = L.n, m = L.m, k = L.k, N = B.N, R = B.R;
   const int w = m + k;
+  const int nb = m + 1 + 2 * k;  // phase-1 columns: L (m), s, then E and Z when k
+  const int nout = m * nb;
   const double *Xm = X + (size_t)mac * N * R;
   const double *az = B.az + (size_t)mac * n;
   const double *f = B.load + (size_t)mac * n;
   double *blk = blocks + (size_t)mac * w * w;
   double *ld = loads + (size_t)mac * w;
+  double *acc = reinterpret_cast<double *>(smem_raw);  // nout
+  double *bsh = acc + nout;                              // COND_ROWS x m
+  double *xsh = bsh + COND_ROWS * m;                     // COND_ROWS x nb
+  double *red = xsh + COND_ROWS * nb;                    // 256
+  for (int o = tid; o < nout; o += 256) acc[o] = 0.0;
   const int nrows = L.crows ? L.ncr : n;
-  for (int t = threadIdx.x; t < m * m; t += 256) {
-    const int a = t / m, bcol = t - a * m;
-    double acc = 0.0;
-    for (int q = 0; q < nrows; ++q) {
-      const int i = L.crows ? L.crows[q] : q;
-      acc += coupling_at(L, B, mac, a, i) * Xm[(size_t)i * R + 1 + bcol];
+  for (int q0 = 0; q0 < nrows; q0 += COND_ROWS) {
+    const int cnt = min(COND_ROWS, nrows - q0);
+    __syncthreads();
+    for (int e = tid; e < cnt * m; e += 256) {
+      const int q = e / m, a = e - q * m;
+      const int i = L.crows ? L.crows[q0 + q] : q0 + q;
+      bsh[e] = coupling_at(L, B, mac, a, i);
     }
-    blk[a * w + bcol] = acc;
-  }
-  for (int a = threadIdx.x; a < m; a += 256) {
-    double acc = 0.0;
-    for (int q = 0; q < nrows; ++q) {
-      const int i = L.crows ? L.crows[q] : q;
-      acc += coupling_at(L, B, mac, a, i) * Xm[(size_t)i * R];
+    for (int e = tid; e < cnt * nb; e += 256) {
+      const int q = e / nb, c = e - q * nb;
+      const int i = L.crows ? L.crows[q0 + q] : q0 + q;
+      double v;
+      if (c < m)
+        v = Xm[(size_t)i * R + 1 + c];
+      else if (c == m)
+        v = Xm[(size_t)i * R];
+      else if (c == m + 1)
+        v = kernel_at(L, B, mac, i) - Xm[(size_t)i * R + 1 + m];
+      else
+        v = kernel_at(L, B, mac, i);
+      xsh[e] = v;
     }
-    ld[a] = acc;
-  }
-  if (!k) return;
-  // (A Z)^T L by column, lanes over rows
-  for (int a = threadIdx.x; a < m; a += 256) {
-    double g = 0.0, zb = 0.0;
-    for (int q = 0; q < nrows; ++q) {
-      const int i = L.crows ? L.crows[q] : q;
-      const double bi = coupling_at(L, B, mac, a, i);
-      g += bi * (kernel_at(L, B, mac, i) - Xm[(size_t)i * R + 1 + m]);
-      zb += kernel_at(L, B, mac, i) * bi;
+    __syncthreads();
+    for (int o = tid; o < nout; o += 256) {
+      const int a = o / nb, c = o - a * nb;
+      double s = acc[o];
+      for (int q = 0; q < cnt; ++q) s += bsh[q * m + a] * xsh[q * nb + c];
+      acc[o] = s;
     }
-    blk[a * w + m] = -g;
-    sal[a] = zb;
   }
   __syncthreads();
-  for (int a = threadIdx.x; a < m; a += 256) {
-    double al = 0.0;
-    for (int i = 0; i < n; ++i) al += az[i] * Xm[(size_t)i * R + 1 + a];
-    blk[m * w + a] = -(sal[a] - al);
+  double zb = 0.0;
+  for (int o = tid; o < nout; o += 256) {
+    const int a = o / nb, c = o - a * nb;
+    if (c < m) blk[a * w + c] = acc[o];
+    else if (c == m) ld[a] = acc[o];
+    else if (c == m + 1) blk[a * w + m] = -acc[o];
   }
-  if (threadIdx.x == 0) {
-    double cm = 0.0, as = 0.0, zf = 0.0;
-    for (int i = 0; i < n; ++i) {
-      cm += az[i] * (kernel_at(L, B, mac, i) - Xm[(size_t)i * R + 1 + m]);
-      as += az[i] * Xm[(size_t)i * R];
-      zf += kernel_at(L, B, mac, i) * f[i];
+  if (!k) return;
+  // phase 2: column c < m -> az.L_c, c == m -> az.E, c == m+1 -> az.s, c == m+2 -> Z.f
+  const int nc = m + 3;
+  const int G = 256 / nc;
+  const int g = tid / nc, c = tid - g * nc;
+  double sum = 0.0;
+  if (g < G) {
+    for (int i = g; i < n; i += G) {
+      const double *xr = Xm + (size_t)i * R;
+      if (c < m)
+        sum += az[i] * xr[1 + c];
+      else if (c == m)
+        sum += az[i] * (kernel_at(L, B, mac, i) - xr[1 + m]);
+      else if (c == m + 1)
+        sum += az[i] * xr[0];
+      else
+        sum += kernel_at(L, B, mac, i) * f[i];
     }
-    blk[m * w + m] = -cm;
-    ld[m] = as - zf;
   }
+  red[tid] = sum;
+  __syncthreads();
+  if (tid < nc) {
+    double t = 0.0;
+    for (int l = 0; l < G; ++l) t += red[l * nc + tid];
+    if (tid < m) {
+      zb = acc[tid * nb + m + 2];  // Z^T B column a = tid
+      blk[m * w + tid] = -(zb - t);
+    } else if (tid == m) {
+      blk[m * w + m] = -t;
+    } else {
+      red[tid] = t;  // az.s at m+1, Z.f at m+2 (read below by thread m+1)
+    }
+  }
+  __syncthreads();
+  if (tid == m + 1) ld[m] = red[m + 1] - red[m + 2];
+}
+
+static size_t condense_lds(int m, int k) {
+  const size_t nb = (size_t)m + 1 + 2 * k;
+  return ((size_t)m * nb + COND_ROWS * ((size_t)m + nb) + 256) * sizeof(double);
 }
 
 // Responses in original rows: source, lifts (n x m row-major), retained basis.
@@ -1332,6 +1401,7 @@ __global__ void __launch_bounds__(256)
 struct mhm_layout {
   int device = 0;
   LayoutK k{};
+  const double *coupling_rm = nullptr;  // template coupling, row-major n x m (band rows)
   std::vector<void *> owned;
   double cmax_host = 0.0;
 };
@@ -1349,6 +1419,7 @@ struct mhm_batch {
   int *colmap = nullptr;
   double *colscale = nullptr, *bcoup = nullptr, *cons = nullptr, *kern = nullptr;
   bool factored = false, condensed = false;
+  bool uw_pending = false;  // u, w and the bordered 2x2 not yet computed for the current factors
   std::vector<void *> owned;
 };
 
@@ -1608,6 +1679,12 @@ int mhm_layout_create(int32_t device, const mhm_layout_desc *d, mhm_layout **out
       (rc = upload(L->owned, d->kernel, n, &k.kernel)) ||
       (rc = upload(L->owned, d->constraint, n, &k.constraint)))
     return cleanup(rc);
+  if (d->coupling && d->m > 0) {
+    std::vector<double> rm((size_t)d->m * n);
+    for (size_t a = 0; a < (size_t)d->m; ++a)
+      for (size_t i = 0; i < n; ++i) rm[i * d->m + a] = d->coupling[a * n + i];
+    if ((rc = upload(L->owned, rm.data(), rm.size(), &L->coupling_rm))) return cleanup(rc);
+  }
   if (d->ne > 0) {
     const size_t ne = d->ne, nb = d->nb, nq = d->nq, dim = d->dim;
     if ((rc = upload(L->owned, d->slot_ptr, (size_t)d->nnz + 1, &k.slot_ptr)) ||
@@ -1671,7 +1748,9 @@ int mhm_batch_create(const mhm_layout *L, int32_t macros, int32_t precision, int
   b->R = 1 + lk.m + lk.k;
   b->N = lk.n + lk.k;
   b->maxcols = max_columns;
-  b->ldy = std::max(std::max(b->R, 2), max_columns);
+  b->ldy = widest;
+  // room for u, w as two extra columns of the first solve
+  if (lk.k && widest + 2 <= 64 && solve_supported(lk.b, widest + 2, tsize, limit)) b->ldy = widest + 2;
   const size_t M = macros, n = lk.n, N = b->N, R = b->R, ncm = std::max(b->R, max_columns);
   BatchK &k = b->bk;
   k.M = macros;
@@ -1877,29 +1956,15 @@ static int run_factor(mhm_batch *b, hipStream_t st) {
       HIP_TRY(hipFuncSetAttribute((const void *)k_factor<T, 256>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
     hipLaunchKernelGGL((k_factor<T, 256>), dim3(b->M), dim3(256), shm, st, L.n, L.b, (T *)b->band,
-                       b->bk.status, tiny);
+                       (T *)b->rband, b->bk.status, tiny);
   } else {
     hipLaunchKernelGGL((k_factor<T, 64>), dim3(b->M), dim3(64), shm, st, L.n, L.b, (T *)b->band,
-                       b->bk.status, tiny);
+                       (T *)b->rband, b->bk.status, tiny);
   }
   LAUNCH_CHECK("k_factor");
-  if (b->rband) {
-    const size_t rshm = (size_t)(BAND_ROWS + L.b) * L.W * sizeof(T);
-    if (rshm > 65536)
-      HIP_TRY(hipFuncSetAttribute((const void *)k_band_rows<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  (int)rshm));
-    hipLaunchKernelGGL((k_band_rows<T>), dim3((L.n + BAND_ROWS - 1) / BAND_ROWS, b->M), dim3(256), rshm, st,
-                       L.n, L.b, (const T *)b->band, (T *)b->rband);
-    LAUNCH_CHECK("k_band_rows");
   }
-  }
-  if (L.k) {
-    hipLaunchKernelGGL((k_load_uw<T>), grid2(L.n, 256, b->M), dim3(256), 0, st, L, b->bk, (T *)b->y);
-    LAUNCH_CHECK("k_load_uw");
-    if (int rc = launch_solve<T>(b, 2, nullptr, nullptr, st)) return rc;
-    hipLaunchKernelGGL((k_saddle_init<T>), dim3(b->M), dim3(256), 0, st, L, b->bk, (const T *)b->y);
-    LAUNCH_CHECK("k_saddle_init");
-  }
+  // u = Ahat^-1 e_p and w = Ahat^-1 c ride along with the first solve
+  b->uw_pending = L.k > 0;
   return MHM_OK;
 }
 
@@ -1928,15 +1993,40 @@ static int run_solve(mhm_batch *b, const double *rhs, double *X, int ncols, int 
   hipLaunchKernelGGL(k_fill_int, dim3((unsigned)(((size_t)b->M * ncols + 255) / 256)), dim3(256), 0, st,
                      b->colstate, 0, (size_t)b->M * ncols);
   hipLaunchKernelGGL(k_fill_int, dim3((b->M + 255) / 256), dim3(256), 0, st, b->bk.iters, 0, (size_t)b->M);
+  const dim3 rgrid((unsigned)b->M);
+  const mhm_layout *lay = b->layout;
+  // u, w of fresh factors: two extra columns of the first solve when the
+  // workspace has room, else a solve of their own before the refinement
+  const bool uw_inline = b->uw_pending && ncols + 2 <= b->ldy;
+  if (b->uw_pending && !uw_inline) {
+    hipLaunchKernelGGL((k_load_uw<T>), grid2(L.n, 256, b->M), dim3(256), 0, st, L, b->bk, (T *)b->y, 0);
+    LAUNCH_CHECK("k_load_uw");
+    if (int rc = launch_solve<T>(b, 2, nullptr, nullptr, st)) return rc;
+    hipLaunchKernelGGL((k_saddle_init<T>), dim3(b->M), dim3(256), 0, st, L, b->bk, (const T *)b->y, 0);
+    LAUNCH_CHECK("k_saddle_init");
+    b->uw_pending = false;
+  }
   for (int it = 0; it <= max_refine; ++it) {
     {
       TimerSpan span(MHM_TIMER_RESIDUAL, st);
-      hipLaunchKernelGGL((k_residual<T>), dim3(b->M), dim3(256), 0, st, L, b->bk, ncols, rhs,
+      hipLaunchKernelGGL((k_residual<T>), rgrid, dim3(64 * RES_WAVES), 0, st, L, b->bk, ncols, rhs,
                          (const double *)X, rtol, fwd, it, (T *)b->y, b->beta, b->colstate,
-                         (const double *)b->colnorm, 0);
+                         (const double *)b->colnorm, 0, L.rowptr, L.cols, (const double *)b->bk.aval,
+                         (const double *)b->bk.dscale, lay->coupling_rm);
       LAUNCH_CHECK("k_residual");
     }
-    {
+    if (it == 0 && uw_inline) {
+      hipLaunchKernelGGL((k_load_uw<T>), grid2(L.n, 256, b->M), dim3(256), 0, st, L, b->bk, (T *)b->y, ncols);
+      LAUNCH_CHECK("k_load_uw");
+      {
+        // every column is pending at it == 0, so no column state is needed
+        TimerSpan span(MHM_TIMER_SOLVE, st);
+        if (int rc = launch_solve<T>(b, ncols + 2, nullptr, b->bk.iters, st)) return rc;
+      }
+      hipLaunchKernelGGL((k_saddle_init<T>), dim3(b->M), dim3(256), 0, st, L, b->bk, (const T *)b->y, ncols);
+      LAUNCH_CHECK("k_saddle_init");
+      b->uw_pending = false;
+    } else {
       TimerSpan span(MHM_TIMER_SOLVE, st);
       if (int rc = launch_solve<T>(b, ncols, b->colstate, b->bk.iters, st)) return rc;
     }
@@ -1944,8 +2034,10 @@ static int run_solve(mhm_batch *b, const double *rhs, double *X, int ncols, int 
                        (const double *)b->beta, X, it, (const int *)b->colstate, b->colnorm);
     LAUNCH_CHECK("k_saddle_apply");
   }
-  hipLaunchKernelGGL((k_residual<T>), dim3(b->M), dim3(256), 0, st, L, b->bk, ncols, rhs, (const double *)X,
-                     rtol, 0.0, 1, (T *)b->y, b->beta, b->colstate, (const double *)b->colnorm, 1);
+  hipLaunchKernelGGL((k_residual<T>), rgrid, dim3(64 * RES_WAVES), 0, st, L, b->bk, ncols, rhs,
+                     (const double *)X, rtol, 0.0, 1, (T *)b->y, b->beta, b->colstate, (const double *)b->colnorm,
+                     1, L.rowptr, L.cols, (const double *)b->bk.aval, (const double *)b->bk.dscale,
+                     lay->coupling_rm);
   LAUNCH_CHECK("k_residual(check)");
   return MHM_OK;
 }
@@ -1962,7 +2054,10 @@ int mhm_batch_condense(mhm_batch *b, int32_t max_refine, double rtol, double *bl
                                     : run_solve<float>(b, nullptr, b->xstd, b->R, max_refine, rtol, st);
   if (rc) return rc;
   TimerSpan span(MHM_TIMER_CONDENSE, st);
-  hipLaunchKernelGGL(k_condense, dim3(b->M), dim3(256), 0, st, L, b->bk, (const double *)b->xstd, blocks,
+  const size_t cshm = condense_lds(L.m, L.k);
+  if (cshm > 65536)
+    HIP_TRY(hipFuncSetAttribute((const void *)k_condense, hipFuncAttributeMaxDynamicSharedMemorySize, (int)cshm));
+  hipLaunchKernelGGL(k_condense, dim3(b->M), dim3(256), cshm, st, L, b->bk, (const double *)b->xstd, blocks,
                      loads);
   LAUNCH_CHECK("k_condense");
   b->condensed = true;
