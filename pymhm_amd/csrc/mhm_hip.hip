@@ -106,6 +106,36 @@ __device__ double block_sum(double v, double *sh) {
   return t;
 }
 
+// Double-double sum over the block in a fixed order (shuffle tree per
+// wavefront, then the wavefront partials in order); the result is returned to
+// every thread.  sh holds 2 * NT / 64 doubles.
+template <int NT>
+__device__ void block_dd_sum(double &hi, double &lo, double *sh) {
+  for (int off = 32; off > 0; off >>= 1) {
+    const double oh = __shfl_down(hi, off), ol = __shfl_down(lo, off);
+    double s, e;
+    two_sum(hi, oh, s, e);
+    hi = s;
+    lo = lo + ol + e;
+  }
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) {
+    sh[2 * (threadIdx.x >> 6)] = hi;
+    sh[2 * (threadIdx.x >> 6) + 1] = lo;
+  }
+  __syncthreads();
+  double th = 0.0, tl = 0.0;
+#pragma unroll
+  for (int i = 0; i < NT / 64; ++i) {
+    double s, e;
+    two_sum(th, sh[2 * i], s, e);
+    th = s;
+    tl += sh[2 * i + 1] + e;
+  }
+  hi = th;
+  lo = tl;
+}
+
 template <int NT>
 __device__ double block_max(double v, double *sh) {
   v = wave_max(v);
@@ -967,116 +997,186 @@ __device__ __forceinline__ double rhs_at(const LayoutK &L, const BatchK &B, cons
 // residual into beta.  it == 0: x = 0 (initial solve).  fwd_tol > 0 also
 // requires forward convergence of the refinement.  check_only: set
 // MHM_STATUS_RESIDUAL for columns that miss the criterion.
-// Block per macro, RES_WAVES wavefronts each owning a contiguous quarter of
-// the rows; lane c = column c.  Rows are visited in order, so the CSR
-// structure and values are wave-uniform scalar loads (SGPR operands of the
-// FMAs) and x row j is one coalesced saddr load of ncols values.  Two rows go
-// together: their CSR slots are contiguous, so one chunked slot loop (8 slots:
-// all loads first, then the arithmetic in slot order) feeds two independent
-// double-double accumulations.  Per-lane norms are combined across the
-// wavefronts in LDS.
-#define RES_WAVES 4
+// Block per macro.  Rows go in chunks of RES_ROWS; wavefront q takes columns
+// [q*cq, q*cq + cq) (cq = ceil(ncols / 4), at most G; columns past ncols are
+// zero padding) and lane r row i0 + r of the chunk.  x rows stream from HBM
+// once into an LDS ring of RES_ROWS + 2b rows (a chunk's CSR columns lie in
+// rows [i0 - b, i0 + RES_ROWS + b)); a thread walks its row's CSR slots
+// once, each slot's value and column index applied to its G double-double
+// accumulators.  Column c's right-hand side is colptr[c][i * colstride[c]] *
+// colscale[c] (explicit rhs, f, A Z, or the coupling through the per-macro
+// column map and scale).  Each thread keeps its rows' norm partials; they are
+// summed over the lanes once at the end in a fixed order.  Without a ring (no
+// LDS room) x is read from HBM directly.
+#define RES_ROWS 64
 
-template <typename T>
-__global__ void __launch_bounds__(64 * RES_WAVES)
+struct ResShared {
+  double tot[4][64];  // per column: ||r||^2, ||b||^2, constraint row (hi, lo)
+  double bad[64];
+  const double *colptr[64];
+  int colstride[64];
+  double colscale[64], mu[64];
+  int pend[64];
+  double zero;
+};
+
+template <typename T, int G>
+__global__ void __launch_bounds__(256)
     k_residual(LayoutK L, BatchK B, int ncols, const double *__restrict__ rhs, const double *__restrict__ X,
                double rtol, double fwd_tol, int it, T *__restrict__ y, double *__restrict__ beta,
                int *__restrict__ colstate, const double *__restrict__ colnorm, int check_only,
                const int *__restrict__ rowptr, const int *__restrict__ cols, const double *__restrict__ aval,
-               const double *__restrict__ dscale, const double *__restrict__ coupling_rm) {
-  __shared__ double part[RES_WAVES][5][64];
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+               const double *__restrict__ dscale, const double *__restrict__ coupling_rm, int ring, int ldr) {
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
+  __shared__ ResShared sh;
+  double *xr = reinterpret_cast<double *>(smem_raw);  // ring x ldr
   const int mac = blockIdx.x;
-  const int n = L.n, N = B.N, m = L.m;
-  const bool on = lane < ncols;
-  const int c = on ? lane : 0;
+  const int tid = threadIdx.x;
+  const int wave = tid >> 6, lane = tid & 63;
+  const int n = L.n, N = B.N, m = L.m, b = L.b;
   int *state = colstate + (size_t)mac * ncols;
   // converged columns were accepted by the residual of their current X
-  const bool skip = !on || state[c];
-  if (!__any(!skip)) return;  // same answer in every wavefront of the block
+  const bool mine = tid < ncols && state[tid] == 0;
+  if (tid < 64) sh.pend[tid] = mine ? 1 : 0;
   const double *x = X + (size_t)mac * N * ncols;
+  if (tid < 64) {
+    const int c = tid;
+    const double *ptr = &sh.zero;
+    int stride = 0;
+    double scale = 1.0;
+    if (c < ncols) {
+      stride = 1;
+      if (rhs) {
+        ptr = rhs + (size_t)mac * N * ncols + c;
+        stride = ncols;
+      } else if (c == 0) {
+        ptr = B.load + (size_t)mac * n;
+      } else if (c > m) {
+        ptr = B.az + (size_t)mac * n;
+      } else if (B.bcoup) {
+        ptr = B.bcoup + ((size_t)mac * m + (c - 1)) * n;
+      } else {
+        ptr = coupling_rm + (B.colmap ? B.colmap[(size_t)mac * m + c - 1] : c - 1);
+        stride = m;
+        if (B.colscale) scale = B.colscale[(size_t)mac * m + c - 1];
+      }
+    }
+    sh.colptr[c] = ptr;
+    sh.colstride[c] = stride;
+    sh.colscale[c] = scale;
+    sh.mu[c] = (c < ncols && L.k && it > 0) ? x[(size_t)n * ncols + c] : 0.0;
+    if (c == 0) sh.zero = 0.0;
+  }
+  if (!__syncthreads_or(mine)) return;
+  const int cq = (ncols + 3) / 4;
+  const int c0 = wave * cq;  // this wavefront's first column
+  const bool use_ring = ring > 0 && it > 0;
+  if (use_ring)
+    for (int e = tid; e < ring * ldr; e += 256) xr[e] = 0.0;  // padding columns stay zero
   const double *a = aval + (size_t)mac * L.nnz;
   const double *ds = dscale + (size_t)mac * n;
   const double cs = B.scal[(size_t)mac * NSCAL + S_CS];
-  const double mu = (L.k && it > 0) ? x[(size_t)n * ncols + c] : 0.0;
   T *Y = y + (size_t)mac * n * B.ldy;
-  // right-hand side column c: explicit rhs, or [f, B, A Z] (B through the
-  // per-macro column map and scale, or per-macro couplings)
-  const int kind = rhs ? 3 : (c == 0 ? 0 : (c <= m ? 1 : 2));
-  const int acol = kind == 1 ? c - 1 : 0;
-  const int src = (kind == 1 && B.colmap) ? B.colmap[(size_t)mac * m + acol] : acol;
-  const double cscale = (kind == 1 && B.colscale) ? B.colscale[(size_t)mac * m + acol] : 1.0;
-  auto rhs_value = [&](int i) -> double {
-    if (kind == 3) return rhs[((size_t)mac * N + i) * ncols + c];
-    if (kind == 0) return B.load[(size_t)mac * n + i];
-    if (kind == 2) return B.az[(size_t)mac * n + i];
-    if (B.bcoup) return B.bcoup[((size_t)mac * m + acol) * n + i];
-    return B.colscale ? coupling_rm[(size_t)i * m + src] * cscale : coupling_rm[(size_t)i * m + src];
-  };
-  double rr = 0.0, bb = 0.0, chi = 0.0, clo = 0.0;
+  double rr[G], bb[G], chi[G], clo[G];
   bool finite = true;
-  auto finish_row = [&](int i, double bi, double hi, double lo) {
-    if (it > 0 && L.k) {
-      const double ci = cs * constraint_at(L, B, mac, i);
-      dd_fma(hi, lo, -ci, mu);
-      dd_fma(chi, clo, ci, x[(size_t)i * ncols + c]);
-    }
-    const double r = hi + lo;
-    finite &= isfinite(r);
-    rr += r * r;
-    bb += bi * bi;
-    if (!check_only && on) Y[(size_t)i * B.ldy + c] = T(ds[i] * r);
-  };
-  const int per = (n + RES_WAVES - 1) / RES_WAVES;
-  const int i0 = min(n, wave * per), i1 = min(n, i0 + per);
-  for (int i = i0; i < i1; i += 2) {
-    const bool two = i + 1 < i1;
-    const double b0 = rhs_value(i);
-    const double b1 = two ? rhs_value(i + 1) : 0.0;
-    double h0 = b0, l0 = 0.0, h1 = b1, l1 = 0.0;
-    if (it > 0) {
-      const int mid = rowptr[i + 1];
-      const int s1 = two ? rowptr[i + 2] : mid;
-      for (int sb = rowptr[i]; sb < s1; sb += 8) {
-        double av[8], xv[8];
 #pragma unroll
-        for (int t = 0; t < 8; ++t) {
-          const bool v = sb + t < s1;
-          const int col = v ? cols[sb + t] : i;
-          av[t] = v ? a[sb + t] : 0.0;
-          xv[t] = uniform_ptr(x + (size_t)col * ncols)[c];
+  for (int t = 0; t < G; ++t) rr[t] = bb[t] = chi[t] = clo[t] = 0.0;
+  const bool cols_here = c0 < ncols;  // uniform per wavefront
+  for (int i0 = 0; i0 < n; i0 += RES_ROWS) {
+    int rb = 0;  // ring slot of row j: j - rb, minus ring when >= ring
+    if (use_ring) {
+      __syncthreads();  // the previous chunk's reads (and the zero fill) are done
+      const int lo = i0 == 0 ? 0 : i0 + b;
+      const int hi = min(n, i0 + RES_ROWS + b);
+      for (int e = tid; e < (hi - lo) * ncols; e += 256) {
+        const int q = e / ncols, c = e - q * ncols;
+        const int j = lo + q;
+        xr[(j % ring) * ldr + c] = x[(size_t)j * ncols + c];
+      }
+      rb = (max(0, i0 - b) / ring) * ring;
+      __syncthreads();
+    }
+    const int i = i0 + lane;
+    if (!cols_here || i >= n) continue;
+    double hi[G], lo[G];
+#pragma unroll
+    for (int t = 0; t < G; ++t) {
+      const int c = min(c0 + t, 63);
+      const double v = sh.colptr[c][(size_t)i * sh.colstride[c]] * sh.colscale[c];
+      bb[t] += v * v;
+      hi[t] = v;
+      lo[t] = 0.0;
+    }
+    if (it > 0) {
+      const int s1 = rowptr[i + 1];
+      for (int s = rowptr[i]; s < s1; ++s) {
+        const double av = -a[s];
+        const int j = cols[s];
+        double xv[G];
+        if (use_ring) {
+          int slot = j - rb;
+          slot = slot >= ring ? slot - ring : slot;
+          const double *src = xr + slot * ldr + c0;
+#pragma unroll
+          for (int t = 0; t < G; ++t) xv[t] = src[t];
+        } else {
+          const double *src = x + (size_t)j * ncols;
+#pragma unroll
+          for (int t = 0; t < G; ++t) xv[t] = c0 + t < ncols ? src[c0 + t] : 0.0;
         }
 #pragma unroll
-        for (int t = 0; t < 8; ++t) {
-          if (sb + t < mid)
-            dd_fma(h0, l0, -av[t], xv[t]);
-          else if (sb + t < s1)
-            dd_fma(h1, l1, -av[t], xv[t]);
+        for (int t = 0; t < G; ++t) dd_fma(hi[t], lo[t], av, xv[t]);
+      }
+      if (L.k) {
+        const double ci = cs * constraint_at(L, B, mac, i);
+        const double *xi = x + (size_t)i * ncols;
+#pragma unroll
+        for (int t = 0; t < G; ++t) {
+          const int c = min(c0 + t, 63);
+          dd_fma(hi[t], lo[t], -ci, sh.mu[c]);
+          dd_fma(chi[t], clo[t], ci, c0 + t < ncols ? xi[c0 + t] : 0.0);
         }
       }
     }
-    finish_row(i, b0, h0, l0);
-    if (two) finish_row(i + 1, b1, h1, l1);
+#pragma unroll
+    for (int t = 0; t < G; ++t) {
+      const double r = hi[t] + lo[t];
+      finite &= isfinite(r);
+      rr[t] += r * r;
+      if (!check_only && c0 + t < ncols) Y[(size_t)i * B.ldy + c0 + t] = T(ds[i] * r);
+    }
   }
-  part[wave][0][lane] = rr;
-  part[wave][1][lane] = bb;
-  part[wave][2][lane] = chi;
-  part[wave][3][lane] = clo;
-  part[wave][4][lane] = finite ? 0.0 : 1.0;
+  // per-column sums over the lanes (each wavefront owns its columns)
+  const double badw = wave_sum(finite ? 0.0 : 1.0);
+#pragma unroll
+  for (int t = 0; t < G; ++t) {
+    const double r2 = wave_sum(rr[t]);
+    const double b2 = wave_sum(bb[t]);
+    double h = chi[t], l = clo[t];
+    for (int off = 32; off > 0; off >>= 1) {
+      const double oh = __shfl_down(h, off), ol = __shfl_down(l, off);
+      double ss, ee;
+      two_sum(h, oh, ss, ee);
+      h = ss;
+      l = l + ol + ee;
+    }
+    if (lane == 0 && cols_here && c0 + t < ncols && t < cq) {
+      sh.tot[0][c0 + t] = r2;
+      sh.tot[1][c0 + t] = b2;
+      sh.tot[2][c0 + t] = h;
+      sh.tot[3][c0 + t] = l;
+      sh.bad[c0 + t] = badw;
+    }
+  }
   __syncthreads();
-  if (wave != 0 || skip) return;
-  double R2 = 0.0, B2 = 0.0, CH = 0.0, CL = 0.0, BAD = 0.0;
-  for (int w = 0; w < RES_WAVES; ++w) {
-    R2 += part[w][0][lane];
-    B2 += part[w][1][lane];
-    dd_add(CH, CL, part[w][2][lane]);
-    CL += part[w][3][lane];
-    BAD += part[w][4][lane];
-  }
+  if (!mine) return;
+  const int c = tid;
+  double R2 = sh.tot[0][c], B2 = sh.tot[1][c];
+  const double BAD = sh.bad[c];
   double rn = 0.0;
   if (L.k) {
     const double bn = rhs ? rhs[((size_t)mac * N + n) * ncols + c] : 0.0;
-    rn = bn - (CH + CL);
+    rn = bn - (sh.tot[2][c] + sh.tot[3][c]);
     R2 += rn * rn;
     B2 += bn * bn;
   }
@@ -1095,6 +1195,16 @@ __global__ void __launch_bounds__(64 * RES_WAVES)
     state[c] = ok ? 1 : 0;
     beta[(size_t)mac * ncols + c] = rn;
   }
+}
+
+// Column width per wavefront of the residual (compile-time G >= ceil(ncols/4)).
+static const int kResWidths[] = {2, 4, 6, 9, 12, 16};
+
+static int res_width(int ncols) {
+  const int cq = (ncols + 3) / 4;
+  for (int g : kResWidths)
+    if (cq <= g) return g;
+  return 0;
 }
 
 // x = D y (+ bordered correction when k == 1); it == 0 sets X, later
@@ -1983,6 +2093,39 @@ int mhm_batch_factor(mhm_batch *b, void *stream) {
   return MHM_OK;
 }
 
+// Residual launch: column width G per wavefront and the x ring when it fits in LDS.
+template <typename T>
+static int launch_residual(mhm_batch *b, int ncols, const double *rhs, const double *X, double rtol, double fwd,
+                           int it, int check_only, hipStream_t st) {
+  const LayoutK &L = b->layout->k;
+  const int G = res_width(ncols);
+  const int ldr = 4 * G + 1;  // odd row stride (in doubles) spreads the ring rows over the LDS banks
+  int ring = RES_ROWS + 2 * L.b;
+  size_t shm = (size_t)ring * ldr * sizeof(double);
+  if (shm + sizeof(ResShared) > (size_t)lds_limit(b->layout->device)) {
+    ring = 0;
+    shm = 0;
+  }
+#define MHM_RES(g)                                                                                          \
+  case g:                                                                                                   \
+    if (shm > 65536)                                                                                        \
+      HIP_TRY(hipFuncSetAttribute((const void *)k_residual<T, g>, hipFuncAttributeMaxDynamicSharedMemorySize, \
+                                  (int)shm));                                                               \
+    hipLaunchKernelGGL((k_residual<T, g>), dim3(b->M), dim3(256), shm, st, L, b->bk, ncols, rhs, X, rtol, fwd, \
+                       it, (T *)b->y, b->beta, b->colstate, (const double *)b->colnorm, check_only, L.rowptr,  \
+                       L.cols, (const double *)b->bk.aval, (const double *)b->bk.dscale,                     \
+                       b->layout->coupling_rm, ring, ldr);                                                  \
+    break;
+  switch (G) {
+    MHM_RES(2) MHM_RES(4) MHM_RES(6) MHM_RES(9) MHM_RES(12) MHM_RES(16)
+    default:
+      return fail(MHM_EINVAL, "too many columns for the residual kernel");
+  }
+#undef MHM_RES
+  LAUNCH_CHECK("k_residual");
+  return MHM_OK;
+}
+
 // Solve ncols columns into the row-major X[mac][row][col] (band rows) with
 // binary64 refinement; rhs == NULL selects the standard columns [f, B, A Z].
 template <typename T>
@@ -1993,8 +2136,6 @@ static int run_solve(mhm_batch *b, const double *rhs, double *X, int ncols, int 
   hipLaunchKernelGGL(k_fill_int, dim3((unsigned)(((size_t)b->M * ncols + 255) / 256)), dim3(256), 0, st,
                      b->colstate, 0, (size_t)b->M * ncols);
   hipLaunchKernelGGL(k_fill_int, dim3((b->M + 255) / 256), dim3(256), 0, st, b->bk.iters, 0, (size_t)b->M);
-  const dim3 rgrid((unsigned)b->M);
-  const mhm_layout *lay = b->layout;
   // u, w of fresh factors: two extra columns of the first solve when the
   // workspace has room, else a solve of their own before the refinement
   const bool uw_inline = b->uw_pending && ncols + 2 <= b->ldy;
@@ -2009,11 +2150,7 @@ static int run_solve(mhm_batch *b, const double *rhs, double *X, int ncols, int 
   for (int it = 0; it <= max_refine; ++it) {
     {
       TimerSpan span(MHM_TIMER_RESIDUAL, st);
-      hipLaunchKernelGGL((k_residual<T>), rgrid, dim3(64 * RES_WAVES), 0, st, L, b->bk, ncols, rhs,
-                         (const double *)X, rtol, fwd, it, (T *)b->y, b->beta, b->colstate,
-                         (const double *)b->colnorm, 0, L.rowptr, L.cols, (const double *)b->bk.aval,
-                         (const double *)b->bk.dscale, lay->coupling_rm);
-      LAUNCH_CHECK("k_residual");
+      if (int rc = launch_residual<T>(b, ncols, rhs, X, rtol, fwd, it, 0, st)) return rc;
     }
     if (it == 0 && uw_inline) {
       hipLaunchKernelGGL((k_load_uw<T>), grid2(L.n, 256, b->M), dim3(256), 0, st, L, b->bk, (T *)b->y, ncols);
@@ -2034,11 +2171,7 @@ static int run_solve(mhm_batch *b, const double *rhs, double *X, int ncols, int 
                        (const double *)b->beta, X, it, (const int *)b->colstate, b->colnorm);
     LAUNCH_CHECK("k_saddle_apply");
   }
-  hipLaunchKernelGGL((k_residual<T>), rgrid, dim3(64 * RES_WAVES), 0, st, L, b->bk, ncols, rhs,
-                     (const double *)X, rtol, 0.0, 1, (T *)b->y, b->beta, b->colstate, (const double *)b->colnorm,
-                     1, L.rowptr, L.cols, (const double *)b->bk.aval, (const double *)b->bk.dscale,
-                     lay->coupling_rm);
-  LAUNCH_CHECK("k_residual(check)");
+  if (int rc = launch_residual<T>(b, ncols, rhs, X, rtol, 0.0, 1, 1, st)) return rc;
   return MHM_OK;
 }
 
