@@ -83,6 +83,17 @@ __device__ __forceinline__ void dd_add(double &hi, double &lo, double v) {
   lo += e;
 }
 
+// Cross-lane hand-off through LDS inside one wavefront: the wavefront-scope
+// release/acquire fences keep the compiler from reusing or reordering LDS
+// values across it (lanes read what other lanes wrote), the wave barrier keeps
+// the scheduler from moving instructions over it.  LDS operations of one
+// wavefront execute in order, so no block barrier is needed.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 __device__ __forceinline__ double wave_sum(double v) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
   return v;
@@ -531,6 +542,188 @@ __global__ void __launch_bounds__(NT)
   if (tid == 0 && bad) atomicOr(&status[mac], bad);
 }
 
+// ----------------------------------------------------------------- blocked band Cholesky
+// Block per macro, panels of FPANEL columns.  The LDS window holds the
+// NB + b active band columns (slot c mod S, entries r = 0..b of column c at
+// rows c + r).  Per panel:
+//  (a) wavefront 0 factors the panel in registers: lane q holds panel rows q
+//      and q + 64 (PR = NB + b rows), column t is pivoted, scaled and applied
+//      to the panel columns to its right; pivots and multipliers pass through
+//      LDS inside the wavefront (no block barrier);
+//  (b) the panel's columns go to HBM as the column band (contiguous) and the
+//      row band (rband[R*W + r] = L(R, R-r), runs of NB per row);
+//  (c) the block applies the symmetric rank-NB update to the b trailing
+//      columns, 4x4 register tiles over the lower triangle;
+//  (d) the NB columns that enter next, prefetched into registers at the start
+//      of the panel, take the panel's slots.
+// Two block barriers per panel instead of two per column.
+#define FPANEL 16
+
+template <typename T>
+constexpr int panel_stride() {  // padded Lp row stride: b128 reads of consecutive rows hit distinct banks
+  return sizeof(T) == 8 ? FPANEL + 2 : FPANEL + 4;
+}
+
+static size_t factor_panel_lds(int b, size_t tsize) {
+  const size_t S = (size_t)FPANEL + b, W = (size_t)b + 1;
+  const size_t ldp = tsize == 8 ? FPANEL + 2 : FPANEL + 4;
+  return (S * W + S * ldp + 64) * tsize;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+    k_factor_panel(int n, int b, T *__restrict__ band, T *__restrict__ rband, int *__restrict__ status,
+                   double tiny) {
+  constexpr int NB = FPANEL;
+  constexpr int LDP = panel_stride<T>();
+  constexpr int NPF = 8;  // NB * W <= 256 * NPF, so W <= 128
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
+  const int W = b + 1;
+  const int S = NB + b;   // window columns
+  const int PR = NB + b;  // panel rows
+  T *win = reinterpret_cast<T *>(smem_raw);  // S x W
+  T *Lp = win + (size_t)S * W;               // PR x LDP: Lp[q*LDP + t] = L(j0 + q, j0 + t)
+  T *piv = Lp + (size_t)PR * LDP;            // pivot broadcast
+  __shared__ int bad;
+  const int mac = blockIdx.x, tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  T *Lg = band + (size_t)mac * n * W;
+  T *Rg = rband ? rband + (size_t)mac * n * W : nullptr;
+  if (tid == 0) bad = 0;
+  for (int e = tid; e < S * W; e += 256) {  // columns 0 .. S-1 in slots 0 .. S-1
+    const int c = e / W, r = e - c * W;
+    win[e] = (c + r < n) ? Lg[e] : T(0);
+  }
+  if (Rg)  // entries L(j, j-r) with j - r < 0
+    for (int e = tid; e < min(b, n) * W; e += 256) {
+      const int j = e / W, r = e - j * W;
+      if (r > j) Rg[e] = T(0);
+    }
+  const int side = (b + 3) / 4;
+  const int ntiles = side * (side + 1) / 2;
+  T pf[NPF];
+  __syncthreads();
+  for (int j0 = 0; j0 < n; j0 += NB) {
+    const int s0 = j0 % S;
+    const int np = min(NB, n - j0);
+    // columns j0 + S .. j0 + S + NB - 1 enter after this panel, into its slots
+#pragma unroll
+    for (int v = 0; v < NPF; ++v) {
+      const int e = tid + 256 * v;
+      const int c = j0 + S + e / W, r = e % W;
+      pf[v] = (e < NB * W && c + r < n) ? Lg[(size_t)(j0 + S) * W + e] : T(0);
+    }
+    // (a) panel factorisation, wavefront 0
+    if (wave == 0) {
+      const int q0 = lane, q1 = lane + 64;
+      T a0[NB], a1[NB];
+#pragma unroll
+      for (int t = 0; t < NB; ++t) {
+        int slot = s0 + t;
+        slot = slot >= S ? slot - S : slot;
+        const T *col = win + (size_t)slot * W;
+        a0[t] = (t < np && q0 >= t && q0 - t <= b) ? col[q0 - t] : T(0);
+        a1[t] = (t < np && q1 < PR && q1 - t <= b) ? col[q1 - t] : T(0);
+      }
+#pragma unroll
+      for (int t = 0; t < NB; ++t) {
+        if (t < np) {
+          if (lane == t) piv[0] = a0[t];
+          wave_lds_sync();
+          T d = piv[0];
+          if (!(d > T(tiny)) || !isfinite((double)d)) {
+            if (lane == 0) bad = MHM_STATUS_PIVOT;
+            d = T(1);
+          }
+          const T root = sqrt(d);
+          const T inv = T(1) / root;
+          a0[t] = q0 > t ? a0[t] * inv : (q0 == t ? root : T(0));
+          a1[t] = a1[t] * inv;  // q1 >= 64 > t
+          if (q0 < PR) Lp[q0 * LDP + t] = a0[t];
+          if (q1 < PR) Lp[q1 * LDP + t] = a1[t];
+          wave_lds_sync();
+#pragma unroll
+          for (int u = t + 1; u < NB; ++u) {
+            const T l = Lp[u * LDP + t];  // L(j0 + u, j0 + t), broadcast
+            a0[u] -= a0[t] * l;
+            a1[u] -= a1[t] * l;
+          }
+        } else {
+          if (q0 < PR) Lp[q0 * LDP + t] = T(0);
+          if (q1 < PR) Lp[q1 * LDP + t] = T(0);
+        }
+      }
+    }
+    __syncthreads();
+    // (b) the panel to HBM: column band (contiguous) and row band
+    for (int e = tid; e < np * W; e += 256) {
+      const int t = e / W, r = e - t * W;
+      Lg[(size_t)j0 * W + e] = Lp[(t + r) * LDP + t];
+    }
+    if (Rg)
+      for (int e = tid; e < PR * NB; e += 256) {
+        const int q = e / NB, t = e - q * NB;
+        const int R = j0 + q, r = q - t;
+        if (t < np && r >= 0 && r <= b && R < n) Rg[(size_t)R * W + r] = Lp[q * LDP + t];
+      }
+    // (c) trailing update: columns q_c in [NB, PR), rows q_r in [q_c, PR)
+    for (int id = tid; id < ntiles; id += 256) {
+      int tr = (int)((sqrtf(8.0f * id + 1.0f) - 1.0f) * 0.5f);
+      while ((tr + 1) * (tr + 2) / 2 <= id) ++tr;
+      while (tr * (tr + 1) / 2 > id) --tr;
+      const int tc = id - tr * (tr + 1) / 2;
+      const int qr0 = NB + 4 * tr, qc0 = NB + 4 * tc;
+      T acc[4][4];
+      int off[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        int slot = s0 + qc0 + j;
+        while (slot >= S) slot -= S;
+        off[j] = slot * W;
+      }
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const int qr = qr0 + i, qc = qc0 + j;
+          acc[i][j] = (qr < PR && qc < PR && qr >= qc) ? win[off[j] + (qr - qc)] : T(0);
+        }
+#pragma unroll 4
+      for (int t = 0; t < NB; ++t) {
+        T ar[4], bc[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) ar[i] = qr0 + i < PR ? Lp[(qr0 + i) * LDP + t] : T(0);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) bc[j] = qc0 + j < PR ? Lp[(qc0 + j) * LDP + t] : T(0);
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+          for (int j = 0; j < 4; ++j) acc[i][j] -= ar[i] * bc[j];
+      }
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const int qr = qr0 + i, qc = qc0 + j;
+          if (qr < PR && qc < PR && qr >= qc) win[off[j] + (qr - qc)] = acc[i][j];
+        }
+    }
+    __syncthreads();
+    // (d) entering columns take the panel's slots
+#pragma unroll
+    for (int v = 0; v < NPF; ++v) {
+      const int e = tid + 256 * v;
+      if (e < NB * W) {
+        const int t = e / W, r = e - t * W;
+        int slot = s0 + t;
+        slot = slot >= S ? slot - S : slot;
+        win[(size_t)slot * W + r] = pf[v];
+      }
+    }
+    __syncthreads();
+  }
+  if (tid == 0 && bad) atomicOr(&status[mac], bad);
+}
+
 // ----------------------------------------------------------------- band solves
 // Block per macro: forward L z = y (axpy form) then backward L^T x = z (dot
 // form, x_j = (z_j - L(j+1..j+b, j)^T x) / L_jj) for ncols columns of the
@@ -854,6 +1047,7 @@ __device__ __forceinline__ void band_sweep(int n_in, int W, int ldy, bool on, un
   T cb0[G], cb1[G];
   issue(0);
   commit(0);
+  wave_lds_sync();
   for (int q = 0; q < nchunks; ++q) {
     const int k0 = q * CH;
     issue(q + 1);
@@ -893,6 +1087,7 @@ __device__ __forceinline__ void band_sweep(int n_in, int W, int ldy, bool on, un
         if (on) dst[cl] = done[i];
       }
     commit((q + 1) & 1);
+    wave_lds_sync();  // other lanes read the staged band values
 #pragma unroll
     for (int i = 0; i < CH; ++i) rin[i] = rnext[i];
   }
@@ -1846,7 +2041,8 @@ int mhm_batch_create(const mhm_layout *L, int32_t macros, int32_t precision, int
   const int widest = std::max(std::max(1 + lk.m + lk.k, 2), max_columns);
   if (!solve_supported(lk.b, widest, tsize, limit) || !solve_supported(lk.b, 2, tsize, limit))
     return fail(MHM_EINVAL, "solve window exceeds the register or LDS budget");
-  if (factor_lds(lk.b, tsize) > (size_t)limit)
+  if (factor_lds(lk.b, tsize) > (size_t)limit &&
+      !(FPANEL + lk.b <= 127 && factor_panel_lds(lk.b, tsize) <= (size_t)limit))
     return fail(MHM_EINVAL, "bandwidth " + std::to_string(lk.b) +
                                 " too large for the LDS-resident factor window at this precision");
   DeviceGuard guard(dev);
@@ -2061,7 +2257,14 @@ static int run_factor(mhm_batch *b, hipStream_t st) {
   const double tiny = (double)L.n * (sizeof(T) == 8 ? 2.220446049250313e-16 : 1.1920929e-07);
   {
   TimerSpan span(MHM_TIMER_FACTOR, st);
-  if (L.b >= 12) {
+  const size_t pshm = factor_panel_lds(L.b, sizeof(T));
+  if (FPANEL + L.b <= 127 && pshm <= (size_t)lds_limit(lay->device)) {
+    if (pshm > 65536)
+      HIP_TRY(hipFuncSetAttribute((const void *)k_factor_panel<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)pshm));
+    hipLaunchKernelGGL((k_factor_panel<T>), dim3(b->M), dim3(256), pshm, st, L.n, L.b, (T *)b->band,
+                       (T *)b->rband, b->bk.status, tiny);
+  } else if (L.b >= 12) {
     if (shm > 65536)
       HIP_TRY(hipFuncSetAttribute((const void *)k_factor<T, 256>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
