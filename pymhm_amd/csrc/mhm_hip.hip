@@ -1221,7 +1221,8 @@ __global__ void __launch_bounds__(256)
                double rtol, double fwd_tol, int it, T *__restrict__ y, double *__restrict__ beta,
                int *__restrict__ colstate, const double *__restrict__ colnorm, int check_only,
                const int *__restrict__ rowptr, const int *__restrict__ cols, const double *__restrict__ aval,
-               const double *__restrict__ dscale, const double *__restrict__ coupling_rm, int ring, int ldr) {
+               const double *__restrict__ dscale, const double *__restrict__ coupling_rm, int ring, int ldr,
+               double *__restrict__ bnorm) {
   extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
   __shared__ ResShared sh;
   double *xr = reinterpret_cast<double *>(smem_raw);  // ring x ldr
@@ -1272,55 +1273,97 @@ __global__ void __launch_bounds__(256)
   const double *ds = dscale + (size_t)mac * n;
   const double cs = B.scal[(size_t)mac * NSCAL + S_CS];
   T *Y = y + (size_t)mac * n * B.ldy;
-  double rr[G], bb[G], chi[G], clo[G];
+  const bool cols_here = c0 < ncols;  // uniform per wavefront
+  auto rhs_value = [&](int i, int t) -> double {
+    const int c = min(c0 + t, 63);
+    return sh.colptr[c][(size_t)i * sh.colstride[c]] * sh.colscale[c];
+  };
+  double rr[G], chi[G], clo[G];
   bool finite = true;
 #pragma unroll
-  for (int t = 0; t < G; ++t) rr[t] = bb[t] = chi[t] = clo[t] = 0.0;
-  const bool cols_here = c0 < ncols;  // uniform per wavefront
-  for (int i0 = 0; i0 < n; i0 += RES_ROWS) {
-    int rb = 0;  // ring slot of row j: j - rb, minus ring when >= ring
-    if (use_ring) {
-      __syncthreads();  // the previous chunk's reads (and the zero fill) are done
-      const int lo = i0 == 0 ? 0 : i0 + b;
-      const int hi = min(n, i0 + RES_ROWS + b);
-      for (int e = tid; e < (hi - lo) * ncols; e += 256) {
-        const int q = e / ncols, c = e - q * ncols;
-        const int j = lo + q;
-        xr[(j % ring) * ldr + c] = x[(size_t)j * ncols + c];
+  for (int t = 0; t < G; ++t) rr[t] = chi[t] = clo[t] = 0.0;
+  if (it == 0) {
+    // x = 0: r = b; ||b||^2 is kept for the later passes of this solve
+    if (cols_here)
+      for (int i = lane; i < n; i += 64) {
+#pragma unroll
+        for (int t = 0; t < G; ++t) {
+          const double v = rhs_value(i, t);
+          finite &= isfinite(v);
+          rr[t] += v * v;
+          if (!check_only && c0 + t < ncols) Y[(size_t)i * B.ldy + c0 + t] = T(ds[i] * v);
+        }
       }
-      rb = (max(0, i0 - b) / ring) * ring;
-      __syncthreads();
+  } else {
+  // ring rows entering before chunk i0: [i0 + b, i0 + RES_ROWS + b) (all of
+  // [0, RES_ROWS + b) before the first); the next chunk's rows are loaded into
+  // registers during the current chunk and committed after it
+  constexpr int RPF = G;  // RES_ROWS * ncols <= RES_ROWS * 4G = 256 * G
+  double xpf[RPF];
+  auto ring_issue = [&](int lo, int hi) {
+#pragma unroll
+    for (int v = 0; v < RPF; ++v) {
+      const int e = tid + 256 * v;
+      const int q = e / ncols, c = e - q * ncols;
+      xpf[v] = (lo + q < hi) ? x[(size_t)(lo + q) * ncols + c] : 0.0;
     }
+  };
+  auto ring_commit = [&](int lo, int hi) {
+#pragma unroll
+    for (int v = 0; v < RPF; ++v) {
+      const int e = tid + 256 * v;
+      const int q = e / ncols, c = e - q * ncols;
+      if (lo + q < hi) xr[((lo + q) % ring) * ldr + c] = xpf[v];
+    }
+  };
+  if (use_ring) {
+    __syncthreads();  // zero fill done
+    for (int e = tid; e < min(n, RES_ROWS + b) * ncols; e += 256) {
+      const int q = e / ncols, c = e - q * ncols;
+      xr[q * ldr + c] = x[(size_t)q * ncols + c];  // rows < ring: slot = row
+    }
+    ring_issue(RES_ROWS + b, min(n, 2 * RES_ROWS + b));
+    __syncthreads();
+  }
+  for (int i0 = 0; i0 < n; i0 += RES_ROWS) {
+    const int rb = use_ring ? (max(0, i0 - b) / ring) * ring : 0;  // ring slot of row j: j - rb (- ring)
     const int i = i0 + lane;
-    if (!cols_here || i >= n) continue;
-    double hi[G], lo[G];
+    if (cols_here && i < n) {
+      double hi[G], lo[G];
 #pragma unroll
-    for (int t = 0; t < G; ++t) {
-      const int c = min(c0 + t, 63);
-      const double v = sh.colptr[c][(size_t)i * sh.colstride[c]] * sh.colscale[c];
-      bb[t] += v * v;
-      hi[t] = v;
-      lo[t] = 0.0;
-    }
-    if (it > 0) {
+      for (int t = 0; t < G; ++t) {
+        hi[t] = rhs_value(i, t);
+        lo[t] = 0.0;
+      }
       const int s1 = rowptr[i + 1];
-      for (int s = rowptr[i]; s < s1; ++s) {
-        const double av = -a[s];
-        const int j = cols[s];
-        double xv[G];
-        if (use_ring) {
-          int slot = j - rb;
-          slot = slot >= ring ? slot - ring : slot;
-          const double *src = xr + slot * ldr + c0;
+      for (int sb = rowptr[i]; sb < s1; sb += 4) {
+        // the chunk's CSR slots are loaded together, then applied in order
+        double av[4];
+        int jv[4];
 #pragma unroll
-          for (int t = 0; t < G; ++t) xv[t] = src[t];
-        } else {
-          const double *src = x + (size_t)j * ncols;
-#pragma unroll
-          for (int t = 0; t < G; ++t) xv[t] = c0 + t < ncols ? src[c0 + t] : 0.0;
+        for (int u = 0; u < 4; ++u) {
+          const bool v = sb + u < s1;
+          av[u] = v ? -a[sb + u] : 0.0;
+          jv[u] = v ? cols[sb + u] : i;
         }
 #pragma unroll
-        for (int t = 0; t < G; ++t) dd_fma(hi[t], lo[t], av, xv[t]);
+        for (int u = 0; u < 4; ++u) {
+          if (sb + u >= s1) break;
+          double xv[G];
+          if (use_ring) {
+            int slot = jv[u] - rb;
+            slot = slot >= ring ? slot - ring : slot;
+            const double *src = xr + slot * ldr + c0;
+#pragma unroll
+            for (int t = 0; t < G; ++t) xv[t] = src[t];
+          } else {
+            const double *src = x + (size_t)jv[u] * ncols;
+#pragma unroll
+            for (int t = 0; t < G; ++t) xv[t] = c0 + t < ncols ? src[c0 + t] : 0.0;
+          }
+#pragma unroll
+          for (int t = 0; t < G; ++t) dd_fma(hi[t], lo[t], av[u], xv[t]);
+        }
       }
       if (L.k) {
         const double ci = cs * constraint_at(L, B, mac, i);
@@ -1332,21 +1375,31 @@ __global__ void __launch_bounds__(256)
           dd_fma(chi[t], clo[t], ci, c0 + t < ncols ? xi[c0 + t] : 0.0);
         }
       }
-    }
 #pragma unroll
-    for (int t = 0; t < G; ++t) {
-      const double r = hi[t] + lo[t];
-      finite &= isfinite(r);
-      rr[t] += r * r;
-      if (!check_only && c0 + t < ncols) Y[(size_t)i * B.ldy + c0 + t] = T(ds[i] * r);
+      for (int t = 0; t < G; ++t) {
+        const double r = hi[t] + lo[t];
+        finite &= isfinite(r);
+        rr[t] += r * r;
+        if (!check_only && c0 + t < ncols) Y[(size_t)i * B.ldy + c0 + t] = T(ds[i] * r);
+      }
     }
+    if (use_ring && i0 + RES_ROWS < n) {
+      // rows of the next chunk into the slots of rows [i0 - b, i0 - b + RES_ROWS)
+      const int lo = i0 + RES_ROWS + b, hi = min(n, i0 + 2 * RES_ROWS + b);
+      __syncthreads();  // the chunk's ring reads are done
+      ring_commit(lo, hi);
+      ring_issue(hi, min(n, i0 + 3 * RES_ROWS + b));
+      __syncthreads();
+    }
+  }
   }
   // per-column sums over the lanes (each wavefront owns its columns)
   const double badw = wave_sum(finite ? 0.0 : 1.0);
 #pragma unroll
   for (int t = 0; t < G; ++t) {
     const double r2 = wave_sum(rr[t]);
-    const double b2 = wave_sum(bb[t]);
+    // ||b||^2: this pass's r = b when it == 0, else the value kept by that pass
+    const double b2 = it == 0 ? r2 : (c0 + t < ncols ? bnorm[(size_t)mac * ncols + c0 + t] : 0.0);
     double h = chi[t], l = clo[t];
     for (int off = 32; off > 0; off >>= 1) {
       const double oh = __shfl_down(h, off), ol = __shfl_down(l, off);
@@ -1358,6 +1411,7 @@ __global__ void __launch_bounds__(256)
     if (lane == 0 && cols_here && c0 + t < ncols && t < cq) {
       sh.tot[0][c0 + t] = r2;
       sh.tot[1][c0 + t] = b2;
+      if (it == 0 && !check_only) bnorm[(size_t)mac * ncols + c0 + t] = r2;
       sh.tot[2][c0 + t] = h;
       sh.tot[3][c0 + t] = l;
       sh.bad[c0 + t] = badw;
@@ -1717,7 +1771,7 @@ struct mhm_batch {
   size_t tsize = 8;
   BatchK bk{};
   void *band = nullptr, *rband = nullptr, *y = nullptr;  // rband: row-major band (wavefront solve)
-  double *xstd = nullptr, *beta = nullptr, *colnorm = nullptr;
+  double *xstd = nullptr, *beta = nullptr, *colnorm = nullptr, *bnorm = nullptr;
   double *rhsu = nullptr, *xu = nullptr;
   int *colstate = nullptr;
   double *ework = nullptr, *fwork = nullptr;
@@ -2075,7 +2129,7 @@ int mhm_batch_create(const mhm_layout *L, int32_t macros, int32_t precision, int
       (rc = dev_alloc(b->owned, &k.status, M)) || (rc = dev_alloc(b->owned, &k.iters, M)) ||
       (rc = dev_alloc(b->owned, &b->xstd, M * R * N)) ||
       (rc = dev_alloc(b->owned, &b->beta, M * ncm)) ||
-      (rc = dev_alloc(b->owned, &b->colnorm, M * ncm * 3)) ||
+      (rc = dev_alloc(b->owned, &b->colnorm, M * ncm * 3)) || (rc = dev_alloc(b->owned, &b->bnorm, M * ncm)) ||
       (rc = dev_alloc(b->owned, &b->colstate, M * ncm)))
     return cleanup(rc);
   if (max_columns > 0 &&
@@ -2317,7 +2371,7 @@ static int launch_residual(mhm_batch *b, int ncols, const double *rhs, const dou
     hipLaunchKernelGGL((k_residual<T, g>), dim3(b->M), dim3(256), shm, st, L, b->bk, ncols, rhs, X, rtol, fwd, \
                        it, (T *)b->y, b->beta, b->colstate, (const double *)b->colnorm, check_only, L.rowptr,  \
                        L.cols, (const double *)b->bk.aval, (const double *)b->bk.dscale,                     \
-                       b->layout->coupling_rm, ring, ldr);                                                  \
+                       b->layout->coupling_rm, ring, ldr, b->bnorm);                                        \
     break;
   switch (G) {
     MHM_RES(2) MHM_RES(4) MHM_RES(6) MHM_RES(9) MHM_RES(12) MHM_RES(16)
