@@ -225,3 +225,109 @@ class PolygonMesh:
                     if i + j < r - 1:
                         tris.append((local[i + 1, j], local[i + 1, j + 1], local[i, j + 1]))
         return TriangleMesh(np.asarray(pts), np.asarray(tris, dtype=np.int64))
+
+
+# ---------------------------------------------------------------- tetrahedra
+_TET_PAIRS = ((0, 1), (0, 2), (0, 3), (1, 2), (1, 3), (2, 3))
+_PERMUTATIONS_3 = ((0, 1, 2), (0, 2, 1), (1, 0, 2), (1, 2, 0), (2, 0, 1), (2, 1, 0))
+
+
+class TetraMesh:
+    """Straight tetrahedra with signed triangular face incidence.
+
+    Follows ``meshes/tetrahedron.py:31-92``: cells with a negative vertex
+    determinant swap local vertices 1 and 2; face ``s`` of a cell is opposite
+    local vertex ``s``, faces are numbered by first encounter over
+    ``(cell, side)`` with the sorted vertex triple as key; a face normal points
+    out of its first adjacent cell, which gets sign +1 (the second gets -1).
+    """
+
+    def __init__(self, points, cells):
+        points = np.array(points, dtype=float, copy=True)
+        cells = np.array(cells, dtype=np.int64, copy=True)
+        if points.ndim != 2 or points.shape[1] != 3 or cells.ndim != 2 or cells.shape[1] != 4:
+            raise ValueError("points (n,3) and cells (m,4) required")
+        edges = points[cells[:, 1:]] - points[cells[:, :1]]
+        det = np.linalg.det(edges)
+        scale = np.prod(np.linalg.norm(edges, axis=2), axis=1)
+        if np.any(np.abs(det) <= 32 * np.finfo(float).eps * scale):
+            raise ValueError("degenerate tetrahedron")
+        neg = det < 0
+        cells[neg, 1], cells[neg, 2] = cells[neg, 2].copy(), cells[neg, 1].copy()
+        self.points, self.cells = points, cells
+        self.volumes = np.abs(det) / 6
+        lookup, faces, owners = {}, [], []
+        self.cell_faces = np.empty((len(cells), 4), dtype=np.int64)
+        for c, ids in enumerate(cells):
+            for side in range(4):
+                key = tuple(sorted(int(v) for k, v in enumerate(ids) if k != side))
+                f = lookup.get(key)
+                if f is None:
+                    f = lookup[key] = len(faces)
+                    faces.append(key)
+                    owners.append([])
+                owners[f].append(c)
+                self.cell_faces[c, side] = f
+        self.faces = np.asarray(faces, dtype=np.int64)
+        self.face_cells = np.array([o + [-1] * (2 - len(o)) for o in owners], dtype=np.int64)
+        fp = points[self.faces]
+        cross = np.cross(fp[:, 1] - fp[:, 0], fp[:, 2] - fp[:, 0])
+        self.areas = np.linalg.norm(cross, axis=1) / 2
+        self.signs = np.empty((len(cells), 4), dtype=np.int64)
+        for f, adjacent in enumerate(owners):
+            for index, c in enumerate(adjacent):
+                self.signs[c, int(np.flatnonzero(self.cell_faces[c] == f)[0])] = 1 - 2 * index
+        self.boundary_faces = np.flatnonzero(self.face_cells[:, 1] < 0)
+
+    @classmethod
+    def unit_cube(cls, n: int) -> "TetraMesh":
+        """Six Freudenthal tetrahedra per lattice cube (``tetrahedron.py:95-108``):
+        for every cube corner (C order) and every axis permutation, the path
+        corner -> +e_p0 -> +e_p1 -> +e_p2."""
+        n = int(n)
+        shape = (n + 1,) * 3
+        points = np.indices(shape).reshape(3, -1).T / n
+        cells = []
+        for corner in np.ndindex((n,) * 3):
+            for perm in _PERMUTATIONS_3:
+                node = list(corner)
+                path = [np.ravel_multi_index(tuple(node), shape)]
+                for axis in perm:
+                    node[axis] += 1
+                    path.append(np.ravel_multi_index(tuple(node), shape))
+                cells.append(path)
+        return cls(points, np.asarray(cells))
+
+    def submesh(self, cell: int, refinement: int) -> "TetraMesh":
+        """Red refinement of one cell into refinement**3 tetrahedra
+        (``tetrahedron.py:111-153``), mapped affinely from barycentric points."""
+        bary, cells = red_refinement(int(refinement))
+        return TetraMesh(bary @ self.points[self.cells[cell]], cells)
+
+
+def red_refinement(refinement: int):
+    """Barycentric points and cells of ``log2(refinement)`` red-refinement
+    rounds of the reference tetrahedron (``tetrahedron.py:121-153``): edge
+    midpoints are numbered by first encounter over each cell's vertex pairs
+    (0,1),(0,2),(0,3),(1,2),(1,3),(2,3); every cell becomes four corner cells
+    and four octahedron cells split along the ab-cd diagonal."""
+    if refinement < 1 or refinement & (refinement - 1):
+        raise ValueError("tetrahedral refinement must be a power of two")
+    points = [row for row in np.eye(4)]
+    cells = [[0, 1, 2, 3]]
+    for _ in range(refinement.bit_length() - 1):
+        mid, out = {}, []
+        for a, b, c, d in cells:
+            ids = (a, b, c, d)
+            m = []
+            for i, j in _TET_PAIRS:
+                key = tuple(sorted((ids[i], ids[j])))
+                if key not in mid:
+                    mid[key] = len(points)
+                    points.append((points[ids[i]] + points[ids[j]]) / 2)
+                m.append(mid[key])
+            ab, ac, ad, bc, bd, cd = m
+            out += [[a, ab, ac, ad], [ab, b, bc, bd], [ac, bc, c, cd], [ad, bd, cd, d],
+                    [ab, ac, ad, cd], [ab, ac, bc, cd], [ab, ad, bd, cd], [ab, bc, bd, cd]]
+        cells = out
+    return np.asarray(points), np.asarray(cells, dtype=np.int64)

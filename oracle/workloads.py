@@ -12,8 +12,52 @@ from __future__ import annotations
 import numpy as np
 
 from .fem import FaceSpace, SkeletonSpace, scalar_operators, trace_coupling
+from .fem3d import cube_face_coupling, tetra_operators
 from .local import LocalProblem
-from .meshes import PolygonMesh
+from .meshes import PolygonMesh, TetraMesh
+
+
+def build_cube_problems(case, macros=None):
+    """LocalProblems of a cube-macro workload (tetrahedral P1/P2, piecewise
+    constant multipliers on ``split x split`` cells of every macroface).
+
+    Macro c = (i, j, k) in C order spans ``[ijk h, ijk h + h]``, its local
+    mesh is ``TetraMesh.unit_cube(r)`` scaled and translated; macrofaces are
+    numbered by first encounter over (macro, -x, +x, -y, +y, -z, +z), the
+    second macro meeting a face couples to it with sign -1.  Returns
+    ``([(c, LocalProblem, mesh)], trace_size)``.
+    """
+    prob = case.problem
+    nx, r, split, degree = prob.nx, prob.refinement, prob.split, prob.degree
+    h = 1.0 / nx
+    unit = TetraMesh.unit_cube(r)
+    ijk = list(np.ndindex((nx,) * 3))
+    faces, cell_faces, signs = {}, [], []
+    for t in ijk:
+        ids, sg = [], []
+        for axis in range(3):
+            for s in (0, 1):
+                key = (axis,) + tuple(t[a] + (s if a == axis else 0) for a in range(3))
+                sg.append(-1.0 if key in faces else 1.0)
+                faces.setdefault(key, len(faces))
+                ids.append(faces[key])
+        cell_faces.append(ids)
+        signs.append(sg)
+    nfc = split * split
+    trace_size = len(faces) * nfc
+    out = []
+    for c in (range(len(ijk)) if macros is None else macros):
+        lo = np.asarray(ijk[c], dtype=float) * h
+        mesh = TetraMesh(lo + unit.points * h, unit.cells)
+        fq = None if case.fq is None else case.fq[c]
+        A, M, f = tetra_operators(mesh, degree, diffusion_q=case.kq[c], source_q=fq,
+                                  source=case.f_const, order=prob.order)
+        B = cube_face_coupling(mesh, degree, lo, lo + h, split) * np.repeat(signs[c], nfc)[None, :]
+        Z = np.ones((A.shape[0], 1))
+        mz = M @ Z
+        tdofs = np.concatenate([fid * nfc + np.arange(nfc) for fid in cell_faces[c]])
+        out.append((c, LocalProblem(A, B, f, tdofs, Z, mz / (Z.T @ mz).item()), mesh))
+    return out, trace_size
 
 
 def build_problems(case, macros=None):
@@ -39,8 +83,11 @@ def build_problems(case, macros=None):
 
 def oracle_config(case, macros=None):
     """Blocks, loads and responses of the selected macros (default: all)."""
-    problems, skel = build_problems(case, macros)
-    trace_size = skel.size
+    if hasattr(case.problem, "split"):
+        problems, trace_size = build_cube_problems(case, macros)
+    else:
+        problems, skel = build_problems(case, macros)
+        trace_size = skel.size
     blocks, loads, src, lifts, ret, corr = [], [], [], [], [], []
     for c, p, _ in problems:
         r = p.condense()

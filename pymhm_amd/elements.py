@@ -18,7 +18,7 @@ import numpy as np
 from numpy.polynomial.legendre import leggauss
 
 TRIANGLE_EDGES = ((0, 1), (1, 2), (2, 0))
-TETRA_EDGES = ((0, 1), (1, 2), (2, 0), (0, 3), (1, 3), (2, 3))  # placeholder order, fixed below
+TETRA_EDGES = ((0, 1), (0, 2), (0, 3), (1, 2), (1, 3), (2, 3))  # combinations(range(4), 2)
 
 
 def triangle_rule(order: int):
@@ -37,7 +37,9 @@ def lattice_nodes(degree: int, dim: int = 2):
     """Barycentric multi-indices: vertices, then edge nodes along each oriented edge.
 
     Triangle edges (0,1), (1,2), (2,0) as ``src/pymhm/fem/scalar/triangle.py:14-24``;
-    interior nodes (degree >= 3) are not needed for P1/P2.
+    tetrahedron edges in ``combinations(range(4), 2)`` order as
+    ``src/pymhm/fem/scalar/tetrahedron_topology.py:27-45``; interior nodes
+    (degree >= 3) are not needed for P1/P2.
     """
     k = int(degree)
     edges = TRIANGLE_EDGES if dim == 2 else TETRA_EDGES
@@ -98,6 +100,38 @@ def triangle_tables(degree: int, bary):
     if degree not in (1, 2):
         raise ValueError("triangle tables support degree 1 or 2")
     return lagrange_tables(degree, bary, lattice_nodes(degree, 2))
+
+
+def tetra_rule(order: int):
+    """Duffy product Gauss rule on the tetrahedron (``fem/scalar/tetrahedron.py:41-51``):
+    barycentric points ``(q, 4)`` and unit-sum weights."""
+    x, w = leggauss(int(order))
+    x, w = (x + 1) / 2, w / 2
+    a, b, c = np.meshgrid(x, x, x, indexing="ij")
+    wa, wb, wc = np.meshgrid(w, w, w, indexing="ij")
+    xyz = np.array([a, (1 - a) * b, (1 - a) * (1 - b) * c]).reshape(3, -1).T
+    bary = np.column_stack((1 - xyz.sum(axis=1), xyz))
+    weights = (6 * wa * wb * wc * (1 - a) ** 2 * (1 - b)).ravel()
+    return bary, weights
+
+
+def tetra_tables(degree: int, bary):
+    """P1/P2 tetrahedron values ``(q, nb)`` and ``dphi/dlambda`` ``(q, nb, 4)``."""
+    if degree not in (1, 2):
+        raise ValueError("tetrahedron tables support degree 1 or 2")
+    return lagrange_tables(degree, bary, lattice_nodes(degree, 3))
+
+
+def tetra_gradients(points, cells):
+    """Gradients of lambda_0..lambda_3 ``(ne, 4, 3)`` and volumes: rows of the
+    inverse vertex Jacobian with lambda_0 the negative sum, volume |det|/6
+    (``meshes/tetrahedron.py:156-166,44-52``), so affine geometry enters
+    assembly with the reference's rounding."""
+    v = np.asarray(points, dtype=float)[cells]
+    edges = v[:, 1:] - v[:, :1]
+    inv = np.linalg.inv(edges.transpose(0, 2, 1))
+    grads = np.concatenate((-inv.sum(axis=1)[:, None], inv), axis=1)
+    return grads, np.abs(np.linalg.det(edges)) / 6
 
 
 def interval_tables(degree: int, s):

@@ -288,3 +288,114 @@ def quadrature_points(mesh: LocalMesh, order: int, offset=(0.0, 0.0)):
     """Physical quadrature points ``(ne, q, 2)`` of the element rule used in assembly."""
     bary, _ = elements.triangle_rule(order)
     return np.einsum("qi,tij->tqj", bary, mesh.points[mesh.cells]) + np.asarray(offset)
+
+
+# ----------------------------------------------------------------- tetrahedra
+_PERMS3 = ((0, 1, 2), (0, 2, 1), (1, 0, 2), (1, 2, 0), (2, 0, 1), (2, 1, 0))
+_TET_PAIRS = ((0, 1), (0, 2), (0, 3), (1, 2), (1, 3), (2, 3))
+
+
+def freudenthal_cube(r: int, lo, size: float):
+    """Points and positively oriented cells of ``r^3`` lattice cubes, six
+    Freudenthal tetrahedra each (``meshes/tetrahedron.py:95-108``: corners in C
+    order, axis permutations in lexicographic order, cells with a negative
+    determinant swap local vertices 1 and 2)."""
+    r = int(r)
+    shape = (r + 1,) * 3
+    pts = np.asarray(lo, dtype=float) + np.indices(shape).reshape(3, -1).T / r * float(size)
+    cells = []
+    for corner in np.ndindex((r,) * 3):
+        for perm in _PERMS3:
+            node = list(corner)
+            path = [np.ravel_multi_index(tuple(node), shape)]
+            for axis in perm:
+                node[axis] += 1
+                path.append(np.ravel_multi_index(tuple(node), shape))
+            cells.append(path)
+    cells = np.asarray(cells, dtype=np.int64)
+    det = np.linalg.det(pts[cells[:, 1:]] - pts[cells[:, :1]])
+    neg = det < 0
+    cells[neg, 1], cells[neg, 2] = cells[neg, 2].copy(), cells[neg, 1].copy()
+    return pts, cells
+
+
+@dataclass(frozen=True)
+class TetMesh:
+    points: np.ndarray
+    cells: np.ndarray
+    degree: int
+    dofs: np.ndarray
+    nodes: np.ndarray
+
+    @property
+    def n(self) -> int:
+        return len(self.nodes)
+
+
+def tet_local_mesh(points, cells, degree: int) -> TetMesh:
+    """P1/P2 numbering: vertices, then edges in sorted-pair order
+    (``fem/scalar/tetrahedron.py:54-66``)."""
+    pts = np.asarray(points, dtype=float)
+    cells = np.asarray(cells, dtype=np.int64)
+    if degree == 1:
+        return TetMesh(pts, cells, 1, cells.copy(), pts.copy())
+    if degree != 2:
+        raise ValueError("degree must be 1 or 2")
+    pairs = np.sort(cells[:, _TET_PAIRS].reshape(-1, 2), axis=1)
+    edges, inverse = np.unique(pairs, axis=0, return_inverse=True)
+    dofs = np.column_stack((cells, len(pts) + inverse.reshape(-1, 6)))
+    return TetMesh(pts, cells, 2, dofs, np.vstack((pts, pts[edges].mean(axis=1))))
+
+
+def tet_mass_ones(mesh: TetMesh):
+    """``M 1`` (integral of every basis function): P1 |T|/4; P2 vertex
+    functions -|T|/20, edge functions |T|/5."""
+    _, vol = elements.tetra_gradients(mesh.points, mesh.cells)
+    out = np.zeros(mesh.n)
+    if mesh.degree == 1:
+        np.add.at(out, mesh.dofs.ravel(), np.repeat(vol / 4.0, 4))
+    else:
+        np.add.at(out, mesh.dofs[:, :4].ravel(), np.repeat(-vol / 20.0, 4))
+        np.add.at(out, mesh.dofs[:, 4:].ravel(), np.repeat(vol / 5.0, 6))
+    return out
+
+
+def cube_coupling(mesh: TetMesh, lo, size: float, split: int):
+    """Coupling ``B (n, 6 split^2)`` of piecewise-constant multipliers on a
+    ``split x split`` grid of every face of the cube ``[lo, lo + size]^3``.
+
+    Faces -x, +x, -y, +y, -z, +z; a face cell's index is ``i + split * j``
+    along the face's lower then higher remaining axis.  On a boundary triangle
+    the integral of a P2 vertex function is 0 and of an edge function |T|/3
+    (P1: |T|/3 per vertex).
+    """
+    lo = np.asarray(lo, dtype=float)
+    size = float(size)
+    nb = mesh.dofs.shape[1]
+    B = np.zeros((mesh.n, 6 * split * split))
+    tol = 1e-12 * size
+    for c, (cell, dofs) in enumerate(zip(mesh.cells, mesh.dofs)):
+        for side in range(4):
+            loc = [k for k in range(4) if k != side]
+            p = mesh.points[cell[loc]]
+            for axis in range(3):
+                for s, bound in ((0, lo[axis]), (1, lo[axis] + size)):
+                    if not np.all(np.abs(p[:, axis] - bound) <= tol):
+                        continue
+                    others = [a for a in range(3) if a != axis]
+                    centre = p.mean(axis=0)
+                    ij = [min(split - 1, int((centre[a] - lo[a]) / size * split)) for a in others]
+                    col = (2 * axis + s) * split * split + ij[0] + split * ij[1]
+                    area = 0.5 * np.linalg.norm(np.cross(p[1] - p[0], p[2] - p[0]))
+                    if nb == 4:
+                        B[cell[loc], col] += area / 3.0
+                    else:
+                        for e, (i, j) in enumerate(_TET_PAIRS):
+                            if i != side and j != side:
+                                B[dofs[4 + e], col] += area / 3.0
+    return B
+
+
+def tet_quadrature_points(mesh: TetMesh, order: int, offset=(0.0, 0.0, 0.0)):
+    bary, _ = elements.tetra_rule(order)
+    return np.einsum("qi,tij->tqj", bary, mesh.points[mesh.cells]) + np.asarray(offset)

@@ -98,3 +98,86 @@ class SquareGridProblem:
         b = LocalBatch(self.layout, len(cm), precision=precision, max_columns=max_columns)
         b.set_coupling(colmap=cm, colscale=cs)
         return b
+
+
+class CubeGridProblem:
+    """Primal MHM scalar diffusion on an ``nx^3`` grid of cube macros.
+
+    Each macro is ``r^3`` lattice cubes of six Freudenthal P1/P2 tetrahedra
+    (``meshes/tetrahedron.py:95-108``); the multipliers are piecewise constant
+    on a ``split x split`` grid of every macroface.  All macros are translates
+    of one template (their binary-fraction coordinates make the geometry
+    tables identical), a face shared by two macros enters the second one (in
+    macro order) with sign -1, and global trace DOFs are numbered face by face
+    in order of first encounter over (macro, face -x, +x, -y, +y, -z, +z).
+    """
+
+    def __init__(self, nx: int, refinement: int, degree: int = 2, split: int = 2, order: int = 4,
+                 device: int = 0):
+        self.nx, self.refinement, self.degree, self.split = int(nx), int(refinement), int(degree), int(split)
+        self.order = max(int(order), self.degree + 2)
+        self.M = self.nx ** 3
+        self.h = 1.0 / self.nx
+        pts, cells = meshing.freudenthal_cube(self.refinement, (0.0, 0.0, 0.0), self.h)
+        self.mesh = mesh = meshing.tet_local_mesh(pts, cells, self.degree)
+        n = mesh.n
+        nfc = self.split * self.split
+        self.m = 6 * nfc
+        coupling = meshing.cube_coupling(mesh, (0.0, 0.0, 0.0), self.h, self.split)
+        mass1 = meshing.tet_mass_ones(mesh)
+        constraint = mass1 / mass1.sum()
+        bary, w = elements.tetra_rule(self.order)
+        vals, der = elements.tetra_tables(self.degree, bary)
+        glam, vol = elements.tetra_gradients(mesh.points, mesh.cells)
+        indptr, indices = element_pattern(n, mesh.dofs)
+        start = np.flatnonzero(np.abs(mesh.nodes[:, 0]) <= 1e-12 * self.h)
+        perm = choose_permutation(indptr, indices, start)
+        self.layout = LocalLayout(
+            n, perm, indptr, indices, m=self.m, k=1, coupling=coupling, constraint=constraint,
+            assembly=dict(dofs=mesh.dofs, dim=3, qweights=w, dtable=der, vtable=vals,
+                          grad_lambda=glam, measure=vol),
+            device=device)
+        self.coupling, self.constraint, self.qbary = coupling, constraint, bary
+        # macro faces: global ids by first encounter, signs, trace numbering
+        ijk = np.array(list(np.ndindex((self.nx,) * 3)))  # macro c = (i, j, k) in C order
+        index = {tuple(t): c for c, t in enumerate(ijk)}
+        face_id, signs = {}, np.ones((self.M, 6))
+        self.cell_faces = np.empty((self.M, 6), dtype=np.int64)
+        for c, t in enumerate(ijk):
+            for axis in range(3):
+                for s in (0, 1):
+                    key = (axis, *(t + (np.arange(3) == axis) * s))  # the face plane and its cell
+                    if key not in face_id:
+                        face_id[key] = len(face_id)
+                    else:
+                        signs[c, 2 * axis + s] = -1.0
+                    self.cell_faces[c, 2 * axis + s] = face_id[key]
+        del index
+        self.nfaces = len(face_id)
+        self.trace_size = self.nfaces * nfc
+        self.trace_dofs = (self.cell_faces[:, :, None] * nfc + np.arange(nfc)[None, None, :]).reshape(self.M, -1)
+        self.colmap = np.tile(np.arange(self.m, dtype=np.int32), (self.M, 1))
+        self.colscale = np.repeat(signs, nfc, axis=1)
+        self.offsets = ijk * self.h
+
+    @property
+    def n(self) -> int:
+        return self.mesh.n
+
+    def quadrature_points(self):
+        """Physical quadrature points ``(M, ne, nq, 3)`` of every macro."""
+        local = np.einsum("qi,tij->tqj", self.qbary, self.mesh.points[self.mesh.cells])
+        return local[None] + self.offsets[:, None, None, :]
+
+    def global_indices(self):
+        coarse = self.trace_size + np.arange(self.M, dtype=np.int64)
+        return np.concatenate((self.trace_dofs, coarse[:, None]), axis=1)
+
+    def kernel_offsets(self):
+        return self.trace_size + np.arange(self.M + 1, dtype=np.int64)
+
+    def batch(self, precision: str = "fp64", max_columns: int = 0, macros: slice | None = None):
+        sl = slice(None) if macros is None else macros
+        b = LocalBatch(self.layout, len(self.colmap[sl]), precision=precision, max_columns=max_columns)
+        b.set_coupling(colmap=self.colmap[sl], colscale=self.colscale[sl])
+        return b

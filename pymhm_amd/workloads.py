@@ -8,6 +8,14 @@
   P1 multipliers on 4 segments per macroface (m = 32), log K a Gaussian random
   field with exponential covariance (correlation length 0.05, variance 4,
   fixed seed) sampled at the 16 quadrature points of every triangle, f = 1.
+* ``config3``: nx^3 cube macros of r^3 lattice cubes x 6 Freudenthal P2
+  tetrahedra, piecewise-constant multipliers on 2 x 2 cells of every
+  macroface (m = 24), K a binary inclusion field (seeded random spheres of
+  radius 0.02 covering 20% of the volume, K = 1e-3 inside, 1 outside) at the
+  64 quadrature points of every tetrahedron, f = 1.  The full workload is
+  nx = 16, r = 4 (n = 729, bandwidth 189); the band kernels here take
+  bandwidths up to 95 (solve) and 111 (factor) on their fast paths, so r = 2
+  (n = 125, bandwidth 65) is the size run on them.
 
 Fields are generated here (seeded) because the reference holds no random
 field generator for these cases.
@@ -20,7 +28,7 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import meshing
-from .problems import SquareGridProblem
+from .problems import CubeGridProblem, SquareGridProblem
 
 
 @dataclass
@@ -71,6 +79,25 @@ def config2(nx: int = 64, seed: int = 20261203, device: int = 0, refinement: int
     q = prob.quadrature_points()
     logk = gaussian_field(q, 0.05, 4.0, seed)
     return Case("config2", prob, np.exp(logk), None, 1.0)
+
+
+def sphere_field(points, radius: float, fraction: float, seed: int, inside: float, outside: float):
+    """Binary inclusion field: ``inside`` within any of N seeded uniform random
+    spheres in [0, 1]^3 (N from the Boolean-model volume fraction
+    1 - exp(-N 4/3 pi r^3) = fraction), ``outside`` elsewhere."""
+    from scipy.spatial import cKDTree
+
+    count = int(round(-np.log(1.0 - fraction) / (4.0 / 3.0 * np.pi * radius ** 3)))
+    centres = np.random.default_rng(seed).random((count, 3))
+    p = np.asarray(points, dtype=float).reshape(-1, 3)
+    dist, _ = cKDTree(centres).query(p, distance_upper_bound=radius)
+    return np.where(np.isfinite(dist), inside, outside).reshape(np.shape(points)[:-1])
+
+
+def config3(nx: int = 16, refinement: int = 2, seed: int = 20261204, device: int = 0) -> Case:
+    prob = CubeGridProblem(nx, refinement, 2, split=2, order=4, device=device)
+    kq = sphere_field(prob.quadrature_points(), 0.02, 0.2, seed, 1e-3, 1.0)
+    return Case("config3", prob, kq, None, 1.0)
 
 
 def run_stage(case: Case, precision: str = "fp64", macros: slice | None = None):

@@ -56,14 +56,21 @@ def config2_small():
     return case, sel, ow.oracle_config(case, macros=sel)
 
 
-@pytest.mark.parametrize("which", ["config1", "config2"])
-def test_assembly_bit_exact(config1, config2_small, which):
+@pytest.fixture(scope="module")
+def config3_small():
+    case = pw.config3(nx=2, refinement=2)
+    print("config3 (2x2x2 cube macros of P2 tetrahedra): oracle", flush=True)
+    return case, ow.oracle_config(case)
+
+
+@pytest.mark.parametrize("which", ["config1", "config2", "config3"])
+def test_assembly_bit_exact(request, which):
     """Stiffness values and loads equal the oracle's assembly bit for bit."""
-    if which == "config1":
-        case, ref = config1
-        sel = list(range(case.problem.M))
+    if which == "config2":
+        case, sel, ref = request.getfixturevalue("config2_small")
     else:
-        case, sel, ref = config2_small
+        case, ref = request.getfixturevalue(which if which == "config1" else "config3_small")
+        sel = list(range(case.problem.M))
     prob = case.problem
     print(f"assembly {which}", flush=True)
     with prob.batch("fp64") as b:
@@ -97,6 +104,17 @@ def test_config2_stage(config2_small, precision):
     for key in ("blocks", "loads", "retained"):
         assert rel(out[key][sel], ref[key]) < TOL, key
     assert rel(responses(out)[sel], responses(ref)) < TOL
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_config3_stage(config3_small, precision):
+    """Tetrahedral P2 cube macros: blocks, loads, responses against the oracle."""
+    case, ref = config3_small
+    print(f"config3 {precision}", flush=True)
+    out = pw.run_stage(case, precision)
+    for key in ("blocks", "loads", "retained"):
+        assert rel(out[key], ref[key]) < TOL, key
+    assert rel(responses(out), responses(ref)) < TOL
 
 
 def test_results_independent_of_batch_composition(config2_small):
