@@ -17,7 +17,12 @@ checked against the FP64 oracle on the CPU-baseline sample; ``value`` is the
 macros/s of the fastest path whose blocks and loads meet 1e-10 relative error
 (``config.precision`` names it), and ``paths`` reports both.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+``--workload config3`` runs the three-dimensional stage instead: 16^3 cube
+macros, each 2^3 x 6 Freudenthal P2 tetrahedra (n = 125), P0 multipliers on
+2x2 cells of every cube face (m = 24), K = 1e-3 inside random spheres of
+radius 0.02 filling 20% of the volume and 1 elsewhere, f = 1.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload config2|config3]
 """
 
 from __future__ import annotations
@@ -44,6 +49,22 @@ HBM_PEAK_GBS = 8000.0
 PEAK_TFLOPS = {"f64": 78.6, "f32": 157.3}
 
 
+WORKLOADS = {
+    "config2": dict(nx=64, describe=lambda p: (
+        f"{p.nx}x{p.nx} square macros x (16x16x2 P2 triangles, n={p.n}), m={p.m} P1 multipliers, "
+        "log K Gaussian field (ell 0.05, var 4), f=1")),
+    "config3": dict(nx=16, describe=lambda p: (
+        f"{p.nx}^3 cube macros x (Freudenthal {p.refinement}^3x6 P2 tetrahedra, n={p.n}), "
+        f"m={p.m} P0 multipliers (2x2 cells per face), K=1e-3 in r=0.02 spheres at 20% volume, f=1")),
+}
+WAVE_WINDOWS = (8, 16, 24, 32, 48, 64, 68, 80, 96)
+
+
+def solve_window(b):
+    """Register window of the wavefront solve for half-bandwidth b (the library's table)."""
+    return next(w for w in WAVE_WINDOWS if b + 1 <= w)
+
+
 _LIMITS = None
 
 
@@ -61,11 +82,12 @@ def cpu_baseline(case, sample_macros, cores):
     """Oracle (FP64 NumPy/SciPy restatement) on a bounded sample, process pool."""
     from oracle import workloads as ow
 
+    spec_of = ow.cube_spec if hasattr(case.problem, "split") else ow.grid_spec
     chunks = [c for c in np.array_split(np.asarray(sample_macros), cores) if len(c)]
-    specs = [ow.grid_spec(case, c.tolist()) for c in chunks]
+    specs = [spec_of(case, c.tolist()) for c in chunks]
     with ProcessPoolExecutor(max_workers=cores, initializer=_cpu_worker_init) as pool:
         # warm-up: each worker builds its macro mesh once (mesh-only, untimed)
-        list(pool.map(ow.condense_spec, [ow.grid_spec(case, [int(c[0])]) for c in chunks]))
+        list(pool.map(ow.condense_spec, [spec_of(case, [int(c[0])]) for c in chunks]))
         t0 = time.perf_counter()
         results = list(pool.map(ow.condense_spec, specs))
         wall = time.perf_counter() - t0
@@ -149,7 +171,9 @@ def main(argv=None):
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--nx", type=int, default=64)
+    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="config2",
+                    help="config2: 2D square macros (headline); config3: 3D cube macros")
+    ap.add_argument("--nx", type=int, default=None, help="macros per axis (default 64 in 2D, 16 in 3D)")
     ap.add_argument("--cpu-sample", type=int, default=768)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--paths", default="mixed,fp64", help="precision paths to run (comma separated)")
@@ -165,7 +189,8 @@ def main(argv=None):
     from pymhm_amd import workloads as pw
 
     t0 = time.perf_counter()
-    case = pw.config2(nx=args.nx)
+    wl = WORKLOADS[args.workload]
+    case = getattr(pw, args.workload)(nx=args.nx or wl["nx"])
     prob = case.problem
     M = prob.M
     print(f"[bench] workload built in {time.perf_counter() - t0:.1f}s: {M} macros, n={prob.n}, "
@@ -229,7 +254,8 @@ def main(argv=None):
         per_launch_bytes = units * traffic["solve_bytes"]
     achieved = per_launch_bytes / avg_s / 1e9
     tname = "double" if path == "fp64" else "float"
-    kname = f"k_factor_panel<{tname}>" if dominant == "factor" else f"k_solve_wave<{tname}, 68>"
+    kname = (f"k_factor_panel<{tname}>" if dominant == "factor"
+             else f"k_solve_wave<{tname}, {solve_window(prob.layout.bandwidth)}>")
     out = {
         "metric": METRIC,
         "value": value,
@@ -244,8 +270,7 @@ def main(argv=None):
         "dtype": "f64" if path == "fp64" else "f32+f64",
         "data": "synthetic",
         "config": {
-            "workload": "64x64 square macros x (16x16x2 P2 triangles, n=1089), m=32 P1 multipliers, "
-                        "log K Gaussian field (ell 0.05, var 4), f=1",
+            "workload": wl["describe"](prob),
             "macros": M,
             "precision": ("fp64 (band Cholesky/solves and refinement in binary64)" if path == "fp64"
                           else "mixed (f32 band Cholesky/solves, f64 refinement)"),
@@ -265,8 +290,8 @@ def main(argv=None):
         "roofline": {
             "kernel": kname, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": (MEASURED_TRAFFIC_PER_MACRO[kname] * units if kname in MEASURED_TRAFFIC_PER_MACRO
-                        else None),
+            "traffic": (MEASURED_TRAFFIC_PER_MACRO[kname] * units
+                        if args.workload == "config2" and kname in MEASURED_TRAFFIC_PER_MACRO else None),
             "units_per_launch": units, "avg_launch_ms": avg_s * 1e3,
         },
         "cpu_baseline": cpu,

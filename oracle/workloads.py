@@ -116,12 +116,44 @@ def grid_spec(case, macros):
                 fq=None if case.fq is None else case.fq[macros], f_const=case.f_const)
 
 
+def cube_spec(case, macros):
+    """Picklable description of selected macros of a cube-grid case."""
+    prob = case.problem
+    macros = list(macros)
+    return dict(kind="cube", nx=prob.nx, refinement=prob.refinement, degree=prob.degree,
+                split=prob.split, order=prob.order, macros=macros, kq=case.kq[macros],
+                fq=None if case.fq is None else case.fq[macros], f_const=case.f_const)
+
+
+class _SpecCase:
+    """Minimal case object rebuilt from a spec inside a worker process."""
+
+    def __init__(self, spec, kq, fq):
+        class P:
+            pass
+        self.problem = P()
+        for key in ("nx", "refinement", "degree", "split", "order"):
+            setattr(self.problem, key, spec[key])
+        self.kq, self.fq, self.f_const = kq, fq, spec["f_const"]
+
+
 def condense_spec(spec):
     """Assemble and condense the macros of ``grid_spec`` (one worker's share).
 
     Same scope as the GPU stage: local mesh, operators, trace coupling,
     augmented SuperLU solve and condensed block/load per macro.
     """
+    if spec.get("kind") == "cube":
+        # the macro list is re-indexed 0..len-1 against the spec's own kq/fq rows
+        full_kq = {c: spec["kq"][r] for r, c in enumerate(spec["macros"])}
+        full_fq = None if spec["fq"] is None else {c: spec["fq"][r] for r, c in enumerate(spec["macros"])}
+        problems, trace_size = build_cube_problems(_SpecCase(spec, full_kq, full_fq), spec["macros"])
+        blocks, loads = [], []
+        for c, p, _ in problems:
+            _, blk, ld = p.condense().global_contribution([trace_size + c])
+            blocks.append(blk)
+            loads.append(ld)
+        return spec["macros"], np.array(blocks), np.array(loads)
     key = (spec["nx"], spec["breaks"], spec["degrees"])
     if key not in _MACRO_CACHE:   # macro mesh and skeleton are mesh-only: built once per worker
         pm = PolygonMesh.unit_square_grid(spec["nx"])

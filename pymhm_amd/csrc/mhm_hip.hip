@@ -244,33 +244,50 @@ __device__ __forceinline__ double kernel_at(const LayoutK &L, const BatchK &B, i
 }
 
 // ----------------------------------------------------------------- assembly
-// Neumaier step of fem/scalar/_integration.py:32-40 (magnitude-aware correction).
-__device__ __forceinline__ void neumaier(double &total, double &corr, double value) {
+// Compensated accumulation: total += value, corr += the exact rounding error of
+// that addition.  Knuth's TwoSum recovers the error without the magnitude test
+// of the Neumaier step (fem/scalar/_integration.py:32-40).  Both steps deliver
+// the exact error - a representable binary64, +0 when the sum is exact - so
+// for finite operands they leave identical totals and corrections.
+__device__ __forceinline__ void comp_add(double &total, double &corr, double value) {
 #pragma clang fp contract(off)
-  const double combined = total + value;
-  if (fabs(total) >= fabs(value))
-    corr += (total - combined) + value;
-  else
-    corr += (value - combined) + total;
-  total = combined;
+  const double s = total + value;
+  const double bv = s - total;
+  corr += (total - (s - bv)) + (value - bv);
+  total = s;
 }
 
-// One thread per (element, basis row i): row i of the element stiffness and
-// load entry i.  The arithmetic repeats the reference Gram kernel operation
-// for operation - physical gradients as the sequential sum
-// sum_i dphi/dlambda_i grad(lambda_i), K grad(phi_j) and the Cartesian
-// contraction compensated, ((c + corr) * w_q) * |T|, compensated quadrature
-// sum (fem/scalar/_integration.py:43-88) - with floating contraction off, so
-// the element matrices match a binary64 CPU evaluation bit for bit.  The load
-// is sum_q ((w_q phi_i) f_q) |T| in quadrature order.  Element matrices are
-// stored code-major, ework[mac][i*NB+j][e], so the stores coalesce.
+// Waves per block of the assembly kernel.
+#define ASM_WAVES 4
+
+// Element stiffness, one lane per (element, basis row i), 64 / NB elements per
+// wavefront; every lane owns row i of its element and the load entry i.
+//
+// The arithmetic repeats the reference Gram kernel operation for operation -
+// physical gradients as the sequential sum sum_l dphi/dlambda_l grad(lambda_l),
+// K grad(phi_j) (for K = k I the compensated sum over b of K[a][b] g[j][b]
+// reduces to the single rounded product k g[j][a]), the Cartesian contraction
+// compensated, ((c + corr) w_q) |T|, compensated quadrature sum
+// (fem/scalar/_integration.py:43-88) - with floating contraction off, so the
+// element matrices match a binary64 CPU evaluation bit for bit.  At every
+// quadrature point a lane forms only its own row's gradient and hands
+// k grad(phi_i) to the other lanes of its element through LDS (double-buffered
+// by point parity, one wavefront fence per point).  The first step of the
+// Cartesian sum starts from (0, 0), where the correction is exactly +0 for a
+// finite product.  The load is sum_q ((w_q phi_i) f_q) |T| in quadrature order.
+// Element matrices are stored code-major, ework[mac][i*NB+j][e].  A block keeps
+// the reference tables in LDS and walks macros blockIdx.y, + gridDim.y, ...
 template <int NB, int D>
-__global__ void __launch_bounds__(128)
-    k_assemble(LayoutK L, const double *__restrict__ kq, const double *__restrict__ fq,
+__global__ void __launch_bounds__(64 * ASM_WAVES)
+    k_assemble(LayoutK L, int M, const double *__restrict__ kq, const double *__restrict__ fq,
                double fconst, double *__restrict__ ework, double *__restrict__ fwork) {
 #pragma clang fp contract(off)
+  constexpr int EPW = 64 / NB;
+  constexpr int DP = D == 3 ? 4 : D;  // padded row of k grad(phi) in LDS
+  constexpr int KG = EPW * NB * DP;   // one buffer of a wavefront
   extern __shared__ __attribute__((aligned(16))) double sh[];
   const int nq = L.nq;
+  const int tab = (nq * NB * (D + 1) + nq * NB + nq + 1) & ~1;
   double *sd = sh;
   double *sv = sd + nq * NB * (D + 1);
   double *sw = sv + nq * NB;
@@ -278,50 +295,66 @@ __global__ void __launch_bounds__(128)
   for (int t = threadIdx.x; t < nq * NB; t += blockDim.x) sv[t] = L.vtab[t];
   for (int t = threadIdx.x; t < nq; t += blockDim.x) sw[t] = L.qw[t];
   __syncthreads();
-  const int tid = blockIdx.x * blockDim.x + threadIdx.x;
-  const int mac = blockIdx.y;
-  if (tid >= L.ne * NB) return;
-  const int e = tid / NB, i = tid - e * NB;
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int e0 = (blockIdx.x * ASM_WAVES + wv) * EPW;
+  if (e0 >= L.ne) return;  // whole wavefront past the last element
+  const int el = min(lane / NB, EPW - 1);
+  const int i = lane - el * NB < NB ? lane - el * NB : NB - 1;
+  const bool own = lane < EPW * NB && e0 + el < L.ne;
+  const int e = min(e0 + el, L.ne - 1);
+  double *kgb = sh + tab + wv * 2 * KG;
   double gl[D + 1][D];
 #pragma unroll
   for (int l = 0; l <= D; ++l)
 #pragma unroll
     for (int a = 0; a < D; ++a) gl[l][a] = L.glam[((size_t)e * (D + 1) + l) * D + a];
   const double meas = L.measure[e];
-  double tot[NB], cor[NB];
+  for (int mac = blockIdx.y; mac < M; mac += gridDim.y) {
+    double tot[NB], cor[NB];
 #pragma unroll
-  for (int j = 0; j < NB; ++j) tot[j] = cor[j] = 0.0;
-  double F = 0.0;
-  const double *kp = kq + ((size_t)mac * L.ne + e) * nq;
-  const double *fp = fq ? fq + ((size_t)mac * L.ne + e) * nq : nullptr;
-  for (int q = 0; q < nq; ++q) {
-    double g[NB][D];
-#pragma unroll
-    for (int b = 0; b < NB; ++b)
+    for (int j = 0; j < NB; ++j) tot[j] = cor[j] = 0.0;
+    double F = 0.0;
+    const double *kp = kq + ((size_t)mac * L.ne + e) * nq;
+    const double *fp = fq ? fq + ((size_t)mac * L.ne + e) * nq : nullptr;
+    double knext = kp[0];
+    double fnext = fp ? fp[0] : fconst;
+    for (int q = 0; q < nq; ++q) {
+      const double k = knext, f = fnext;
+      if (q + 1 < nq) {
+        knext = kp[q + 1];
+        if (fp) fnext = fp[q + 1];
+      }
+      const double *dq = sd + (q * NB + i) * (D + 1);
+      double gi[D];
 #pragma unroll
       for (int a = 0; a < D; ++a) {
-        double acc = sd[(q * NB + b) * (D + 1)] * gl[0][a];
+        double acc = dq[0] * gl[0][a];
 #pragma unroll
-        for (int l = 1; l <= D; ++l) acc = acc + sd[(q * NB + b) * (D + 1) + l] * gl[l][a];
-        g[b][a] = acc;
+        for (int l = 1; l <= D; ++l) acc = acc + dq[l] * gl[l][a];
+        gi[a] = acc;
       }
-    const double k = kp[q];
-    const double w = sw[q];
+      double *buf = kgb + (q & 1) * KG;
 #pragma unroll
-    for (int j = 0; j < NB; ++j) {
-      // K grad(phi_j) for K = k I: the compensated sum over b of K[a][b] g[j][b]
-      // reduces to the single rounded product k * g[j][a]
-      double ct = 0.0, cc = 0.0;
+      for (int a = 0; a < D; ++a) buf[(el * NB + i) * DP + a] = k * gi[a];
+      wave_lds_sync();
+      const double w = sw[q];
+      const double *row = buf + el * NB * DP;
 #pragma unroll
-      for (int a = 0; a < D; ++a) neumaier(ct, cc, g[i][a] * (k * g[j][a]));
-      neumaier(tot[j], cor[j], ((ct + cc) * w) * meas);
+      for (int j = 0; j < NB; ++j) {
+        double ct = 0.0 + gi[0] * row[j * DP], cc = 0.0;
+#pragma unroll
+        for (int a = 1; a < D; ++a) comp_add(ct, cc, gi[a] * row[j * DP + a]);
+        comp_add(tot[j], cor[j], ((ct + cc) * w) * meas);
+      }
+      F = F + ((w * sv[q * NB + i]) * f) * meas;
     }
-    F = F + ((w * sv[q * NB + i]) * (fp ? fp[q] : fconst)) * meas;
-  }
-  const size_t base = (size_t)mac * NB * NB * L.ne;
+    if (own) {
+      const size_t base = (size_t)mac * NB * NB * L.ne;
 #pragma unroll
-  for (int j = 0; j < NB; ++j) ework[base + (size_t)(i * NB + j) * L.ne + e] = tot[j] + cor[j];
-  fwork[(size_t)mac * NB * L.ne + (size_t)i * L.ne + e] = F;
+      for (int j = 0; j < NB; ++j) ework[base + (size_t)(i * NB + j) * L.ne + e] = tot[j] + cor[j];
+      fwork[(size_t)mac * NB * L.ne + (size_t)i * L.ne + e] = F;
+    }
+  }
 }
 
 // One thread per band row: ordered gather of the element contributions of each
@@ -2171,9 +2204,18 @@ template <int NB, int D>
 static int launch_assemble(const mhm_batch *b, const double *kq, const double *fq, double fc,
                            hipStream_t st) {
   const LayoutK &L = b->layout->k;
-  const size_t shm = ((size_t)L.nq * NB * (D + 1) + (size_t)L.nq * NB + L.nq) * sizeof(double);
-  hipLaunchKernelGGL((k_assemble<NB, D>), grid2((size_t)L.ne * NB, 128, b->M), dim3(128), shm, st, L, kq, fq, fc,
-                     b->ework, b->fwork);
+  constexpr int EPW = 64 / NB, DP = D == 3 ? 4 : D;
+  const size_t tab = ((size_t)L.nq * NB * (D + 1) + (size_t)L.nq * NB + L.nq + 1) & ~(size_t)1;
+  const size_t shm = (tab + (size_t)ASM_WAVES * 2 * EPW * NB * DP) * sizeof(double);
+  if (shm > 64 * 1024) {
+    if (shm > 160 * 1024) return fail(MHM_EINVAL, "assembly tables exceed the LDS of a workgroup");
+    HIP_TRY(hipFuncSetAttribute((const void *)k_assemble<NB, D>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)shm));
+  }
+  const int tasks = (L.ne + EPW - 1) / EPW;
+  const dim3 grid((unsigned)((tasks + ASM_WAVES - 1) / ASM_WAVES), (unsigned)std::min(b->M, 2048));
+  hipLaunchKernelGGL((k_assemble<NB, D>), grid, dim3(64 * ASM_WAVES), shm, st, L, b->M, kq, fq, fc, b->ework,
+                     b->fwork);
   LAUNCH_CHECK("k_assemble");
   return MHM_OK;
 }

@@ -112,3 +112,17 @@ def test_host_tables_equal_oracle():
     B = meshing.cube_coupling(host, (0.5, 0.25, 0.75), 0.25, 2)
     OB = fem3d.cube_face_coupling(oracle_mesh, 2, [0.5, 0.25, 0.75], [0.75, 0.5, 1.0], 2)
     assert np.abs(B - OB).max() < 1e-16
+
+
+def test_cube_spec_worker_matches_oracle_config():
+    """The picklable per-worker path of the CPU baseline gives the same blocks
+    and loads as the direct oracle evaluation."""
+    from oracle import workloads as ow
+    from pymhm_amd import workloads as pw
+
+    case = pw.config3(nx=2)
+    sample = [1, 6]
+    macros, blocks, loads = ow.condense_spec(ow.cube_spec(case, sample))
+    ref = ow.oracle_config(case, sample)
+    assert macros == sample
+    assert np.array_equal(blocks, ref["blocks"]) and np.array_equal(loads, ref["loads"])
