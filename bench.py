@@ -18,9 +18,10 @@ macros/s of the fastest path whose blocks and loads meet 1e-10 relative error
 (``config.precision`` names it), and ``paths`` reports both.
 
 ``--workload config3`` runs the three-dimensional stage instead: 16^3 cube
-macros, each 2^3 x 6 Freudenthal P2 tetrahedra (n = 125), P0 multipliers on
-2x2 cells of every cube face (m = 24), K = 1e-3 inside random spheres of
-radius 0.02 filling 20% of the volume and 1 elsewhere, f = 1.
+macros, each 4^3 x 6 Freudenthal P2 tetrahedra (n = 729, bandwidth 189 after
+the band ordering), P0 multipliers on 2x2 cells of every cube face (m = 24),
+K = 1e-3 inside random spheres of radius 0.02 filling 20% of the volume and 1
+elsewhere, f = 1.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload config2|config3]
 """
@@ -50,19 +51,33 @@ PEAK_TFLOPS = {"f64": 78.6, "f32": 157.3}
 
 
 WORKLOADS = {
-    "config2": dict(nx=64, describe=lambda p: (
+    "config2": dict(nx=64, refinement=16, sample=768, describe=lambda p: (
         f"{p.nx}x{p.nx} square macros x (16x16x2 P2 triangles, n={p.n}), m={p.m} P1 multipliers, "
         "log K Gaussian field (ell 0.05, var 4), f=1")),
-    "config3": dict(nx=16, describe=lambda p: (
+    "config3": dict(nx=16, refinement=4, sample=256, describe=lambda p: (
         f"{p.nx}^3 cube macros x (Freudenthal {p.refinement}^3x6 P2 tetrahedra, n={p.n}), "
         f"m={p.m} P0 multipliers (2x2 cells per face), K=1e-3 in r=0.02 spheres at 20% volume, f=1")),
 }
 WAVE_WINDOWS = (8, 16, 24, 32, 48, 64, 68, 80, 96)
+BLOCK_SLOTS = (4, 8, 12, 16, 24, 32, 64)
 
 
-def solve_window(b):
-    """Register window of the wavefront solve for half-bandwidth b (the library's table)."""
-    return next(w for w in WAVE_WINDOWS if b + 1 <= w)
+def factor_kernel(b, tname):
+    """The factor kernel the library picks for half-bandwidth b: the LDS window
+    while 16 + b <= 127, the in-place wide-band kernel beyond."""
+    return f"k_factor_panel<{tname}>" if 16 + b <= 127 else f"k_factor_wide<{tname}>"
+
+
+def solve_kernel(b, ncols, tname):
+    """The solve kernel for half-bandwidth b and ncols columns: the wavefront
+    solve with the smallest register window >= b + 1, else the block solve
+    with its register slots per thread (the library's tables)."""
+    w = next((w for w in WAVE_WINDOWS if b + 1 <= w), None)
+    if w is not None and ncols <= 64:
+        return f"k_solve_wave<{tname}, {w}>"
+    nt = 256 if b >= 12 else 64
+    slots = -(-(b + 1) // (nt // ncols))
+    return f"k_solve<{tname}, {nt}, {next(s for s in BLOCK_SLOTS if slots <= s)}>"
 
 
 _LIMITS = None
@@ -174,7 +189,10 @@ def main(argv=None):
     ap.add_argument("--workload", choices=sorted(WORKLOADS), default="config2",
                     help="config2: 2D square macros (headline); config3: 3D cube macros")
     ap.add_argument("--nx", type=int, default=None, help="macros per axis (default 64 in 2D, 16 in 3D)")
-    ap.add_argument("--cpu-sample", type=int, default=768)
+    ap.add_argument("--refinement", type=int, default=None,
+                    help="config3 only: tetrahedral cubes per macro edge (default 4)")
+    ap.add_argument("--cpu-sample", type=int, default=None,
+                    help="macros in the oracle CPU-baseline sample (default 768 in 2D, 256 in 3D)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--paths", default="mixed,fp64", help="precision paths to run (comma separated)")
     ap.add_argument("--copy-bytes", type=int, default=0,
@@ -190,14 +208,17 @@ def main(argv=None):
 
     t0 = time.perf_counter()
     wl = WORKLOADS[args.workload]
-    case = getattr(pw, args.workload)(nx=args.nx or wl["nx"])
+    kw = dict(nx=args.nx or wl["nx"])
+    if args.workload == "config3":
+        kw["refinement"] = args.refinement or wl["refinement"]
+    case = getattr(pw, args.workload)(**kw)
     prob = case.problem
     M = prob.M
     print(f"[bench] workload built in {time.perf_counter() - t0:.1f}s: {M} macros, n={prob.n}, "
           f"m={prob.m}, bandwidth={prob.layout.bandwidth}", file=sys.stderr, flush=True)
 
     cpu = None
-    sample = np.unique(np.linspace(0, M - 1, min(args.cpu_sample, M)).astype(np.int64)).tolist()
+    sample = np.unique(np.linspace(0, M - 1, min(args.cpu_sample or wl["sample"], M)).astype(np.int64)).tolist()
     if not args.no_cpu:
         cores = max(1, min(16, len(os.sched_getaffinity(0))))
         wall, oblocks, oloads = cpu_baseline(case, sample, cores)
@@ -254,8 +275,9 @@ def main(argv=None):
         per_launch_bytes = units * traffic["solve_bytes"]
     achieved = per_launch_bytes / avg_s / 1e9
     tname = "double" if path == "fp64" else "float"
-    kname = (f"k_factor_panel<{tname}>" if dominant == "factor"
-             else f"k_solve_wave<{tname}, {solve_window(prob.layout.bandwidth)}>")
+    lay = prob.layout
+    kname = (factor_kernel(lay.bandwidth, tname) if dominant == "factor"
+             else solve_kernel(lay.bandwidth, 1 + lay.m + lay.k + 2, tname))
     out = {
         "metric": METRIC,
         "value": value,

@@ -757,6 +757,157 @@ __global__ void __launch_bounds__(256)
   if (tid == 0 && bad) atomicOr(&status[mac], bad);
 }
 
+// ----------------------------------------------------------------- wide-band Cholesky
+// Bands too wide for the LDS window of k_factor_panel (NB + b > 127): the same
+// blocked right-looking algorithm with the band array itself as the window.
+// Per panel of FPANEL columns:
+//  (a) wavefront 0 reads the panel (PR = NB + b <= 256 rows, four per lane)
+//      from the band, factors it in registers and leaves the multipliers in
+//      LDS, transposed (LpT[t * LDQ + q] = L(j0 + q, j0 + t));
+//  (b) the panel goes back to the column band (and to the row band if one is
+//      kept);
+//  (c) the block applies the rank-NB update to the trailing triangle in place
+//      in the band, 4x4 register tiles; consecutive threads take consecutive
+//      row tiles of a tile column, so their band reads and writes are
+//      contiguous and their LDS reads of LpT are conflict-free.
+// Block barriers order the in-place updates between panels.
+#define WIDE_ROWS 4  // panel rows per lane of wavefront 0
+
+static int wide_ldq(int b) { return ((FPANEL + b + 3) & ~3) + 4; }
+
+static size_t factor_wide_lds(int b, size_t tsize) {
+  return ((size_t)FPANEL * wide_ldq(b) + 64) * tsize;
+}
+
+static bool factor_wide_ok(int b, size_t tsize, int limit) {
+  return FPANEL + b <= 64 * WIDE_ROWS && factor_wide_lds(b, tsize) <= (size_t)limit;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+    k_factor_wide(int n, int b, int ldq, T *__restrict__ band, T *__restrict__ rband, int *__restrict__ status,
+                  double tiny) {
+  constexpr int NB = FPANEL, NR = WIDE_ROWS;
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
+  const int W = b + 1, PR = NB + b;
+  T *LpT = reinterpret_cast<T *>(smem_raw);  // NB x ldq
+  T *piv = LpT + (size_t)NB * ldq;
+  __shared__ int bad;
+  const int mac = blockIdx.x, tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  T *Lg = band + (size_t)mac * n * W;
+  T *Rg = rband ? rband + (size_t)mac * n * W : nullptr;
+  if (tid == 0) bad = 0;
+  if (Rg)  // entries L(j, j-r) with j - r < 0
+    for (int e = tid; e < min(b, n) * W; e += 256) {
+      const int j = e / W, r = e - j * W;
+      if (r > j) Rg[e] = T(0);
+    }
+  const int side = (b + 3) / 4;  // tile rows/columns of the trailing triangle
+  const int ntiles = side * (side + 1) / 2;
+  __syncthreads();
+  for (int j0 = 0; j0 < n; j0 += NB) {
+    const int np = min(NB, n - j0);
+    // (a) panel factorisation, wavefront 0
+    if (wave == 0) {
+      T a[NR][NB];
+#pragma unroll
+      for (int v = 0; v < NR; ++v)
+#pragma unroll
+        for (int t = 0; t < NB; ++t) {
+          const int q = lane + 64 * v;
+          a[v][t] = (t < np && q >= t && q - t <= b && j0 + q < n) ? Lg[(size_t)(j0 + t) * W + (q - t)] : T(0);
+        }
+#pragma unroll
+      for (int t = 0; t < NB; ++t) {
+        if (t < np) {
+          if (lane == t) piv[0] = a[0][t];
+          wave_lds_sync();
+          T d = piv[0];
+          if (!(d > T(tiny)) || !isfinite((double)d)) {
+            if (lane == 0) bad = MHM_STATUS_PIVOT;
+            d = T(1);
+          }
+          const T root = sqrt(d);
+          const T inv = T(1) / root;
+          a[0][t] = lane > t ? a[0][t] * inv : (lane == t ? root : T(0));
+#pragma unroll
+          for (int v = 1; v < NR; ++v) a[v][t] = a[v][t] * inv;  // rows >= 64 > t
+#pragma unroll
+          for (int v = 0; v < NR; ++v)
+            if (lane + 64 * v < PR) LpT[t * ldq + lane + 64 * v] = a[v][t];
+          wave_lds_sync();
+#pragma unroll
+          for (int u = t + 1; u < NB; ++u) {
+            const T l = LpT[t * ldq + u];  // L(j0 + u, j0 + t), broadcast
+#pragma unroll
+            for (int v = 0; v < NR; ++v) a[v][u] -= a[v][t] * l;
+          }
+        } else {
+#pragma unroll
+          for (int v = 0; v < NR; ++v)
+            if (lane + 64 * v < PR) LpT[t * ldq + lane + 64 * v] = T(0);
+        }
+      }
+    }
+    __syncthreads();
+    // (b) the panel to the column band and the row band
+    for (int e = tid; e < np * W; e += 256) {
+      const int t = e / W, r = e - t * W;
+      Lg[(size_t)j0 * W + e] = LpT[t * ldq + t + r];
+    }
+    if (Rg)
+      for (int e = tid; e < PR * NB; e += 256) {
+        const int q = e / NB, t = e - q * NB;
+        const int R = j0 + q, r = q - t;
+        if (t < np && r >= 0 && r <= b && R < n) Rg[(size_t)R * W + r] = LpT[t * ldq + q];
+      }
+    // (c) trailing update: columns qc in [NB, PR), rows qr in [qc, PR) with
+    // j0 + qr < n; tiles numbered down the tile columns of the lower triangle
+    for (int id = tid; id < ntiles; id += 256) {
+      const float s2 = 2.0f * side + 1.0f;
+      int tc = (int)((s2 - sqrtf(s2 * s2 - 8.0f * id)) * 0.5f);
+      tc = max(0, min(tc, side - 1));
+      while (tc > 0 && tc * side - tc * (tc - 1) / 2 > id) --tc;
+      while (tc + 1 < side && (tc + 1) * side - (tc + 1) * tc / 2 <= id) ++tc;
+      const int tr = tc + (id - (tc * side - tc * (tc - 1) / 2));
+      const int qr0 = NB + 4 * tr, qc0 = NB + 4 * tc;
+      T acc[4][4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const T *col = Lg + (size_t)(j0 + qc0 + j) * W - (qc0 + j);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int qr = qr0 + i, qc = qc0 + j;
+          acc[i][j] = (qr < PR && qr >= qc && j0 + qr < n) ? col[qr] : T(0);
+        }
+      }
+#pragma unroll 4
+      for (int t = 0; t < NB; ++t) {
+        T ar[4], bc[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) ar[i] = LpT[t * ldq + qr0 + i];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) bc[j] = LpT[t * ldq + qc0 + j];
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+          for (int j = 0; j < 4; ++j) acc[i][j] -= ar[i] * bc[j];
+      }
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        T *col = Lg + (size_t)(j0 + qc0 + j) * W - (qc0 + j);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int qr = qr0 + i, qc = qc0 + j;
+          if (qr < PR && qr >= qc && j0 + qr < n) col[qr] = acc[i][j];
+        }
+      }
+    }
+    __syncthreads();
+  }
+  if (tid == 0 && bad) atomicOr(&status[mac], bad);
+}
+
 // ----------------------------------------------------------------- band solves
 // Block per macro: forward L z = y (axpy form) then backward L^T x = z (dot
 // form, x_j = (z_j - L(j+1..j+b, j)^T x) / L_jj) for ncols columns of the
@@ -2129,7 +2280,8 @@ int mhm_batch_create(const mhm_layout *L, int32_t macros, int32_t precision, int
   if (!solve_supported(lk.b, widest, tsize, limit) || !solve_supported(lk.b, 2, tsize, limit))
     return fail(MHM_EINVAL, "solve window exceeds the register or LDS budget");
   if (factor_lds(lk.b, tsize) > (size_t)limit &&
-      !(FPANEL + lk.b <= 127 && factor_panel_lds(lk.b, tsize) <= (size_t)limit))
+      !(FPANEL + lk.b <= 127 && factor_panel_lds(lk.b, tsize) <= (size_t)limit) &&
+      !factor_wide_ok(lk.b, tsize, limit))
     return fail(MHM_EINVAL, "bandwidth " + std::to_string(lk.b) +
                                 " too large for the LDS-resident factor window at this precision");
   DeviceGuard guard(dev);
@@ -2360,6 +2512,13 @@ static int run_factor(mhm_batch *b, hipStream_t st) {
                                   (int)pshm));
     hipLaunchKernelGGL((k_factor_panel<T>), dim3(b->M), dim3(256), pshm, st, L.n, L.b, (T *)b->band,
                        (T *)b->rband, b->bk.status, tiny);
+  } else if (factor_wide_ok(L.b, sizeof(T), lds_limit(lay->device))) {
+    const size_t wshm = factor_wide_lds(L.b, sizeof(T));
+    if (wshm > 65536)
+      HIP_TRY(hipFuncSetAttribute((const void *)k_factor_wide<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)wshm));
+    hipLaunchKernelGGL((k_factor_wide<T>), dim3(b->M), dim3(256), wshm, st, L.n, L.b, wide_ldq(L.b),
+                       (T *)b->band, (T *)b->rband, b->bk.status, tiny);
   } else if (L.b >= 12) {
     if (shm > 65536)
       HIP_TRY(hipFuncSetAttribute((const void *)k_factor<T, 256>,

@@ -13,9 +13,9 @@
   macroface (m = 24), K a binary inclusion field (seeded random spheres of
   radius 0.02 covering 20% of the volume, K = 1e-3 inside, 1 outside) at the
   64 quadrature points of every tetrahedron, f = 1.  The full workload is
-  nx = 16, r = 4 (n = 729, bandwidth 189); the band kernels here take
-  bandwidths up to 95 (solve) and 111 (factor) on their fast paths, so r = 2
-  (n = 125, bandwidth 65) is the size run on them.
+  nx = 16, r = 4 (n = 729, bandwidth 189, factored by the wide-band kernel
+  and solved by the block solve); r = 2 (n = 125, bandwidth 65) is the small
+  size that fits the LDS-window factor and the wavefront solve.
 
 Fields are generated here (seeded) because the reference holds no random
 field generator for these cases.
@@ -94,7 +94,7 @@ def sphere_field(points, radius: float, fraction: float, seed: int, inside: floa
     return np.where(np.isfinite(dist), inside, outside).reshape(np.shape(points)[:-1])
 
 
-def config3(nx: int = 16, refinement: int = 2, seed: int = 20261204, device: int = 0) -> Case:
+def config3(nx: int = 16, refinement: int = 4, seed: int = 20261204, device: int = 0) -> Case:
     prob = CubeGridProblem(nx, refinement, 2, split=2, order=4, device=device)
     kq = sphere_field(prob.quadrature_points(), 0.02, 0.2, seed, 1e-3, 1.0)
     return Case("config3", prob, kq, None, 1.0)

@@ -117,6 +117,28 @@ def test_config3_stage(config3_small, precision):
     assert rel(responses(out), responses(ref)) < TOL
 
 
+@pytest.fixture(scope="module")
+def config3_wide():
+    case = pw.config3(nx=2, refinement=4)
+    print(f"config3 at 4^3 x 6 tetrahedra per cube (bandwidth {case.problem.layout.bandwidth}): oracle",
+          flush=True)
+    return case, ow.oracle_config(case)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_config3_wide_band_stage(config3_wide, precision):
+    """Cube macros of 4^3 x 6 P2 tetrahedra (n = 729, bandwidth 189): the band
+    is too wide for the LDS window, so the in-place wide-band factor and the
+    block solve run; blocks, loads and responses against the oracle."""
+    case, ref = config3_wide
+    assert case.problem.layout.bandwidth + 16 > 127
+    print(f"config3 wide {precision}", flush=True)
+    out = pw.run_stage(case, precision)
+    for key in ("blocks", "loads", "retained"):
+        assert rel(out[key], ref[key]) < TOL, key
+    assert rel(responses(out), responses(ref)) < TOL
+
+
 def test_results_independent_of_batch_composition(config2_small):
     """A macro's result is bitwise the same alone and inside a larger batch."""
     case, _, _ = config2_small

@@ -120,7 +120,7 @@ def test_cube_spec_worker_matches_oracle_config():
     from oracle import workloads as ow
     from pymhm_amd import workloads as pw
 
-    case = pw.config3(nx=2)
+    case = pw.config3(nx=2, refinement=2)
     sample = [1, 6]
     macros, blocks, loads = ow.condense_spec(ow.cube_spec(case, sample))
     ref = ow.oracle_config(case, sample)
