@@ -94,6 +94,31 @@ __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Value of v in lane l (wave-uniform l) as a scalar: v_readlane, no LDS.
+__device__ __forceinline__ double lane_bcast(double v, int l) {
+  const long long x = __double_as_longlong(v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)x, l);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(x >> 32), l);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+__device__ __forceinline__ float lane_bcast(float v, int l) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+
+// 1/sqrt(d) for a positive normal d: the hardware estimate refined by Newton
+// steps y <- y + y (1 - d y^2) / 2 (two in binary64, one in binary32), within
+// a few ulp; the factorisations take root = d / sqrt(d) = d * y.
+__device__ __forceinline__ double rsqrt_nr(double d) {
+  double y = __builtin_amdgcn_rsq(d);
+#pragma unroll
+  for (int i = 0; i < 2; ++i) y = fma(0.5 * y, fma(-d, y * y, 1.0), y);
+  return y;
+}
+__device__ __forceinline__ float rsqrt_nr(float d) {
+  const float y = __builtin_amdgcn_rsqf(d);
+  return fmaf(0.5f * y, fmaf(-d, y * y, 1.0f), y);
+}
+
 __device__ __forceinline__ double wave_sum(double v) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
   return v;
@@ -581,8 +606,9 @@ __global__ void __launch_bounds__(NT)
 // rows c + r).  Per panel:
 //  (a) wavefront 0 factors the panel in registers: lane q holds panel rows q
 //      and q + 64 (PR = NB + b rows), column t is pivoted, scaled and applied
-//      to the panel columns to its right; pivots and multipliers pass through
-//      LDS inside the wavefront (no block barrier);
+//      to the panel columns to its right; the pivot and the multipliers
+//      L(j0 + u, j0 + t) are read from their lanes with v_readlane (no LDS, no
+//      barrier), 1/sqrt by the hardware estimate and Newton steps;
 //  (b) the panel's columns go to HBM as the column band (contiguous) and the
 //      row band (rband[R*W + r] = L(R, R-r), runs of NB per row);
 //  (c) the block applies the symmetric rank-NB update to the b trailing
@@ -616,7 +642,6 @@ __global__ void __launch_bounds__(256)
   const int PR = NB + b;  // panel rows
   T *win = reinterpret_cast<T *>(smem_raw);  // S x W
   T *Lp = win + (size_t)S * W;               // PR x LDP: Lp[q*LDP + t] = L(j0 + q, j0 + t)
-  T *piv = Lp + (size_t)PR * LDP;            // pivot broadcast
   __shared__ int bad;
   const int mac = blockIdx.x, tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   T *Lg = band + (size_t)mac * n * W;
@@ -660,23 +685,20 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
       for (int t = 0; t < NB; ++t) {
         if (t < np) {
-          if (lane == t) piv[0] = a0[t];
-          wave_lds_sync();
-          T d = piv[0];
+          T d = lane_bcast(a0[t], t);  // pivot, held by lane t
           if (!(d > T(tiny)) || !isfinite((double)d)) {
             if (lane == 0) bad = MHM_STATUS_PIVOT;
             d = T(1);
           }
-          const T root = sqrt(d);
-          const T inv = T(1) / root;
+          const T inv = rsqrt_nr(d);
+          const T root = d * inv;
           a0[t] = q0 > t ? a0[t] * inv : (q0 == t ? root : T(0));
           a1[t] = a1[t] * inv;  // q1 >= 64 > t
           if (q0 < PR) Lp[q0 * LDP + t] = a0[t];
           if (q1 < PR) Lp[q1 * LDP + t] = a1[t];
-          wave_lds_sync();
 #pragma unroll
           for (int u = t + 1; u < NB; ++u) {
-            const T l = Lp[u * LDP + t];  // L(j0 + u, j0 + t), broadcast
+            const T l = lane_bcast(a0[t], u);  // L(j0 + u, j0 + t), held by lane u
             a0[u] -= a0[t] * l;
             a1[u] -= a1[t] * l;
           }
@@ -762,8 +784,9 @@ __global__ void __launch_bounds__(256)
 // blocked right-looking algorithm with the band array itself as the window.
 // Per panel of FPANEL columns:
 //  (a) wavefront 0 reads the panel (PR = NB + b <= 256 rows, four per lane)
-//      from the band, factors it in registers and leaves the multipliers in
-//      LDS, transposed (LpT[t * LDQ + q] = L(j0 + q, j0 + t));
+//      from the band, factors it in registers (pivots and multipliers by
+//      v_readlane, as in k_factor_panel) and leaves it in LDS, transposed
+//      (LpT[t * ldq + q] = L(j0 + q, j0 + t));
 //  (b) the panel goes back to the column band (and to the row band if one is
 //      kept);
 //  (c) the block applies the rank-NB update to the trailing triangle in place
@@ -791,7 +814,6 @@ __global__ void __launch_bounds__(256)
   extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
   const int W = b + 1, PR = NB + b;
   T *LpT = reinterpret_cast<T *>(smem_raw);  // NB x ldq
-  T *piv = LpT + (size_t)NB * ldq;
   __shared__ int bad;
   const int mac = blockIdx.x, tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   T *Lg = band + (size_t)mac * n * W;
@@ -820,25 +842,22 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
       for (int t = 0; t < NB; ++t) {
         if (t < np) {
-          if (lane == t) piv[0] = a[0][t];
-          wave_lds_sync();
-          T d = piv[0];
+          T d = lane_bcast(a[0][t], t);  // pivot, held by lane t
           if (!(d > T(tiny)) || !isfinite((double)d)) {
             if (lane == 0) bad = MHM_STATUS_PIVOT;
             d = T(1);
           }
-          const T root = sqrt(d);
-          const T inv = T(1) / root;
+          const T inv = rsqrt_nr(d);
+          const T root = d * inv;
           a[0][t] = lane > t ? a[0][t] * inv : (lane == t ? root : T(0));
 #pragma unroll
           for (int v = 1; v < NR; ++v) a[v][t] = a[v][t] * inv;  // rows >= 64 > t
 #pragma unroll
           for (int v = 0; v < NR; ++v)
             if (lane + 64 * v < PR) LpT[t * ldq + lane + 64 * v] = a[v][t];
-          wave_lds_sync();
 #pragma unroll
           for (int u = t + 1; u < NB; ++u) {
-            const T l = LpT[t * ldq + u];  // L(j0 + u, j0 + t), broadcast
+            const T l = lane_bcast(a[0][t], u);  // L(j0 + u, j0 + t), held by lane u
 #pragma unroll
             for (int v = 0; v < NR; ++v) a[v][u] -= a[v][t] * l;
           }
@@ -2045,12 +2064,252 @@ static int wave_width(int b, int ncols) {
   return 0;
 }
 
+// Blocked solve (wide bands): rows per block and columns per wavefront.
+#define SBLK 16
+static const int kBlkCols[] = {2, 4, 7, 8, 12, 16};  // columns per wavefront
+
+static int blk_cpt(int ncols) {
+  for (int c : kBlkCols)
+    if (ncols <= 4 * c) return c;
+  return 0;
+}
+
+static size_t solve_blk_lds(int b, int ncols, size_t tsize) {
+  const size_t W = (size_t)b + 1, ldr = 4 * (size_t)blk_cpt(ncols) + 1, RB = (size_t)SBLK + b;
+  return (SBLK * W + RB * ldr + SBLK * ldr) * tsize;
+}
+
+static bool solve_blk_ok(int b, int ncols, size_t tsize, int limit) {
+  return b >= 1 && b <= 255 && blk_cpt(ncols) > 0 && solve_blk_lds(b, ncols, tsize) <= (size_t)limit;
+}
+
 // Either the wavefront kernel or the block kernel can run a solve of ncols columns.
 static bool solve_supported(int b, int ncols, size_t tsize, int limit) {
-  if (wave_width(b, ncols) > 0) return true;
+  if (wave_width(b, ncols) > 0 || solve_blk_ok(b, ncols, tsize, limit)) return true;
   const int cap = block_threads(b) == 64 ? 16 : 64;
   return solve_slots(b, ncols) <= cap && solve_stage_fits(b, ncols) &&
          solve_lds(b, ncols, tsize) <= (size_t)limit;
+}
+
+// ----------------------------------------------------------------- blocked band solves
+// Block per macro for bandwidths past the wavefront solve's register window
+// (b + 1 > 96).  Rows go in blocks of SBLK; the band columns of a block (NB x W,
+// one contiguous run of the column band) are loaded a block ahead into
+// registers and staged in LDS, and the working rows of y live in an LDS ring of
+// NB + b rows (row r in slot r mod (NB + b), odd row stride).  Wavefront w
+// owns columns [w CPT, w CPT + CPT).
+// Forward L z = y, block k = rows [k, k + NB):
+//   (A) stage the band block; wavefronts 1-3 read the rows entering the
+//       window, [k + b, k + b + NB), from y into the ring;
+//   (B) wavefront 0 solves the NB x NB diagonal block (lane = column) and
+//       writes z to the ring and to y;
+//   (C) lane l updates rows k + NB + l + 64 v of the window [k + NB, k + NB + b)
+//       with the block's z rows (right-looking).
+// Backward L^T x = z, blocks from the last:
+//   (A) stage the band block; wavefront 0 reads the block's z from y;
+//   (C) lane (u, p) = (l mod 16, l / 16) sums L(r, k + u) x_r over rows
+//       r = k + NB + p, + 4, ... of the window for its wavefront's columns;
+//       the four partial sums are added across lanes and left in LDS;
+//   (B) wavefront 0 finishes the block's x upwards (lane = column) and writes
+//       x to the ring and to y.
+// Three block barriers per NB rows instead of one per row.
+template <typename T, int CPT>
+__global__ void __launch_bounds__(256)
+    k_solve_blk(int n, int b, int ldy, int ncols, const T *__restrict__ band, T *__restrict__ y,
+                const int *__restrict__ colstate, int *__restrict__ iters) {
+  constexpr int NB = SBLK;
+  constexpr int PFR = 16;  // NB * W <= 256 * PFR for b <= 255
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
+  const int mac = blockIdx.x, tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  __shared__ int active;
+  if (colstate) {
+    if (tid == 0) active = 0;
+    __syncthreads();
+    if (tid < ncols && colstate[(size_t)mac * ncols + tid] == 0) active = 1;
+    __syncthreads();
+    if (!active) return;
+  }
+  if (iters && tid == 0) iters[mac] += 1;
+  const int W = b + 1, RB = NB + b, ldr = 4 * CPT + 1;
+  T *bb = reinterpret_cast<T *>(smem_raw);  // NB x W: bb[t*W + d] = L(k + t + d, k + t)
+  T *ring = bb + NB * W;                    // RB x ldr
+  T *sred = ring + (size_t)RB * ldr;        // NB x ldr
+  const T *Lg = band + (size_t)mac * n * W;
+  T *Y = y + (size_t)mac * n * ldy;
+  const int c0 = wave * CPT;
+  const int nblk = (n + NB - 1) / NB;
+  const int rpt = (b + 63) / 64;
+  T pf[PFR];
+  auto band_issue = [&](int k) {  // band columns [k, k + NB) of [0, n)
+    const int cnt = (k >= 0 && k < n) ? min(NB, n - k) * W : 0;
+#pragma unroll
+    for (int v = 0; v < PFR; ++v) {
+      const int e = tid + 256 * v;
+      pf[v] = e < cnt ? Lg[(size_t)k * W + e] : T(0);
+    }
+  };
+  auto band_store = [&]() {
+#pragma unroll
+    for (int v = 0; v < PFR; ++v) {
+      const int e = tid + 256 * v;
+      if (e < NB * W) bb[e] = pf[v];
+    }
+  };
+  // ---------------- forward
+  band_issue(0);
+  for (int e = tid; e < min(n, RB) * ncols; e += 256) {
+    const int r = e / ncols, c = e - r * ncols;
+    ring[r * ldr + c] = Y[(size_t)r * ldy + c];
+  }
+  for (int kb = 0; kb < nblk; ++kb) {
+    const int k = kb * NB, np = min(NB, n - k);
+    const int sk = k % RB;
+    __syncthreads();
+    band_store();
+    band_issue(k + NB);
+    if (kb > 0 && wave > 0) {
+      const int r0 = k + b, r1 = min(n, k + b + NB);
+      for (int e = tid - 64; e < (r1 - r0) * ncols; e += 192) {
+        const int r = r0 + e / ncols, c = e % ncols;
+        ring[(r % RB) * ldr + c] = Y[(size_t)r * ldy + c];
+      }
+    }
+    __syncthreads();
+    if (wave == 0 && lane < ncols) {
+      T v[NB];
+#pragma unroll
+      for (int t = 0; t < NB; ++t) {
+        const int st = sk + t >= RB ? sk + t - RB : sk + t;
+        v[t] = t < np ? ring[st * ldr + lane] : T(0);
+      }
+#pragma unroll
+      for (int t = 0; t < NB; ++t) {
+        if (t < np) {
+          const T z = v[t] / bb[t * W];
+          v[t] = z;
+#pragma unroll
+          for (int u = t + 1; u < NB; ++u)
+            if (u < np && u - t <= b) v[u] -= bb[t * W + (u - t)] * z;
+        }
+      }
+#pragma unroll
+      for (int t = 0; t < NB; ++t)
+        if (t < np) {
+          const int st = sk + t >= RB ? sk + t - RB : sk + t;
+          ring[st * ldr + lane] = v[t];
+          Y[(size_t)(k + t) * ldy + lane] = v[t];
+        }
+    }
+    __syncthreads();
+    const int rend = min(n, k + NB + b);
+#pragma unroll
+    for (int vv = 0; vv < 4; ++vv) {
+      const int r = k + NB + lane + 64 * vv;
+      if (vv < rpt && r < rend) {
+        T *rp = ring + (r % RB) * ldr + c0;
+        T acc[CPT];
+#pragma unroll
+        for (int c = 0; c < CPT; ++c) acc[c] = rp[c];
+#pragma unroll 4
+        for (int t = 0; t < NB; ++t) {
+          if (t < np) {
+            const int d = r - k - t;
+            const T l = d <= b ? bb[t * W + d] : T(0);
+            const int st = sk + t >= RB ? sk + t - RB : sk + t;
+            const T *zp = ring + st * ldr + c0;
+#pragma unroll
+            for (int c = 0; c < CPT; ++c) acc[c] -= l * zp[c];
+          }
+        }
+#pragma unroll
+        for (int c = 0; c < CPT; ++c) rp[c] = acc[c];
+      }
+    }
+  }
+  // ---------------- backward
+  band_issue((nblk - 1) * NB);
+  for (int kb = nblk - 1; kb >= 0; --kb) {
+    const int k = kb * NB, np = min(NB, n - k);
+    const int sk = k % RB;
+    __syncthreads();
+    band_store();
+    band_issue(k - NB);
+    T zv[NB];
+    if (wave == 0 && lane < ncols) {
+#pragma unroll
+      for (int t = 0; t < NB; ++t) zv[t] = t < np ? Y[(size_t)(k + t) * ldy + lane] : T(0);
+    }
+    __syncthreads();
+    {
+      const int u = lane & 15, p = lane >> 4;
+      T acc[CPT];
+#pragma unroll
+      for (int c = 0; c < CPT; ++c) acc[c] = T(0);
+      const int rend = min(n, k + NB + b);
+      int sr = (k + NB + p) % RB;
+      for (int r = k + NB + p; r < rend; r += 4) {
+        const int d = r - k - u;
+        const T l = d <= b ? bb[u * W + d] : T(0);
+        const T *xp = ring + sr * ldr + c0;
+#pragma unroll
+        for (int c = 0; c < CPT; ++c) acc[c] += l * xp[c];
+        sr += 4;
+        if (sr >= RB) sr -= RB;
+      }
+#pragma unroll
+      for (int c = 0; c < CPT; ++c) {
+        acc[c] += __shfl_xor(acc[c], 16, 64);
+        acc[c] += __shfl_xor(acc[c], 32, 64);
+      }
+      if (p == 0 && u < np)
+#pragma unroll
+        for (int c = 0; c < CPT; ++c) sred[u * ldr + c0 + c] = acc[c];
+    }
+    __syncthreads();
+    if (wave == 0 && lane < ncols) {
+      T xv[NB];
+#pragma unroll
+      for (int u = NB - 1; u >= 0; --u) {
+        xv[u] = T(0);
+        if (u < np) {
+          T acc = zv[u] - sred[u * ldr + lane];
+#pragma unroll
+          for (int v = u + 1; v < NB; ++v)
+            if (v < np && v - u <= b) acc -= bb[u * W + (v - u)] * xv[v];
+          xv[u] = acc / bb[u * W];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < NB; ++u)
+        if (u < np) {
+          const int su = sk + u >= RB ? sk + u - RB : sk + u;
+          ring[su * ldr + lane] = xv[u];
+          Y[(size_t)(k + u) * ldy + lane] = xv[u];
+        }
+    }
+  }
+}
+
+template <typename T>
+static int launch_solve_blk(const mhm_batch *b, int ncols, const int *colstate, int *iters, hipStream_t st) {
+  const LayoutK &L = b->layout->k;
+  const size_t shm = solve_blk_lds(L.b, ncols, sizeof(T));
+#define MHM_BLK(c)                                                                                         \
+  case c:                                                                                                  \
+    if (shm > 65536)                                                                                       \
+      HIP_TRY(hipFuncSetAttribute((const void *)k_solve_blk<T, c>, hipFuncAttributeMaxDynamicSharedMemorySize, \
+                                  (int)shm));                                                              \
+    hipLaunchKernelGGL((k_solve_blk<T, c>), dim3(b->M), dim3(256), shm, st, L.n, L.b, b->ldy, ncols,         \
+                       (const T *)b->band, (T *)b->y, colstate, iters);                                    \
+    break;
+  switch (blk_cpt(ncols)) {
+    MHM_BLK(2) MHM_BLK(4) MHM_BLK(7) MHM_BLK(8) MHM_BLK(12) MHM_BLK(16)
+    default:
+      return fail(MHM_EINVAL, "too many columns for the blocked solve");
+  }
+#undef MHM_BLK
+  LAUNCH_CHECK("k_solve_blk");
+  return MHM_OK;
 }
 
 template <typename T, int MAXW>
@@ -2077,6 +2336,8 @@ static int launch_solve(const mhm_batch *b, int ncols, const int *colstate, int 
     default:
       break;
   }
+  if (solve_blk_ok(L.b, ncols, sizeof(T), lds_limit(b->layout->device)))
+    return launch_solve_blk<T>(b, ncols, colstate, iters, st);
   const size_t shm = solve_lds(L.b, ncols, sizeof(T));
   const int slots = solve_slots(L.b, ncols);
   const dim3 grid(b->M);
