@@ -94,6 +94,20 @@ __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// e / d for 0 <= e < 2^22, 1 <= d < 2^16: a binary32 reciprocal estimate is
+// within one of the quotient, one correction each way makes it exact.
+struct FastDiv {
+  int d;
+  float r;
+  __device__ explicit FastDiv(int dv) : d(dv), r(1.0f / (float)dv) {}
+  __device__ __forceinline__ int operator()(int e) const {
+    int q = (int)((float)e * r);
+    q += (q + 1) * d <= e;
+    q -= q * d > e;
+    return q;
+  }
+};
+
 // Value of v in lane l (wave-uniform l) as a scalar: v_readlane, no LDS.
 __device__ __forceinline__ double lane_bcast(double v, int l) {
   const long long x = __double_as_longlong(v);
@@ -237,6 +251,7 @@ struct LayoutK {
   const double *coupling, *kernel, *constraint;
   const int *slot_ptr, *slot_code, *load_ptr, *load_code;
   const double *glam, *measure, *qw, *dtab, *vtab;
+  const int *bslot;  // bslot[j*W + r] = CSR slot of A(j + r, j), or -1
 };
 
 // Per-macro scalars kept in scal[mac * NSCAL + ...].
@@ -382,26 +397,35 @@ __global__ void __launch_bounds__(64 * ASM_WAVES)
   }
 }
 
-// One thread per band row: ordered gather of the element contributions of each
-// CSR slot (fixed element order -> deterministic, bitwise symmetric A), the
-// load row, and A Z accumulated in double-double.
-__global__ void __launch_bounds__(128)
+// Ordered gather of the element contributions into the CSR values of
+// GATHER_ROWS band rows: one thread per CSR slot sums the slot's contributions
+// in the reference's duplicate-summation order (first value, then +=, fixed
+// element order -> deterministic, bitwise symmetric A); the block then forms
+// the load rows and A Z (double-double, slot order) one thread per row from
+// the slot values staged in LDS.
+#define GATHER_ROWS 64
+__global__ void __launch_bounds__(256)
     k_gather(LayoutK L, BatchK B, const double *__restrict__ ework,
              const double *__restrict__ fwork) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  extern __shared__ double gsv[];
   const int mac = blockIdx.y;
-  if (i >= L.n) return;
+  const int i0 = blockIdx.x * GATHER_ROWS, i1 = min(L.n, i0 + GATHER_ROWS);
+  const int s0 = L.rowptr[i0], s1 = L.rowptr[i1];
   const double *ew = ework + (size_t)mac * L.nb * L.nb * L.ne;
-  double hi = 0.0, lo = 0.0;
-  for (int s = L.rowptr[i]; s < L.rowptr[i + 1]; ++s) {
-    // stored order = the reference's duplicate-summation order (first value, then +=)
-    double v = 0.0;
+  double *av = B.aval + (size_t)mac * L.nnz;
+  for (int s = s0 + threadIdx.x; s < s1; s += blockDim.x) {
     const int t0 = L.slot_ptr[s], t1 = L.slot_ptr[s + 1];
+    double v = 0.0;
     if (t1 > t0) v = ew[L.slot_code[t0]];
     for (int t = t0 + 1; t < t1; ++t) v += ew[L.slot_code[t]];
-    B.aval[(size_t)mac * L.nnz + s] = v;
-    dd_fma(hi, lo, v, kernel_at(L, B, mac, L.cols[s]));
+    av[s] = v;
+    gsv[s - s0] = v;
   }
+  __syncthreads();
+  const int i = i0 + threadIdx.x;
+  if (i >= i1) return;
+  double hi = 0.0, lo = 0.0;
+  for (int s = L.rowptr[i]; s < L.rowptr[i + 1]; ++s) dd_fma(hi, lo, gsv[s - s0], kernel_at(L, B, mac, L.cols[s]));
   B.az[(size_t)mac * L.n + i] = hi + lo;
   const double *fw = fwork + (size_t)mac * L.nb * L.ne;
   double f = 0.0;
@@ -453,18 +477,27 @@ __global__ void __launch_bounds__(256) k_prepare(LayoutK L, BatchK B, T *__restr
     }
     B.dscale[(size_t)mac * L.n + i] = 1.0 / sqrt(d);
   }
+  __syncthreads();  // dscale of every row written
+  // band[j*W + r] = ds[j+r] A(j+r, j) ds[j] through the band slot map, one
+  // coalesced pass (positions outside the pattern or past row n - 1 are 0)
   T *bd = band + (size_t)mac * L.n * L.W;
-  for (size_t t = threadIdx.x; t < (size_t)L.n * L.W; t += 256) bd[t] = T(0);
-  __syncthreads();
   const double *ds = B.dscale + (size_t)mac * L.n;
-  for (int i = threadIdx.x; i < L.n; i += 256) {
-    const double di = ds[i];
-    for (int s = L.rowptr[i]; s < L.rowptr[i + 1]; ++s) {
-      const int j = L.cols[s];
-      if (j > i) break;
+  const int W = L.W, dj = 256 / W, dr = 256 - dj * W;
+  int j = threadIdx.x / W, r = threadIdx.x - j * W;
+  for (int e = threadIdx.x; e < L.n * W; e += 256) {
+    const int s = L.bslot[e];
+    T out = T(0);
+    if (s >= 0) {
       double v = a[s];
-      if (i == j && i == L.pin) v += sigma;
-      bd[(size_t)j * L.W + (i - j)] = T(di * v * ds[j]);
+      if (r == 0 && j == L.pin) v += sigma;
+      out = T(ds[j + r] * v * ds[j]);
+    }
+    bd[e] = out;
+    j += dj;
+    r += dr;
+    if (r >= W) {
+      r -= W;
+      ++j;
     }
   }
   if (bad) atomicOr(&B.status[mac], bad);
@@ -602,62 +635,106 @@ __global__ void __launch_bounds__(NT)
 
 // ----------------------------------------------------------------- blocked band Cholesky
 // Block per macro, panels of FPANEL columns.  The LDS window holds the
-// NB + b active band columns (slot c mod S, entries r = 0..b of column c at
-// rows c + r).  Per panel:
+// NB + b active band columns (slot c mod S at win[slot * ldw], entries
+// r = 0..b of column c at rows c + r).  Per panel:
 //  (a) wavefront 0 factors the panel in registers: lane q holds panel rows q
-//      and q + 64 (PR = NB + b rows), column t is pivoted, scaled and applied
-//      to the panel columns to its right; the pivot and the multipliers
-//      L(j0 + u, j0 + t) are read from their lanes with v_readlane (no LDS, no
-//      barrier), 1/sqrt by the hardware estimate and Newton steps;
+//      and q + 64 (PR = NB + b <= 127 rows), column t is pivoted, scaled and
+//      applied to the panel columns to its right; the pivot and the
+//      multipliers L(j0 + u, j0 + t) are read from their lanes with
+//      v_readlane (no LDS, no barrier), 1/sqrt by the hardware estimate and
+//      Newton steps.  The step is branch-free: loads at clamped addresses with
+//      selects, a failed pivot flagged and replaced by 1, the columns of a
+//      short last panel padded with identity columns (which leave the others
+//      unchanged).  The panel goes to LDS transposed, LpT[t * ldq + q] =
+//      L(j0 + q, j0 + t), rows PR .. ldq - 1 zero;
 //  (b) the panel's columns go to HBM as the column band (contiguous) and the
 //      row band (rband[R*W + r] = L(R, R-r), runs of NB per row);
-//  (c) the block applies the symmetric rank-NB update to the b trailing
-//      columns, 4x4 register tiles over the lower triangle;
+//  (c) the rank-NB update of the b trailing columns on the matrix cores:
+//      16 x 16 tiles of the lower triangle, round-robin over the wavefronts,
+//      C = C - L_I L_J^T in four 16x16x4 MFMA steps with the C fragment read
+//      from and written back to the window (ldw makes those accesses
+//      conflict-free in binary64, ldq the operand reads);
 //  (d) the NB columns that enter next, prefetched into registers at the start
 //      of the panel, take the panel's slots.
-// Two block barriers per panel instead of two per column.
+// Three block barriers per panel instead of two per column.
 #define FPANEL 16
 
 template <typename T>
-constexpr int panel_stride() {  // padded Lp row stride: b128 reads of consecutive rows hit distinct banks
-  return sizeof(T) == 8 ? FPANEL + 2 : FPANEL + 4;
+struct Mfma16;
+template <>
+struct Mfma16<double> {  // v_mfma_f64_16x16x4f64: D[4r + lane/16][lane%16] in register r
+  typedef double v4 __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ v4 step(double a, double b, v4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int r, int g) { return 4 * r + g; }
+};
+template <>
+struct Mfma16<float> {  // v_mfma_f32_16x16x4f32: D[4(lane/16) + r][lane%16] in register r
+  typedef float v4 __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ v4 step(float a, float b, v4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int r, int g) { return 4 * g + r; }
+};
+// (A operand: lane l holds A[l%16][l/16]; B operand: lane l holds B[l/16][l%16].)
+
+// Window column stride: in binary64 the C fragments of a 16 x 16 tile hit 64
+// distinct banks when ldw - 1 = 2 mod 4; in binary32 an odd ldw - 1 = 3 mod 4
+// leaves at most two-way conflicts.
+static int panel_ldw(int b, size_t tsize) {
+  int w = b + 1;
+  while ((w & 3) != 3) ++w;
+  if (tsize == 4 && (w & 1) == 0) ++w;
+  return w;
+}
+
+// LpT row stride: >= 128 rows (the two rows per lane of wavefront 0) and
+// >= NB + 16 * ceil(b / 16) (the tiles), = 16 mod 32 so the four 16-lane
+// groups of an operand read use disjoint banks.
+static int panel_ldq(int b) {
+  int q = std::max(128, FPANEL + 16 * ((b + 15) / 16));
+  while ((q & 31) != 16) ++q;
+  return q;
 }
 
 static size_t factor_panel_lds(int b, size_t tsize) {
-  const size_t S = (size_t)FPANEL + b, W = (size_t)b + 1;
-  const size_t ldp = tsize == 8 ? FPANEL + 2 : FPANEL + 4;
-  return (S * W + S * ldp + 64) * tsize;
+  const size_t S = (size_t)FPANEL + b;
+  return (S * panel_ldw(b, tsize) + (size_t)FPANEL * panel_ldq(b)) * tsize;
 }
 
 template <typename T>
-__global__ void __launch_bounds__(256)
-    k_factor_panel(int n, int b, T *__restrict__ band, T *__restrict__ rband, int *__restrict__ status,
-                   double tiny) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+    k_factor_panel(int n, int b, int ldw, int ldq, T *__restrict__ band, T *__restrict__ rband,
+                   int *__restrict__ status, double tiny) {
   constexpr int NB = FPANEL;
-  constexpr int LDP = panel_stride<T>();
   constexpr int NPF = 8;  // NB * W <= 256 * NPF, so W <= 128
+  typedef typename Mfma16<T>::v4 V4;
   extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
   const int W = b + 1;
   const int S = NB + b;   // window columns
   const int PR = NB + b;  // panel rows
-  T *win = reinterpret_cast<T *>(smem_raw);  // S x W
-  T *Lp = win + (size_t)S * W;               // PR x LDP: Lp[q*LDP + t] = L(j0 + q, j0 + t)
+  T *win = reinterpret_cast<T *>(smem_raw);  // S x ldw
+  T *LpT = win + (size_t)S * ldw;            // NB x ldq
   __shared__ int bad;
   const int mac = blockIdx.x, tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   T *Lg = band + (size_t)mac * n * W;
   T *Rg = rband ? rband + (size_t)mac * n * W : nullptr;
+  const FastDiv divW(W);
   if (tid == 0) bad = 0;
   for (int e = tid; e < S * W; e += 256) {  // columns 0 .. S-1 in slots 0 .. S-1
-    const int c = e / W, r = e - c * W;
-    win[e] = (c + r < n) ? Lg[e] : T(0);
+    const int c = divW(e), r = e - c * W;
+    win[c * ldw + r] = (c + r < n) ? Lg[e] : T(0);
   }
+  for (int e = tid; e < NB * ldq; e += 256) LpT[e] = T(0);
   if (Rg)  // entries L(j, j-r) with j - r < 0
     for (int e = tid; e < min(b, n) * W; e += 256) {
-      const int j = e / W, r = e - j * W;
+      const int j = divW(e), r = e - j * W;
       if (r > j) Rg[e] = T(0);
     }
-  const int side = (b + 3) / 4;
-  const int ntiles = side * (side + 1) / 2;
+  const int nt = (b + 15) / 16;  // tile rows / columns of the trailing block
+  const int ntile = nt * (nt + 1) / 2;
+  const int j16 = lane & 15, g16 = lane >> 4;
   T pf[NPF];
   __syncthreads();
   for (int j0 = 0; j0 < n; j0 += NB) {
@@ -667,8 +744,8 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
     for (int v = 0; v < NPF; ++v) {
       const int e = tid + 256 * v;
-      const int c = j0 + S + e / W, r = e % W;
-      pf[v] = (e < NB * W && c + r < n) ? Lg[(size_t)(j0 + S) * W + e] : T(0);
+      const int t = divW(e);
+      pf[v] = (e < NB * W && j0 + S + t + (e - t * W) < n) ? Lg[(size_t)(j0 + S) * W + e] : T(0);
     }
     // (a) panel factorisation, wavefront 0
     if (wave == 0) {
@@ -678,89 +755,79 @@ __global__ void __launch_bounds__(256)
       for (int t = 0; t < NB; ++t) {
         int slot = s0 + t;
         slot = slot >= S ? slot - S : slot;
-        const T *col = win + (size_t)slot * W;
-        a0[t] = (t < np && q0 >= t && q0 - t <= b) ? col[q0 - t] : T(0);
-        a1[t] = (t < np && q1 < PR && q1 - t <= b) ? col[q1 - t] : T(0);
+        const T *col = win + slot * ldw;
+        const T v0 = col[min(max(q0 - t, 0), b)];
+        const T v1 = col[min(q1 - t, b)];
+        const bool pad = t >= np;  // identity column of a short last panel
+        a0[t] = pad ? (q0 == t ? T(1) : T(0)) : (q0 >= t && q0 - t <= b ? v0 : T(0));
+        a1[t] = !pad && q1 < PR && q1 - t <= b ? v1 : T(0);
       }
+      bool failed = false;
 #pragma unroll
       for (int t = 0; t < NB; ++t) {
-        if (t < np) {
-          T d = lane_bcast(a0[t], t);  // pivot, held by lane t
-          if (!(d > T(tiny)) || !isfinite((double)d)) {
-            if (lane == 0) bad = MHM_STATUS_PIVOT;
-            d = T(1);
-          }
-          const T inv = rsqrt_nr(d);
-          const T root = d * inv;
-          a0[t] = q0 > t ? a0[t] * inv : (q0 == t ? root : T(0));
-          a1[t] = a1[t] * inv;  // q1 >= 64 > t
-          if (q0 < PR) Lp[q0 * LDP + t] = a0[t];
-          if (q1 < PR) Lp[q1 * LDP + t] = a1[t];
+        T d = lane_bcast(a0[t], t);  // pivot, held by lane t
+        const bool ok = d > T(tiny) && isfinite((double)d);
+        failed |= !ok;
+        d = ok ? d : T(1);
+        const T inv = rsqrt_nr(d);
+        const T root = d * inv;
+        a0[t] = q0 > t ? a0[t] * inv : (q0 == t ? root : T(0));
+        a1[t] = a1[t] * inv;  // q1 >= 64 > t
+        LpT[t * ldq + q0] = a0[t];
+        LpT[t * ldq + q1] = a1[t];
 #pragma unroll
-          for (int u = t + 1; u < NB; ++u) {
-            const T l = lane_bcast(a0[t], u);  // L(j0 + u, j0 + t), held by lane u
-            a0[u] -= a0[t] * l;
-            a1[u] -= a1[t] * l;
-          }
-        } else {
-          if (q0 < PR) Lp[q0 * LDP + t] = T(0);
-          if (q1 < PR) Lp[q1 * LDP + t] = T(0);
+        for (int u = t + 1; u < NB; ++u) {
+          const T l = lane_bcast(a0[t], u);  // L(j0 + u, j0 + t), held by lane u
+          a0[u] -= a0[t] * l;
+          a1[u] -= a1[t] * l;
         }
       }
+      if (failed && lane == 0) bad = MHM_STATUS_PIVOT;
     }
     __syncthreads();
     // (b) the panel to HBM: column band (contiguous) and row band
     for (int e = tid; e < np * W; e += 256) {
-      const int t = e / W, r = e - t * W;
-      Lg[(size_t)j0 * W + e] = Lp[(t + r) * LDP + t];
+      const int t = divW(e), r = e - t * W;
+      Lg[(size_t)j0 * W + e] = LpT[t * ldq + t + r];
     }
     if (Rg)
       for (int e = tid; e < PR * NB; e += 256) {
         const int q = e / NB, t = e - q * NB;
         const int R = j0 + q, r = q - t;
-        if (t < np && r >= 0 && r <= b && R < n) Rg[(size_t)R * W + r] = Lp[q * LDP + t];
+        if (t < np && r >= 0 && r <= b && R < n) Rg[(size_t)R * W + r] = LpT[t * ldq + q];
       }
-    // (c) trailing update: columns q_c in [NB, PR), rows q_r in [q_c, PR)
-    for (int id = tid; id < ntiles; id += 256) {
-      int tr = (int)((sqrtf(8.0f * id + 1.0f) - 1.0f) * 0.5f);
-      while ((tr + 1) * (tr + 2) / 2 <= id) ++tr;
-      while (tr * (tr + 1) / 2 > id) --tr;
-      const int tc = id - tr * (tr + 1) / 2;
-      const int qr0 = NB + 4 * tr, qc0 = NB + 4 * tc;
-      T acc[4][4];
-      int off[4];
-#pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        int slot = s0 + qc0 + j;
-        while (slot >= S) slot -= S;
-        off[j] = slot * W;
+    // (c) trailing update on the matrix cores: tile k = (I, J), J <= I < nt,
+    // columns qc0 = NB + 16 J, rows qr0 = NB + 16 I, for wavefront k mod 4
+    for (int k = wave; k < ntile; k += 4) {
+      int J = 0, rem = k;
+      while (rem >= nt - J) {
+        rem -= nt - J;
+        ++J;
       }
+      const int qr0 = NB + 16 * (J + rem), qc0 = NB + 16 * J;
+      const int qc = qc0 + j16;  // this lane's column
+      int slot = s0 + qc;
+      slot = slot >= S ? slot - S : slot;
+      slot = slot >= S ? slot - S : slot;
+      T *colp = win + slot * ldw;
+      V4 acc;
 #pragma unroll
-      for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-          const int qr = qr0 + i, qc = qc0 + j;
-          acc[i][j] = (qr < PR && qc < PR && qr >= qc) ? win[off[j] + (qr - qc)] : T(0);
-        }
-#pragma unroll 4
-      for (int t = 0; t < NB; ++t) {
-        T ar[4], bc[4];
-#pragma unroll
-        for (int i = 0; i < 4; ++i) ar[i] = qr0 + i < PR ? Lp[(qr0 + i) * LDP + t] : T(0);
-#pragma unroll
-        for (int j = 0; j < 4; ++j) bc[j] = qc0 + j < PR ? Lp[(qc0 + j) * LDP + t] : T(0);
-#pragma unroll
-        for (int i = 0; i < 4; ++i)
-#pragma unroll
-          for (int j = 0; j < 4; ++j) acc[i][j] -= ar[i] * bc[j];
+      for (int r = 0; r < 4; ++r) {
+        const int qr = qr0 + Mfma16<T>::row(r, g16);
+        const bool in = qr >= qc && qr < PR && qc < PR;
+        acc[r] = colp[in ? qr - qc : 0];
       }
 #pragma unroll
-      for (int i = 0; i < 4; ++i)
+      for (int t0 = 0; t0 < NB; t0 += 4) {
+        const T a = -LpT[(t0 + g16) * ldq + qr0 + j16];  // A[i][k] = -L(j0 + qr0 + i, j0 + t0 + k)
+        const T bb = LpT[(t0 + g16) * ldq + qc0 + j16];  // B[k][j] = L(j0 + qc0 + j, j0 + t0 + k)
+        acc = Mfma16<T>::step(a, bb, acc);
+      }
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-          const int qr = qr0 + i, qc = qc0 + j;
-          if (qr < PR && qc < PR && qr >= qc) win[off[j] + (qr - qc)] = acc[i][j];
-        }
+      for (int r = 0; r < 4; ++r) {
+        const int qr = qr0 + Mfma16<T>::row(r, g16);
+        if (qr >= qc && qr < PR && qc < PR) colp[qr - qc] = acc[r];
+      }
     }
     __syncthreads();
     // (d) entering columns take the panel's slots
@@ -768,10 +835,10 @@ __global__ void __launch_bounds__(256)
     for (int v = 0; v < NPF; ++v) {
       const int e = tid + 256 * v;
       if (e < NB * W) {
-        const int t = e / W, r = e - t * W;
+        const int t = divW(e), r = e - t * W;
         int slot = s0 + t;
         slot = slot >= S ? slot - S : slot;
-        win[(size_t)slot * W + r] = pf[v];
+        win[slot * ldw + r] = pf[v];
       }
     }
     __syncthreads();
@@ -1687,6 +1754,7 @@ __global__ void __launch_bounds__(256)
     const double cs = sc[S_CS];
     double cy = 0.0;
     if (!skip)
+#pragma unroll 4
       for (int i = lane; i < n; i += lanes)
         cy += cs * constraint_at(L, B, mac, i) * (ds[i] * (double)Y[(size_t)i * B.ldy + c]);
     sred[0][tid] = cy;
@@ -1705,6 +1773,7 @@ __global__ void __launch_bounds__(256)
   double dn = 0.0, xn = 0.0;
   if (!skip) {
     const double xp = L.k ? sxp[c] : 0.0, mu = L.k ? smu[c] : 0.0;
+#pragma unroll 4
     for (int i = lane; i < n; i += lanes) {
       double v = ds[i] * (double)Y[(size_t)i * B.ldy + c];
       if (L.k) v += sigma * xp * u[i] - mu * w[i];
@@ -1759,14 +1828,16 @@ __global__ void k_fill_int(int *p, int v, size_t count) {
 // E = Z - X[:,m+1] (X row-major, R columns).
 // Phase 1 (coupled rows, in chunks of COND_ROWS staged in LDS): the products
 // B^T [L | s | E | Z], one output per (a, col), accumulated in LDS.
-// Phase 2 (all rows): (A Z)^T [L | E | s] and Z^T f, thread (group, col) with
-// coalesced row reads, then a reduction over the groups.
+// Phase 2 (all rows): (A Z)^T [L | E | s], thread (group, col) with coalesced
+// row reads and four rows in flight, then a reduction over the groups; Z^T f
+// by a block reduction.
 #define COND_ROWS 32
 
 __global__ void __launch_bounds__(256)
     k_condense(LayoutK L, BatchK B, const double *__restrict__ X, double *__restrict__ blocks,
                double *__restrict__ loads) {
   extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
+  __shared__ double fin[2];
   const int mac = blockIdx.x;
   const int tid = threadIdx.x;
   const int n = L.n, m = L.m, k = L.k, N = B.N, R = B.R;
@@ -1823,40 +1894,58 @@ __global__ void __launch_bounds__(256)
     else if (c == m + 1) blk[a * w + m] = -acc[o];
   }
   if (!k) return;
-  // phase 2: column c < m -> az.L_c, c == m -> az.E, c == m+1 -> az.s, c == m+2 -> Z.f
+  // phase 2: column c < m -> az.L_c, c == m -> az.E, c == m+1 -> az.s (X
+  // column 1 + c, 1 + m, 0), four rows in flight per thread; c == m+2 -> Z.f,
+  // a block reduction of its own
   const int nc = m + 3;
-  const int G = 256 / nc;
-  const int g = tid / nc, c = tid - g * nc;
-  double sum = 0.0;
+  const int G = 256 / (m + 2);
+  const int g = tid / (m + 2), c = tid - g * (m + 2);
+  const int xc = c < m ? 1 + c : (c == m ? 1 + m : 0);
+  double p0 = 0.0, p1 = 0.0, p2 = 0.0, p3 = 0.0;
   if (g < G) {
-    for (int i = g; i < n; i += G) {
-      const double *xr = Xm + (size_t)i * R;
-      if (c < m)
-        sum += az[i] * xr[1 + c];
-      else if (c == m)
-        sum += az[i] * (kernel_at(L, B, mac, i) - xr[1 + m]);
-      else if (c == m + 1)
-        sum += az[i] * xr[0];
-      else
-        sum += kernel_at(L, B, mac, i) * f[i];
+    const double *xcol = Xm + xc;
+    auto term = [&](int i) {
+      const double x = xcol[(size_t)i * R];
+      return az[i] * (c == m ? kernel_at(L, B, mac, i) - x : x);
+    };
+    int i = g;
+    for (; i + 3 * G < n; i += 4 * G) {
+      p0 += term(i);
+      p1 += term(i + G);
+      p2 += term(i + 2 * G);
+      p3 += term(i + 3 * G);
     }
+    for (; i < n; i += G) p0 += term(i);
   }
-  red[tid] = sum;
+  double zf = 0.0;
+  for (int i = tid; i < n; i += 256) zf += kernel_at(L, B, mac, i) * f[i];
+  red[tid] = zf;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (tid < o) red[tid] += red[tid + o];
+    __syncthreads();
+  }
+  zf = red[0];
+  __syncthreads();
+  red[tid] = (p0 + p1) + (p2 + p3);
   __syncthreads();
   if (tid < nc) {
     double t = 0.0;
-    for (int l = 0; l < G; ++l) t += red[l * nc + tid];
+    if (tid < m + 2)
+      for (int l = 0; l < G; ++l) t += red[l * (m + 2) + tid];
+    else
+      t = zf;
     if (tid < m) {
       zb = acc[tid * nb + m + 2];  // Z^T B column a = tid
       blk[m * w + tid] = -(zb - t);
     } else if (tid == m) {
       blk[m * w + m] = -t;
     } else {
-      red[tid] = t;  // az.s at m+1, Z.f at m+2 (read below by thread m+1)
+      fin[tid - m - 1] = t;  // az.s, then Z.f (read below by thread m+1)
     }
   }
   __syncthreads();
-  if (tid == m + 1) ld[m] = red[m + 1] - red[m + 2];
+  if (tid == m + 1) ld[m] = fin[0] - fin[1];
 }
 
 static size_t condense_lds(int m, int k) {
@@ -1963,6 +2052,7 @@ __global__ void __launch_bounds__(256)
 struct mhm_layout {
   int device = 0;
   LayoutK k{};
+  int gather_slots = 0;  // most CSR slots in GATHER_ROWS consecutive rows
   const double *coupling_rm = nullptr;  // template coupling, row-major n x m (band rows)
   std::vector<void *> owned;
   double cmax_host = 0.0;
@@ -2489,6 +2579,17 @@ int mhm_layout_create(int32_t device, const mhm_layout_desc *d, mhm_layout **out
       for (size_t i = 0; i < n; ++i) rm[i * d->m + a] = d->coupling[a * n + i];
     if ((rc = upload(L->owned, rm.data(), rm.size(), &L->coupling_rm))) return cleanup(rc);
   }
+  {
+    std::vector<int> bslot(n * (size_t)k.W, -1);
+    for (int i = 0; i < d->n; ++i)
+      for (int s = d->rowptr[i]; s < d->rowptr[i + 1]; ++s) {
+        const int j = d->cols[s];
+        if (j <= i) bslot[(size_t)j * k.W + (i - j)] = s;
+      }
+    if ((rc = upload(L->owned, bslot.data(), bslot.size(), &k.bslot))) return cleanup(rc);
+    for (int i0 = 0; i0 < d->n; i0 += GATHER_ROWS)
+      L->gather_slots = std::max(L->gather_slots, d->rowptr[std::min(d->n, i0 + GATHER_ROWS)] - d->rowptr[i0]);
+  }
   if (d->ne > 0) {
     const size_t ne = d->ne, nb = d->nb, nq = d->nq, dim = d->dim;
     if ((rc = upload(L->owned, d->slot_ptr, (size_t)d->nnz + 1, &k.slot_ptr)) ||
@@ -2656,7 +2757,12 @@ int mhm_batch_assemble(mhm_batch *b, const double *kq, const double *fq, double 
   else
     rc = launch_assemble<10, 3>(b, kq, fq, f_const, st);
   if (rc) return rc;
-  hipLaunchKernelGGL(k_gather, grid2(L.n, 128, b->M), dim3(128), 0, st, L, b->bk, b->ework, b->fwork);
+  const size_t gshm = (size_t)b->layout->gather_slots * sizeof(double);
+  if (gshm > (size_t)lds_limit(b->layout->device))
+    return fail(MHM_EINVAL, "pattern too dense for the gather stage (" + std::to_string(GATHER_ROWS) + " rows)");
+  if (gshm > 65536)
+    HIP_TRY(hipFuncSetAttribute((const void *)k_gather, hipFuncAttributeMaxDynamicSharedMemorySize, (int)gshm));
+  hipLaunchKernelGGL(k_gather, grid2(L.n, GATHER_ROWS, b->M), dim3(256), gshm, st, L, b->bk, b->ework, b->fwork);
   LAUNCH_CHECK("k_gather");
   b->factored = b->condensed = false;
   return MHM_OK;
@@ -2771,8 +2877,8 @@ static int run_factor(mhm_batch *b, hipStream_t st) {
     if (pshm > 65536)
       HIP_TRY(hipFuncSetAttribute((const void *)k_factor_panel<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                   (int)pshm));
-    hipLaunchKernelGGL((k_factor_panel<T>), dim3(b->M), dim3(256), pshm, st, L.n, L.b, (T *)b->band,
-                       (T *)b->rband, b->bk.status, tiny);
+    hipLaunchKernelGGL((k_factor_panel<T>), dim3(b->M), dim3(256), pshm, st, L.n, L.b, panel_ldw(L.b, sizeof(T)),
+                       panel_ldq(L.b), (T *)b->band, (T *)b->rband, b->bk.status, tiny);
   } else if (factor_wide_ok(L.b, sizeof(T), lds_limit(lay->device))) {
     const size_t wshm = factor_wide_lds(L.b, sizeof(T));
     if (wshm > 65536)
