@@ -183,6 +183,20 @@ int mhm_batch_responses(mhm_batch *batch, double *d_source, double *d_lifts, dou
 int mhm_batch_solve(mhm_batch *batch, const double *d_rhs, double *d_x, int32_t ncols,
                     int32_t max_refine, double rtol, void *stream);
 
+/* New source fields against the resident factors, `ncols` at once (the online
+ * step: OfflineLocalProblem.response, core/offline.py:121-127, with
+ * LocalProblem.with_load, core/contracts.py:275-298).  For each field c the
+ * loads are assembled from d_fq[c][macro][element][q] (NULL: f_const
+ * everywhere) with the operations of mhm_batch_assemble, [f; 0] is solved with
+ * the refinement of mhm_batch_condense, and the condensed loads
+ * [B_t^T s ; (A Z)^T s - Z^T f] (core/reconstruction.py:91-127) go to
+ * d_loads[c][macro][m+k]; d_source[c][macro][n] (original rows) receives the
+ * source responses s when not NULL.  A field's results do not depend on the
+ * other fields of the call.  Requires a layout with an assembly description
+ * and ncols <= max_columns. */
+int mhm_batch_sources(mhm_batch *batch, const double *d_fq, double f_const, int32_t ncols,
+                      int32_t max_refine, double rtol, double *d_loads, double *d_source, void *stream);
+
 /* Fields u = s - L lambda + E c for `ncases` trace/coarse vectors:
  * d_trace[macro][case][m] (local trace order), d_coarse[macro][case][k],
  * d_fields[macro][case][n] ORIGINAL rows.

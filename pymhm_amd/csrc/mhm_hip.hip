@@ -433,6 +433,82 @@ __global__ void __launch_bounds__(256)
   B.load[(size_t)mac * L.n + i] = f;
 }
 
+// New source fields against resident factors, block per (macro, field c):
+// element loads sum_q ((w_q phi_i) f_q) |T| in quadrature order (the same
+// operations as k_assemble, so equal loads give equal bits) staged in LDS, then
+// the ordered load gather of k_gather into column c of the solve's right-hand
+// side rhs[mac][N][ncols] (band rows; the constraint row is 0).
+template <int NB, int D>
+__global__ void __launch_bounds__(256)
+    k_source_load(LayoutK L, int M, int ncols, const double *__restrict__ fq, double fconst,
+                  double *__restrict__ rhs, int N) {
+#pragma clang fp contract(off)
+  extern __shared__ double fe[];  // NB x ne, code i*ne + e
+  const int mac = blockIdx.x, c = blockIdx.y;
+  const int nq = L.nq;
+  for (int t = threadIdx.x; t < NB * L.ne; t += blockDim.x) {
+    const int i = t / L.ne, e = t - i * L.ne;
+    const double *fp = fq ? fq + (((size_t)c * M + mac) * L.ne + e) * nq : nullptr;
+    const double meas = L.measure[e];
+    double F = 0.0;
+    for (int q = 0; q < nq; ++q) F = F + ((L.qw[q] * L.vtab[q * NB + i]) * (fp ? fp[q] : fconst)) * meas;
+    fe[t] = F;
+  }
+  __syncthreads();
+  double *out = rhs + (size_t)mac * N * ncols + c;
+  for (int row = threadIdx.x; row < N; row += blockDim.x) {
+    double f = 0.0;
+    if (row < L.n)
+      for (int t = L.load_ptr[row]; t < L.load_ptr[row + 1]; ++t) f += fe[L.load_code[t]];
+    out[(size_t)row * ncols] = f;
+  }
+}
+
+// Condensed loads of new sources (core/reconstruction.py:91-127), block per
+// (macro, field c), from s = X[:, c] and f = rhs[:, c]: B_t^T s over the
+// coupled rows in row order (as k_condense), and with a retained mode
+// (A Z)^T s - Z^T f by fixed-order block reductions; optionally the source
+// responses in original rows.  Each field's arithmetic is independent of the
+// others, so a request's result does not depend on its batch.
+__global__ void __launch_bounds__(256)
+    k_source_condense(LayoutK L, BatchK B, int ncols, const double *__restrict__ X,
+                      const double *__restrict__ rhs, double *__restrict__ loads, double *__restrict__ source) {
+  __shared__ double red[2][256];
+  const int mac = blockIdx.x, c = blockIdx.y, tid = threadIdx.x;
+  const int n = L.n, m = L.m, N = B.N, w = L.m + L.k;
+  const double *x = X + (size_t)mac * N * ncols + c;
+  const double *f = rhs + (size_t)mac * N * ncols + c;
+  double *ld = loads + ((size_t)c * B.M + mac) * w;
+  const int nrows = L.crows ? L.ncr : n;
+  for (int a = tid; a < m; a += 256) {
+    double acc = 0.0;
+    for (int q = 0; q < nrows; ++q) {
+      const int i = L.crows ? L.crows[q] : q;
+      acc += coupling_at(L, B, mac, a, i) * x[(size_t)i * ncols];
+    }
+    ld[a] = acc;
+  }
+  if (source)
+    for (int i = tid; i < n; i += 256) source[((size_t)c * B.M + mac) * n + L.perm[i]] = x[(size_t)i * ncols];
+  if (!L.k) return;
+  double as = 0.0, zf = 0.0;
+  for (int i = tid; i < n; i += 256) {
+    as += B.az[(size_t)mac * n + i] * x[(size_t)i * ncols];
+    zf += kernel_at(L, B, mac, i) * f[(size_t)i * ncols];
+  }
+  red[0][tid] = as;
+  red[1][tid] = zf;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (tid < o) {
+      red[0][tid] += red[0][tid + o];
+      red[1][tid] += red[1][tid + o];
+    }
+    __syncthreads();
+  }
+  if (tid == 0) ld[m] = red[0][0] - red[1][0];
+}
+
 // Assembled-operator path: A Z from supplied CSR values; loads permuted to band rows.
 __global__ void __launch_bounds__(128)
     k_rowsum(LayoutK L, BatchK B, const double *__restrict__ loads_orig) {
@@ -1729,12 +1805,18 @@ static int res_width(int ncols) {
 // x = D y (+ bordered correction when k == 1); it == 0 sets X, later
 // iterations add the correction and record ||delta||^2, ||x||^2 and the
 // previous ||delta||^2 in colnorm[3 * col].
+// Per-column sums over the rows go in fixed chunks of SADDLE_ROWS rows, each
+// summed in row order, then the chunk sums in chunk order: the result does not
+// depend on how many columns share the call (a column's bits are the same
+// alone and in a batch).
+#define SADDLE_ROWS 32
+
 template <typename T>
 __global__ void __launch_bounds__(256)
     k_saddle_apply(LayoutK L, BatchK B, int ncols, const T *__restrict__ y, const double *__restrict__ beta,
                    double *__restrict__ X, int it, const int *__restrict__ colstate,
                    double *__restrict__ colnorm) {
-  __shared__ double sred[2][256];
+  extern __shared__ double part[];  // 2 x nchunk x ncols
   __shared__ double sxp[64], smu[64];
   const int mac = blockIdx.x;
   const int tid = threadIdx.x;
@@ -1742,6 +1824,7 @@ __global__ void __launch_bounds__(256)
   const int lane = tid / ncols, c = tid - lane * ncols;
   const bool on = lane < lanes;
   const int n = L.n, N = B.N;
+  const int nchunk = (n + SADDLE_ROWS - 1) / SADDLE_ROWS;
   const bool skip = !on || (it > 0 && colstate[(size_t)mac * ncols + c]);
   double *x = X + (size_t)mac * N * ncols;
   const double *ds = B.dscale + (size_t)mac * n;
@@ -1752,16 +1835,19 @@ __global__ void __launch_bounds__(256)
   const double sigma = sc[S_SIGMA];
   if (L.k) {
     const double cs = sc[S_CS];
-    double cy = 0.0;
     if (!skip)
+      for (int k = lane; k < nchunk; k += lanes) {
+        double cy = 0.0;
+        const int i1 = min(n, (k + 1) * SADDLE_ROWS);
 #pragma unroll 4
-      for (int i = lane; i < n; i += lanes)
-        cy += cs * constraint_at(L, B, mac, i) * (ds[i] * (double)Y[(size_t)i * B.ldy + c]);
-    sred[0][tid] = cy;
+        for (int i = k * SADDLE_ROWS; i < i1; ++i)
+          cy += cs * constraint_at(L, B, mac, i) * (ds[i] * (double)Y[(size_t)i * B.ldy + c]);
+        part[k * ncols + c] = cy;
+      }
     __syncthreads();
     if (tid < ncols) {
       double s = 0.0;
-      for (int l = 0; l < lanes; ++l) s += sred[0][l * ncols + tid];
+      for (int k = 0; k < nchunk; ++k) s += part[k * ncols + tid];
       const double yp = ds[L.pin] * (double)Y[(size_t)L.pin * B.ldy + tid];
       const double r1 = yp, r2 = s - beta[(size_t)mac * ncols + tid];
       const double det = sc[S_DET];
@@ -1770,22 +1856,27 @@ __global__ void __launch_bounds__(256)
     }
     __syncthreads();
   }
-  double dn = 0.0, xn = 0.0;
   if (!skip) {
     const double xp = L.k ? sxp[c] : 0.0, mu = L.k ? smu[c] : 0.0;
+    for (int k = lane; k < nchunk; k += lanes) {
+      double dn = 0.0, xn = 0.0;
+      const int i1 = min(n, (k + 1) * SADDLE_ROWS);
 #pragma unroll 4
-    for (int i = lane; i < n; i += lanes) {
-      double v = ds[i] * (double)Y[(size_t)i * B.ldy + c];
-      if (L.k) v += sigma * xp * u[i] - mu * w[i];
-      double *xi = x + (size_t)i * ncols + c;
-      if (it > 0) {
-        const double nv = *xi + v;
-        *xi = nv;
-        dn += v * v;
-        xn += nv * nv;
-      } else {
-        *xi = v;
+      for (int i = k * SADDLE_ROWS; i < i1; ++i) {
+        double v = ds[i] * (double)Y[(size_t)i * B.ldy + c];
+        if (L.k) v += sigma * xp * u[i] - mu * w[i];
+        double *xi = x + (size_t)i * ncols + c;
+        if (it > 0) {
+          const double nv = *xi + v;
+          *xi = nv;
+          dn += v * v;
+          xn += nv * nv;
+        } else {
+          *xi = v;
+        }
       }
+      part[k * ncols + c] = dn;
+      part[(nchunk + k) * ncols + c] = xn;
     }
     if (L.k && lane == 0) {
       double *xm = x + (size_t)n * ncols + c;
@@ -1795,14 +1886,12 @@ __global__ void __launch_bounds__(256)
         *xm = mu;
     }
   }
-  sred[0][tid] = dn;
-  sred[1][tid] = xn;
   __syncthreads();
-  if (tid < ncols && !(skip && on)) {
+  if (tid < ncols && !(it > 0 && colstate[(size_t)mac * ncols + tid])) {
     double D2 = 0.0, X2 = 0.0;
-    for (int l = 0; l < lanes; ++l) {
-      D2 += sred[0][l * ncols + tid];
-      X2 += sred[1][l * ncols + tid];
+    for (int k = 0; k < nchunk; ++k) {
+      D2 += part[k * ncols + tid];
+      X2 += part[(nchunk + k) * ncols + tid];
     }
     double *cn = colnorm + ((size_t)mac * ncols + tid) * 3;
     if (it > 0) {
@@ -1814,6 +1903,10 @@ __global__ void __launch_bounds__(256)
       cn[1] = 0.0;
     }
   }
+}
+
+static size_t saddle_lds(int n, int ncols) {
+  return (size_t)2 * ((n + SADDLE_ROWS - 1) / SADDLE_ROWS) * ncols * sizeof(double);
 }
 
 __global__ void k_fill_int(int *p, int v, size_t count) {
@@ -2641,6 +2734,8 @@ int mhm_batch_create(const mhm_layout *L, int32_t macros, int32_t precision, int
   const int widest = std::max(std::max(1 + lk.m + lk.k, 2), max_columns);
   if (!solve_supported(lk.b, widest, tsize, limit) || !solve_supported(lk.b, 2, tsize, limit))
     return fail(MHM_EINVAL, "solve window exceeds the register or LDS budget");
+  if (saddle_lds(lk.n, widest) > (size_t)limit)
+    return fail(MHM_EINVAL, "too many local rows for the refinement's per-column sums");
   if (factor_lds(lk.b, tsize) > (size_t)limit &&
       !(FPANEL + lk.b <= 127 && factor_panel_lds(lk.b, tsize) <= (size_t)limit) &&
       !factor_wide_ok(lk.b, tsize, limit))
@@ -2992,7 +3087,11 @@ static int run_solve(mhm_batch *b, const double *rhs, double *X, int ncols, int 
       TimerSpan span(MHM_TIMER_SOLVE, st);
       if (int rc = launch_solve<T>(b, ncols, b->colstate, b->bk.iters, st)) return rc;
     }
-    hipLaunchKernelGGL((k_saddle_apply<T>), dim3(b->M), dim3(256), 0, st, L, b->bk, ncols, (const T *)b->y,
+    if (saddle_lds(L.n, ncols) > 65536)
+      HIP_TRY(hipFuncSetAttribute((const void *)k_saddle_apply<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)saddle_lds(L.n, ncols)));
+    hipLaunchKernelGGL((k_saddle_apply<T>), dim3(b->M), dim3(256), saddle_lds(L.n, ncols), st, L, b->bk, ncols,
+                       (const T *)b->y,
                        (const double *)b->beta, X, it, (const int *)b->colstate, b->colnorm);
     LAUNCH_CHECK("k_saddle_apply");
   }
@@ -3051,6 +3150,56 @@ int mhm_batch_solve(mhm_batch *b, const double *rhs, double *x, int32_t ncols, i
   hipLaunchKernelGGL(k_permute_cols, grid2((size_t)b->N * ncols, 256, b->M), dim3(256), 0, st, L.perm, L.n,
                      b->N, ncols, (const double *)b->xu, x, 0);
   LAUNCH_CHECK("k_permute_cols");
+  return MHM_OK;
+}
+
+template <int NB, int D>
+static void launch_source_load(const mhm_batch *b, const double *fq, double fc, int ncols, size_t shm,
+                               hipStream_t st) {
+  hipLaunchKernelGGL((k_source_load<NB, D>), dim3(b->M, ncols), dim3(256), shm, st, b->layout->k, b->M, ncols, fq,
+                     fc, b->rhsu, b->N);
+}
+
+int mhm_batch_sources(mhm_batch *b, const double *fq, double f_const, int32_t ncols, int32_t max_refine,
+                      double rtol, double *loads, double *source, void *stream) {
+  if (!b || !loads) return fail(MHM_EINVAL, "batch/loads is NULL");
+  if (!b->factored) return fail(MHM_EINVAL, "batch is not factored");
+  if (ncols <= 0 || ncols > b->maxcols) return fail(MHM_EINVAL, "ncols outside [1, max_columns]");
+  if (max_refine < 0 || max_refine > 64 || !(rtol > 0.0)) return fail(MHM_EINVAL, "bad refinement arguments");
+  const LayoutK &L = b->layout->k;
+  if (L.ne <= 0) return fail(MHM_EINVAL, "layout has no assembly description");
+  const size_t shm = (size_t)L.nb * L.ne * sizeof(double);
+  if (shm > (size_t)lds_limit(b->layout->device)) return fail(MHM_EINVAL, "too many element loads per macro");
+  DeviceGuard guard(b->layout->device);
+  hipStream_t st = (hipStream_t)stream;
+  if (L.dim == 2 && L.nb == 3) {
+    if (shm > 65536)
+      HIP_TRY(hipFuncSetAttribute((const void *)k_source_load<3, 2>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)shm));
+    launch_source_load<3, 2>(b, fq, f_const, ncols, shm, st);
+  } else if (L.dim == 2 && L.nb == 6) {
+    if (shm > 65536)
+      HIP_TRY(hipFuncSetAttribute((const void *)k_source_load<6, 2>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)shm));
+    launch_source_load<6, 2>(b, fq, f_const, ncols, shm, st);
+  } else if (L.dim == 3 && L.nb == 4) {
+    if (shm > 65536)
+      HIP_TRY(hipFuncSetAttribute((const void *)k_source_load<4, 3>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)shm));
+    launch_source_load<4, 3>(b, fq, f_const, ncols, shm, st);
+  } else {
+    if (shm > 65536)
+      HIP_TRY(hipFuncSetAttribute((const void *)k_source_load<10, 3>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)shm));
+    launch_source_load<10, 3>(b, fq, f_const, ncols, shm, st);
+  }
+  LAUNCH_CHECK("k_source_load");
+  int rc = b->precision == MHM_FP64 ? run_solve<double>(b, b->rhsu, b->xu, ncols, max_refine, rtol, st)
+                                    : run_solve<float>(b, b->rhsu, b->xu, ncols, max_refine, rtol, st);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_source_condense, dim3(b->M, ncols), dim3(256), 0, st, L, b->bk, ncols,
+                     (const double *)b->xu, (const double *)b->rhsu, loads, source);
+  LAUNCH_CHECK("k_source_condense");
   return MHM_OK;
 }
 

@@ -149,6 +149,35 @@ class LocalBatch:
             self.raise_on_failure("solve")
         return x
 
+    def sources(self, fq=None, f_const: float = 0.0, *, ncols: int | None = None, max_refine: int | None = None,
+                rtol: float = 1e-10, source: bool = False, check: bool = True):
+        """Condensed loads of new source fields against the resident factors.
+
+        ``fq`` holds ``(ncols, M, elements, q)`` source values at the quadrature
+        points (``None``: ``f_const`` everywhere, ``ncols`` fields).  Returns the
+        loads ``(ncols, M, m+k)`` and, with ``source=True``, the source responses
+        ``(ncols, M, n)`` in original rows (``OfflineLocalProblem.response``,
+        ``core/offline.py:121-127``; ``local_condensed_load``,
+        ``core/reconstruction.py:91-127``).
+        """
+        if max_refine is None:
+            max_refine = DEFAULT_REFINE[self.precision]
+        if fq is not None:
+            fq = self._dev(fq, torch.float64)
+            if fq.dim() != 4 or fq.shape[1] != self.macros:
+                raise ValueError("fq must have shape (fields, macros, elements, q)")
+            ncols = int(fq.shape[0])
+        elif ncols is None:
+            raise ValueError("ncols is required without fq")
+        loads = torch.empty((ncols, self.macros, self.width), dtype=torch.float64, device=self.device)
+        src = (torch.empty((ncols, self.macros, self.n), dtype=torch.float64, device=self.device)
+               if source else None)
+        _lib.call("mhm_batch_sources", self.handle, _ptr(fq), float(f_const), int(ncols), int(max_refine),
+                  float(rtol), _ptr(loads), _ptr(src), _stream())
+        if check:
+            self.raise_on_failure("sources")
+        return (loads, src) if source else loads
+
     def reconstruct(self, trace, coarse=None):
         """Fields ``(M, cases, n)`` from local traces ``(M, cases, m)`` and amplitudes ``(M, cases, k)``."""
         trace = self._dev(trace, torch.float64)
