@@ -174,3 +174,88 @@ def condense_spec(spec):
         blocks.append(blk)
         loads.append(ld)
     return spec["macros"], np.array(blocks), np.array(loads)
+
+
+def _macro_problems(spec):
+    """(c, LocalProblem, mesh, fq row or None) of a spec's macros, assembled as in condense_spec."""
+    if spec.get("kind") == "cube":
+        full_kq = {c: spec["kq"][r] for r, c in enumerate(spec["macros"])}
+        full_fq = None if spec["fq"] is None else {c: spec["fq"][r] for r, c in enumerate(spec["macros"])}
+        problems, _ = build_cube_problems(_SpecCase(spec, full_kq, full_fq), spec["macros"])
+        return [(c, p, mesh) for c, p, mesh in problems]
+    key = (spec["nx"], spec["breaks"], spec["degrees"])
+    if key not in _MACRO_CACHE:
+        pm = PolygonMesh.unit_square_grid(spec["nx"])
+        space = FaceSpace(spec["breaks"], spec["degrees"])
+        _MACRO_CACHE[key] = (pm, SkeletonSpace(pm, [space] * len(pm.faces)))
+    pm, skel = _MACRO_CACHE[key]
+    out = []
+    for r, c in enumerate(spec["macros"]):
+        fine = pm.submesh(c, spec["refinement"])
+        fq = None if spec["fq"] is None else spec["fq"][r]
+        A, M, f = scalar_operators(fine, spec["degree"], diffusion_q=spec["kq"][r], source_q=fq,
+                                   source=spec["f_const"], order=spec["order"])
+        B = trace_coupling(pm, c, fine, skel, spec["degree"])
+        Z = np.ones((A.shape[0], 1))
+        mz = M @ Z
+        out.append((c, LocalProblem(A, B, f, skel.cell_dofs(c), Z, mz / (Z.T @ mz).item()), fine))
+    return out
+
+
+def _load_only(spec, mesh, fq):
+    """The load vector of source values ``fq (cells, q)`` - the load part of
+    scalar_operators / tetra_operators alone (same formulas)."""
+    from .fem import element_tabulate, nodal_space, triangle_quadrature
+    from .fem3d import tetra_element_tabulate, tetra_nodal_space, tetrahedron_quadrature
+
+    degree, order = spec["degree"], spec["order"]
+    if spec.get("kind") == "cube":
+        bary, w = tetrahedron_quadrature(max(order, degree + 2))
+        vals, _ = tetra_element_tabulate(mesh, degree, bary)
+        dofs, nodes = tetra_nodal_space(mesh, degree)
+        load_e = np.zeros((len(mesh.cells), vals.shape[1]))
+        for q in range(len(w)):
+            load_e = load_e + ((w[q] * vals[q])[None, :] * fq[:, q, None]) * mesh.volumes[:, None]
+        load = np.zeros(len(nodes))
+        np.add.at(load, dofs, load_e)
+        return load
+    bary, w = triangle_quadrature(max(order, degree + 2))
+    vals, _ = element_tabulate(mesh, degree, bary)
+    dofs, nodes = nodal_space(mesh, degree)
+    ncell, nq = len(mesh.cells), len(w)
+    basis = np.broadcast_to(vals, (ncell, nq, vals.shape[1]))
+    load = np.einsum("tq,tqi,tq,t->ti", np.broadcast_to(w, (ncell, nq)), basis, fq, mesh.areas)
+    return np.bincount(dofs.ravel(), weights=load.ravel(), minlength=len(nodes))
+
+
+def serving_cost_spec(spec):
+    """CPU seconds, summed over the spec's macros, of the two request kinds of
+    the serving workload: a coefficient request (local assembly and
+    condensation, as condense_spec) and a source request against a resident
+    factorization (the new load, one refined solve of [f; 0], the condensed
+    load).  The source values are the spec's ``src`` rows."""
+    import time
+
+    from .local import Factorization
+
+    t_coef = t_src = 0.0
+    problems = _macro_problems(spec)  # warm the mesh caches
+    t0 = time.perf_counter()
+    problems = _macro_problems(spec)
+    for _, p, _ in problems:
+        p.condense()
+    t_coef = time.perf_counter() - t0
+    for r, (c, p, mesh) in enumerate(problems):
+        fac = Factorization(p.condensation_matrix())
+        t0 = time.perf_counter()
+        load = _load_only(spec, mesh, spec["src"][r])
+        rhs = np.zeros(p.n + p.k)
+        rhs[:p.n] = load
+        s = fac.solve(rhs)[:p.n]
+        tr = p.test_coupling.T @ s
+        co = -p.kernel.T @ load
+        if p.correct_kernel:
+            co = p.test_action.T @ s + co
+        np.concatenate((tr, co))
+        t_src += time.perf_counter() - t0
+    return len(problems), t_coef, t_src

@@ -437,11 +437,12 @@ __global__ void __launch_bounds__(256)
 // element loads sum_q ((w_q phi_i) f_q) |T| in quadrature order (the same
 // operations as k_assemble, so equal loads give equal bits) staged in LDS, then
 // the ordered load gather of k_gather into column c of the solve's right-hand
-// side rhs[mac][N][ncols] (band rows; the constraint row is 0).
+// side rhs[mac][N][ncols] (band rows; the constraint row is 0), scaled by a
+// power of two per column (unscale[mac][c] undoes it).
 template <int NB, int D>
 __global__ void __launch_bounds__(256)
     k_source_load(LayoutK L, int M, int ncols, const double *__restrict__ fq, double fconst,
-                  double *__restrict__ rhs, int N) {
+                  double *__restrict__ rhs, int N, double *__restrict__ unscale) {
 #pragma clang fp contract(off)
   extern __shared__ double fe[];  // NB x ne, code i*ne + e
   const int mac = blockIdx.x, c = blockIdx.y;
@@ -456,12 +457,24 @@ __global__ void __launch_bounds__(256)
   }
   __syncthreads();
   double *out = rhs + (size_t)mac * N * ncols + c;
+  double amax = 0.0;
   for (int row = threadIdx.x; row < N; row += blockDim.x) {
     double f = 0.0;
     if (row < L.n)
       for (int t = L.load_ptr[row]; t < L.load_ptr[row + 1]; ++t) f += fe[L.load_code[t]];
     out[(size_t)row * ncols] = f;
+    amax = fmax(amax, fabs(f));
   }
+  // scale the column by 2^-e, max|f| = m 2^e, m in [0.5, 1): exact, and it keeps
+  // a source that is tiny on this macro inside the binary32 range of the mixed
+  // solve; the results are scaled back by 2^e
+  __shared__ double red[4];
+  amax = block_max<256>(amax, red);
+  int e = 0;
+  if (amax > 0.0 && isfinite(amax)) (void)frexp(amax, &e);
+  const double scale = ldexp(1.0, -e);
+  for (int row = threadIdx.x; row < N; row += blockDim.x) out[(size_t)row * ncols] *= scale;
+  if (threadIdx.x == 0) unscale[(size_t)mac * ncols + c] = ldexp(1.0, e);
 }
 
 // Condensed loads of new sources (core/reconstruction.py:91-127), block per
@@ -472,9 +485,11 @@ __global__ void __launch_bounds__(256)
 // others, so a request's result does not depend on its batch.
 __global__ void __launch_bounds__(256)
     k_source_condense(LayoutK L, BatchK B, int ncols, const double *__restrict__ X,
-                      const double *__restrict__ rhs, double *__restrict__ loads, double *__restrict__ source) {
+                      const double *__restrict__ rhs, const double *__restrict__ unscale,
+                      double *__restrict__ loads, double *__restrict__ source) {
   __shared__ double red[2][256];
   const int mac = blockIdx.x, c = blockIdx.y, tid = threadIdx.x;
+  const double us = unscale[(size_t)mac * ncols + c];  // a power of two
   const int n = L.n, m = L.m, N = B.N, w = L.m + L.k;
   const double *x = X + (size_t)mac * N * ncols + c;
   const double *f = rhs + (size_t)mac * N * ncols + c;
@@ -486,10 +501,10 @@ __global__ void __launch_bounds__(256)
       const int i = L.crows ? L.crows[q] : q;
       acc += coupling_at(L, B, mac, a, i) * x[(size_t)i * ncols];
     }
-    ld[a] = acc;
+    ld[a] = acc * us;
   }
   if (source)
-    for (int i = tid; i < n; i += 256) source[((size_t)c * B.M + mac) * n + L.perm[i]] = x[(size_t)i * ncols];
+    for (int i = tid; i < n; i += 256) source[((size_t)c * B.M + mac) * n + L.perm[i]] = x[(size_t)i * ncols] * us;
   if (!L.k) return;
   double as = 0.0, zf = 0.0;
   for (int i = tid; i < n; i += 256) {
@@ -506,7 +521,7 @@ __global__ void __launch_bounds__(256)
     }
     __syncthreads();
   }
-  if (tid == 0) ld[m] = red[0][0] - red[1][0];
+  if (tid == 0) ld[m] = (red[0][0] - red[1][0]) * us;
 }
 
 // Assembled-operator path: A Z from supplied CSR values; loads permuted to band rows.
@@ -2158,7 +2173,7 @@ struct mhm_batch {
   BatchK bk{};
   void *band = nullptr, *rband = nullptr, *y = nullptr;  // rband: row-major band (wavefront solve)
   double *xstd = nullptr, *beta = nullptr, *colnorm = nullptr, *bnorm = nullptr;
-  double *rhsu = nullptr, *xu = nullptr;
+  double *rhsu = nullptr, *xu = nullptr, *srcscale = nullptr;
   int *colstate = nullptr;
   double *ework = nullptr, *fwork = nullptr;
   int *colmap = nullptr;
@@ -2776,7 +2791,8 @@ int mhm_batch_create(const mhm_layout *L, int32_t macros, int32_t precision, int
     return cleanup(rc);
   if (max_columns > 0 &&
       ((rc = dev_alloc(b->owned, &b->rhsu, M * max_columns * N)) ||
-       (rc = dev_alloc(b->owned, &b->xu, M * max_columns * N))))
+       (rc = dev_alloc(b->owned, &b->xu, M * max_columns * N)) ||
+       (rc = dev_alloc(b->owned, &b->srcscale, M * max_columns))))
     return cleanup(rc);
   {
     unsigned char *p = nullptr;
@@ -3157,7 +3173,7 @@ template <int NB, int D>
 static void launch_source_load(const mhm_batch *b, const double *fq, double fc, int ncols, size_t shm,
                                hipStream_t st) {
   hipLaunchKernelGGL((k_source_load<NB, D>), dim3(b->M, ncols), dim3(256), shm, st, b->layout->k, b->M, ncols, fq,
-                     fc, b->rhsu, b->N);
+                     fc, b->rhsu, b->N, b->srcscale);
 }
 
 int mhm_batch_sources(mhm_batch *b, const double *fq, double f_const, int32_t ncols, int32_t max_refine,
@@ -3198,7 +3214,7 @@ int mhm_batch_sources(mhm_batch *b, const double *fq, double f_const, int32_t nc
                                     : run_solve<float>(b, b->rhsu, b->xu, ncols, max_refine, rtol, st);
   if (rc) return rc;
   hipLaunchKernelGGL(k_source_condense, dim3(b->M, ncols), dim3(256), 0, st, L, b->bk, ncols,
-                     (const double *)b->xu, (const double *)b->rhsu, loads, source);
+                     (const double *)b->xu, (const double *)b->rhsu, (const double *)b->srcscale, loads, source);
   LAUNCH_CHECK("k_source_condense");
   return MHM_OK;
 }

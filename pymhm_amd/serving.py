@@ -111,13 +111,15 @@ def bump_sources(points, seeds, bumps: int = 8, width=(0.02, 0.08)):
 
 
 def coefficient_field(base_kq, seed: int):
-    """A new log-normal coefficient: ``exp(a log K + b)`` of the base field with
-    seeded a in [0.5, 1.5] and b in [-1, 1]."""
+    """A new coefficient from the base field's family: ``exp(s log K + b)`` with
+    a seeded sign s and shift b in [-1, 1].  A centred Gaussian log K keeps its
+    covariance under the sign flip, and a two-valued K keeps its contrast, so
+    the new field is as well conditioned as the base one."""
     import torch
 
     rng = np.random.default_rng(int(seed))
-    a, b = rng.uniform(0.5, 1.5), rng.uniform(-1.0, 1.0)
-    return torch.exp(a * torch.log(base_kq) + b)
+    sgn, b = (1.0 if rng.random() < 0.5 else -1.0), rng.uniform(-1.0, 1.0)
+    return torch.exp(sgn * torch.log(base_kq) + b)
 
 
 class Server:
@@ -125,15 +127,21 @@ class Server:
 
     ``cases`` are workload cases (``pymhm_amd.workloads``); each gets one
     batch with ``max_columns`` re-solve columns, assembled, factored and
-    condensed with its base coefficient before the first request.
+    condensed with its base coefficient before the first request.  With
+    ``source_pool`` > 0 each problem keeps that many seeded source fields on
+    the device, built up front, and a source request takes field
+    ``seed mod source_pool`` (the field evaluation stays out of the served
+    time); otherwise the field is evaluated per request.
     """
 
-    def __init__(self, cases, precision: str = "fp64", max_columns: int = 8):
+    def __init__(self, cases, precision: str = "fp64", max_columns: int = 8, source_pool: int = 0):
         import torch
 
         self.cases = list(cases)
         self.precision = precision
         self.max_columns = int(max_columns)
+        self.source_pool = int(source_pool)
+        self.pools = []
         self.batches, self.base_kq, self.points, self.current_kq, self.fq = [], [], [], [], []
         for case in self.cases:
             prob = case.problem
@@ -145,8 +153,18 @@ class Server:
             self.base_kq.append(kq)
             self.points.append(pts)
             self.current_kq.append(kq)
+            if self.source_pool:
+                self.pools.append(bump_sources(pts, [1000003 * (len(self.pools) + 1) + k
+                                                     for k in range(self.source_pool)]))
             self._stage(len(self.batches) - 1, kq)
         torch.cuda.synchronize()
+
+    def source_fields(self, p: int, seeds):
+        """Source fields of the given request seeds for problem p."""
+        if self.source_pool:
+            idx = [int(s) % self.source_pool for s in seeds]
+            return self.pools[p][idx]
+        return bump_sources(self.points[p], seeds)
 
     def _stage(self, p: int, kq):
         case, b = self.cases[p], self.batches[p]
@@ -162,7 +180,7 @@ class Server:
             kq = coefficient_field(self.base_kq[p], req.seed)
             self.current_kq[p] = kq
             return [self._stage(p, kq)]
-        fq = bump_sources(self.points[p], [r.seed for r in reqs])
+        fq = self.source_fields(p, [r.seed for r in reqs])
         loads = self.batches[p].sources(fq)
         return [loads[c] for c in range(len(reqs))]
 
@@ -199,7 +217,7 @@ class Server:
     def close(self):
         for b in self.batches:
             b.close()
-        self.batches = []
+        self.batches, self.pools = [], []
 
 
 def latency_stats(requests: list[Request]) -> dict:

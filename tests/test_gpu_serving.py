@@ -60,7 +60,7 @@ def test_served_results_equal_alone():
     cases = [pw.config1(nx=4), pw.config3(nx=2, refinement=2)]
     print("serving: 14 requests over two problems", flush=True)
     reqs = serving.poisson_requests(14, rate=1000.0, nproblems=2, coefficient_fraction=0.3, seed=5)
-    srv = serving.Server(cases, "fp64", max_columns=4)
+    srv = serving.Server(cases, "fp64", max_columns=4, source_pool=5)
     try:
         done = srv.run(reqs, realtime=False)
         assert all(np.isfinite(r.done) for r in done)
@@ -81,7 +81,7 @@ def test_served_results_equal_alone():
                         blocks, loads = b.condense()
                         assert torch.equal(blocks, r.result[0]) and torch.equal(loads, r.result[1]), r.rid
                     else:
-                        fq = serving.bump_sources(srv.points[p], [r.seed])
+                        fq = srv.source_fields(p, [r.seed])
                         assert torch.equal(b.sources(fq)[0], r.result), r.rid
         stats = serving.latency_stats(done)
         assert stats["requests"] == 14 and stats["p99_ms"] >= stats["p50_ms"]
