@@ -183,6 +183,16 @@ int mhm_batch_responses(mhm_batch *batch, double *d_source, double *d_lifts, dou
 int mhm_batch_solve(mhm_batch *batch, const double *d_rhs, double *d_x, int32_t ncols,
                     int32_t max_refine, double rtol, void *stream);
 
+/* Per-macro element geometry for macros that share a local topology but not
+ * a shape (uneven polygonal macros): d_grad_lambda[macro][element][dim+1][dim]
+ * (barycentric gradients) and d_measure[macro][element] replace the layout's
+ * for mhm_batch_assemble and mhm_batch_sources.  Replaces the per-cell
+ * geometry of physical_simplex_tabulation (fem/reference.py:763-820) and
+ * TriangleMesh.areas (meshes/triangle.py:122-147) for cells of
+ * PolygonMesh.submesh (meshes/polygonal.py:217-265). */
+int mhm_batch_set_geometry(mhm_batch *batch, const double *d_grad_lambda, const double *d_measure,
+                           void *stream);
+
 /* New source fields against the resident factors, `ncols` at once (the online
  * step: OfflineLocalProblem.response, core/offline.py:121-127, with
  * LocalProblem.with_load, core/contracts.py:275-298).  For each field c the

@@ -81,10 +81,41 @@ def build_problems(case, macros=None):
     return out, skel
 
 
+def build_polygon_problems(case, macros=None):
+    """LocalProblems of a polygonal macro workload with per-macro refinement:
+    ``PolygonMesh`` of the macro cells (ear triangulation, first-encounter
+    faces and signs, ``meshes/polygonal.py:83-190``), ``submesh(c, r_c)``
+    (``:217-265``), ``scalar_operators`` and ``trace_coupling`` as the primal
+    Darcy provider (``_legacy/models/darcy/primal.py:55-78``)."""
+    prob = case.problem
+    pm = PolygonMesh(prob.macro.points, [list(cell) for cell in prob.macro.cells])
+    sp = prob.space
+    skel = SkeletonSpace(pm, [FaceSpace(sp.breaks, sp.degrees)] * len(pm.faces))
+    out = []
+    for c in (range(len(pm.cells)) if macros is None else macros):
+        fine = pm.submesh(c, int(prob.refinements[c]))
+        fq = None if case.fq is None else case.fq[c]
+        A, M, f = scalar_operators(fine, prob.degree, diffusion_q=case.kq[c], source_q=fq,
+                                   source=case.f_const, order=prob.order)
+        B = trace_coupling(pm, c, fine, skel, prob.degree)
+        Z = np.ones((A.shape[0], 1))
+        mz = M @ Z
+        out.append((c, LocalProblem(A, B, f, skel.cell_dofs(c), Z, mz / (Z.T @ mz).item()), fine))
+    return out, skel
+
+
+def _stack(rows):
+    """An array when every row has one shape, else the list of rows."""
+    return np.array(rows) if len({np.shape(r) for r in rows}) <= 1 else list(rows)
+
+
 def oracle_config(case, macros=None):
     """Blocks, loads and responses of the selected macros (default: all)."""
     if hasattr(case.problem, "split"):
         problems, trace_size = build_cube_problems(case, macros)
+    elif hasattr(case.problem, "buckets"):
+        problems, skel = build_polygon_problems(case, macros)
+        trace_size = skel.size
     else:
         problems, skel = build_problems(case, macros)
         trace_size = skel.size
@@ -98,8 +129,8 @@ def oracle_config(case, macros=None):
         lifts.append(r.lifts)
         ret.append(r.retained_basis[:, 0])
         corr.append(p.correct_kernel)
-    return dict(blocks=np.array(blocks), loads=np.array(loads), source=np.array(src),
-                lifts=np.array(lifts), retained=np.array(ret), correct_kernel=np.array(corr),
+    return dict(blocks=_stack(blocks), loads=_stack(loads), source=_stack(src),
+                lifts=_stack(lifts), retained=_stack(ret), correct_kernel=np.array(corr),
                 trace_size=trace_size, problems=problems)
 
 

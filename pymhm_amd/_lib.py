@@ -22,7 +22,7 @@ EXPORTS = (
     "mhm_last_error", "mhm_device_count", "mhm_version",
     "mhm_layout_create", "mhm_layout_destroy", "mhm_layout_info",
     "mhm_batch_create", "mhm_batch_destroy", "mhm_batch_assemble", "mhm_batch_set_operator",
-    "mhm_batch_operator", "mhm_batch_set_coupling", "mhm_batch_factor", "mhm_batch_condense", "mhm_batch_responses",
+    "mhm_batch_operator", "mhm_batch_set_coupling", "mhm_batch_set_geometry", "mhm_batch_factor", "mhm_batch_condense", "mhm_batch_responses",
     "mhm_batch_solve", "mhm_batch_sources", "mhm_batch_reconstruct", "mhm_batch_status",
     "mhm_batch_iterations", "mhm_scatter_coordinates", "mhm_ordered_gather",
     "mhm_timing_enable", "mhm_timing_reset", "mhm_timing_read",
@@ -82,6 +82,7 @@ def load(path: str | os.PathLike | None = None):
         "mhm_batch_set_operator": (P, P, P, P),
         "mhm_batch_operator": (P, P, P, P, P),
         "mhm_batch_set_coupling": (P, P, P, P, P, P, P),
+        "mhm_batch_set_geometry": (P, P, P, P),
         "mhm_batch_factor": (P, P),
         "mhm_batch_condense": (P, I32, D, P, P, P),
         "mhm_batch_responses": (P, P, P, P, P),

@@ -263,6 +263,7 @@ struct BatchK {
   int *status, *iters;
   const int *colmap;                          // per-macro signed column map or null
   const double *colscale, *bcoup, *cons, *kern;  // per-macro overrides (band rows) or null
+  const double *glam, *meas;                     // per-macro element geometry or null
 };
 
 __device__ __forceinline__ double coupling_at(const LayoutK &L, const BatchK &B, int mac, int a,
@@ -320,7 +321,8 @@ __device__ __forceinline__ void comp_add(double &total, double &corr, double val
 template <int NB, int D>
 __global__ void __launch_bounds__(64 * ASM_WAVES)
     k_assemble(LayoutK L, int M, const double *__restrict__ kq, const double *__restrict__ fq,
-               double fconst, double *__restrict__ ework, double *__restrict__ fwork) {
+               double fconst, double *__restrict__ ework, double *__restrict__ fwork,
+               const double *__restrict__ bglam, const double *__restrict__ bmeas) {
 #pragma clang fp contract(off)
   constexpr int EPW = 64 / NB;
   constexpr int DP = D == 3 ? 4 : D;  // padded row of k grad(phi) in LDS
@@ -343,13 +345,19 @@ __global__ void __launch_bounds__(64 * ASM_WAVES)
   const bool own = lane < EPW * NB && e0 + el < L.ne;
   const int e = min(e0 + el, L.ne - 1);
   double *kgb = sh + tab + wv * 2 * KG;
+  // element geometry: the layout's, or the macro's own (uneven macros)
   double gl[D + 1][D];
+  double meas = 0.0;
+  auto geometry = [&](const double *g, const double *ms) {
 #pragma unroll
-  for (int l = 0; l <= D; ++l)
+    for (int l = 0; l <= D; ++l)
 #pragma unroll
-    for (int a = 0; a < D; ++a) gl[l][a] = L.glam[((size_t)e * (D + 1) + l) * D + a];
-  const double meas = L.measure[e];
+      for (int a = 0; a < D; ++a) gl[l][a] = g[((size_t)e * (D + 1) + l) * D + a];
+    meas = ms[e];
+  };
+  if (!bglam) geometry(L.glam, L.measure);
   for (int mac = blockIdx.y; mac < M; mac += gridDim.y) {
+    if (bglam) geometry(bglam + (size_t)mac * L.ne * (D + 1) * D, bmeas + (size_t)mac * L.ne);
     double tot[NB], cor[NB];
 #pragma unroll
     for (int j = 0; j < NB; ++j) tot[j] = cor[j] = 0.0;
@@ -442,7 +450,8 @@ __global__ void __launch_bounds__(256)
 template <int NB, int D>
 __global__ void __launch_bounds__(256)
     k_source_load(LayoutK L, int M, int ncols, const double *__restrict__ fq, double fconst,
-                  double *__restrict__ rhs, int N, double *__restrict__ unscale) {
+                  double *__restrict__ rhs, int N, double *__restrict__ unscale,
+                  const double *__restrict__ bmeas) {
 #pragma clang fp contract(off)
   extern __shared__ double fe[];  // NB x ne, code i*ne + e
   const int mac = blockIdx.x, c = blockIdx.y;
@@ -450,7 +459,7 @@ __global__ void __launch_bounds__(256)
   for (int t = threadIdx.x; t < NB * L.ne; t += blockDim.x) {
     const int i = t / L.ne, e = t - i * L.ne;
     const double *fp = fq ? fq + (((size_t)c * M + mac) * L.ne + e) * nq : nullptr;
-    const double meas = L.measure[e];
+    const double meas = bmeas ? bmeas[(size_t)mac * L.ne + e] : L.measure[e];
     double F = 0.0;
     for (int q = 0; q < nq; ++q) F = F + ((L.qw[q] * L.vtab[q * NB + i]) * (fp ? fp[q] : fconst)) * meas;
     fe[t] = F;
@@ -2174,6 +2183,7 @@ struct mhm_batch {
   void *band = nullptr, *rband = nullptr, *y = nullptr;  // rband: row-major band (wavefront solve)
   double *xstd = nullptr, *beta = nullptr, *colnorm = nullptr, *bnorm = nullptr;
   double *rhsu = nullptr, *xu = nullptr, *srcscale = nullptr;
+  double *glam = nullptr, *meas = nullptr;  // per-macro element geometry (uneven macros)
   int *colstate = nullptr;
   double *ework = nullptr, *fwork = nullptr;
   int *colmap = nullptr;
@@ -2840,7 +2850,7 @@ static int launch_assemble(const mhm_batch *b, const double *kq, const double *f
   const int tasks = (L.ne + EPW - 1) / EPW;
   const dim3 grid((unsigned)((tasks + ASM_WAVES - 1) / ASM_WAVES), (unsigned)std::min(b->M, 2048));
   hipLaunchKernelGGL((k_assemble<NB, D>), grid, dim3(64 * ASM_WAVES), shm, st, L, b->M, kq, fq, fc, b->ework,
-                     b->fwork);
+                     b->fwork, b->bk.glam, b->bk.meas);
   LAUNCH_CHECK("k_assemble");
   return MHM_OK;
 }
@@ -2969,6 +2979,24 @@ int mhm_batch_set_coupling(mhm_batch *b, const int32_t *colmap, const double *co
   if (!L.coupling && !b->bk.bcoup && L.m > 0)
     return fail(MHM_EINVAL, "no coupling: the layout has none and no per-macro couplings were set");
   if (L.k && !L.constraint && !b->bk.cons) return fail(MHM_EINVAL, "no constraint vectors");
+  return MHM_OK;
+}
+
+int mhm_batch_set_geometry(mhm_batch *b, const double *grad_lambda, const double *measure, void *stream) {
+  if (!b || !grad_lambda || !measure) return fail(MHM_EINVAL, "batch/grad_lambda/measure is NULL");
+  const LayoutK &L = b->layout->k;
+  if (L.ne <= 0) return fail(MHM_EINVAL, "layout has no assembly description");
+  DeviceGuard guard(b->layout->device);
+  hipStream_t st = (hipStream_t)stream;
+  const size_t ng = (size_t)b->M * L.ne * (L.dim + 1) * L.dim, nm = (size_t)b->M * L.ne;
+  int rc;
+  if (!b->glam && ((rc = dev_alloc(b->owned, &b->glam, ng)) || (rc = dev_alloc(b->owned, &b->meas, nm))))
+    return rc;
+  HIP_TRY(hipMemcpyAsync(b->glam, grad_lambda, ng * sizeof(double), hipMemcpyDeviceToDevice, st));
+  HIP_TRY(hipMemcpyAsync(b->meas, measure, nm * sizeof(double), hipMemcpyDeviceToDevice, st));
+  b->bk.glam = b->glam;
+  b->bk.meas = b->meas;
+  b->factored = b->condensed = false;
   return MHM_OK;
 }
 
@@ -3173,7 +3201,7 @@ template <int NB, int D>
 static void launch_source_load(const mhm_batch *b, const double *fq, double fc, int ncols, size_t shm,
                                hipStream_t st) {
   hipLaunchKernelGGL((k_source_load<NB, D>), dim3(b->M, ncols), dim3(256), shm, st, b->layout->k, b->M, ncols, fq,
-                     fc, b->rhsu, b->N, b->srcscale);
+                     fc, b->rhsu, b->N, b->srcscale, b->bk.meas);
 }
 
 int mhm_batch_sources(mhm_batch *b, const double *fq, double f_const, int32_t ncols, int32_t max_refine,

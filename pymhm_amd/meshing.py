@@ -88,6 +88,73 @@ def square_grid(nx: int, ny: int | None = None) -> MacroMesh:
     return macro_topology(pts, polys)
 
 
+def voronoi_square(npts: int, seed: int, min_edge: float | None = None) -> MacroMesh:
+    """Voronoi cells of ``npts`` seeded uniform points, bounded by the unit
+    square: the seeds are mirrored across the four sides, so the bisectors
+    with the mirrors are the sides themselves.  Vertices within 1e-12 of each
+    other are merged and snapped onto the sides; cells are counterclockwise.
+
+    Edges shorter than ``min_edge`` (default 0.05 / sqrt(npts), a twentieth of
+    the mean seed spacing) are collapsed - a Voronoi diagram of uniform points
+    has edges down to 1e-6 whose fan triangles have angles of 1e-3 degrees.
+    The endpoints merge into a corner, else a side vertex, else the midpoint."""
+    from scipy.spatial import Voronoi
+
+    rng = np.random.default_rng(seed)
+    pts = rng.random((npts, 2))
+    mirrors = [pts, pts * (-1, 1), pts * (-1, 1) + (2, 0), pts * (1, -1), pts * (1, -1) + (0, 2)]
+    vor = Voronoi(np.concatenate(mirrors))
+    verts = vor.vertices.copy()
+    verts[np.abs(verts) < 1e-12] = 0.0
+    verts[np.abs(verts - 1.0) < 1e-12] = 1.0
+    key = np.round(verts / 1e-12).astype(np.int64)
+    _, first, inverse = np.unique(key, axis=0, return_index=True, return_inverse=True)
+    polys = []
+    for i in range(npts):
+        region = vor.regions[vor.point_region[i]]
+        if -1 in region or not region:
+            raise ValueError("unbounded Voronoi region inside the mirrored set")
+        idx = [int(inverse[v]) for v in region]
+        idx = [v for k, v in enumerate(idx) if v != idx[k - 1]]
+        polys.append(idx)
+    points = verts[first]
+    tol = 0.05 / np.sqrt(npts) if min_edge is None else float(min_edge)
+    while True:  # collapse short edges until none is left
+        parent = np.arange(len(points))
+
+        def find(v):
+            while parent[v] != v:
+                parent[v] = parent[parent[v]]
+                v = parent[v]
+            return v
+
+        short = False
+        for poly in polys:
+            for u, v in zip(poly, poly[1:] + poly[:1]):
+                ru, rv = find(u), find(v)
+                if ru != rv and np.linalg.norm(points[ru] - points[rv]) < tol:
+                    on = [np.sum((points[w] == 0.0) | (points[w] == 1.0)) for w in (ru, rv)]
+                    keep, drop = (ru, rv) if on[0] >= on[1] else (rv, ru)
+                    if on[0] == on[1] == 0:
+                        points[keep] = 0.5 * (points[ru] + points[rv])
+                    parent[drop] = keep
+                    short = True
+        if not short:
+            break
+        merged = []
+        for poly in polys:
+            idx = [find(v) for v in poly]
+            idx = [v for k, v in enumerate(idx) if v != idx[k - 1]]
+            if len(idx) < 3:
+                raise ValueError("edge collapse removed a cell")
+            merged.append(idx)
+        polys = merged
+    used = np.unique(np.concatenate(polys))
+    renum = -np.ones(len(points), dtype=np.int64)
+    renum[used] = np.arange(len(used))
+    return macro_topology(points[used], [renum[p] for p in polys])
+
+
 # ----------------------------------------------------------------- local meshes
 def ear_split(xy):
     """Ear clipping of a simple counterclockwise polygon (first valid ear, repeatedly)."""

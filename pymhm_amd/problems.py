@@ -181,3 +181,136 @@ class CubeGridProblem:
         b = LocalBatch(self.layout, len(self.colmap[sl]), precision=precision, max_columns=max_columns)
         b.set_coupling(colmap=self.colmap[sl], colscale=self.colscale[sl])
         return b
+
+
+class PolygonProblem:
+    """Primal MHM scalar diffusion on a polygonal macro mesh with uneven macros.
+
+    Every macro is ear-split (``meshes/polygonal.py:33-57``) and each ear
+    refined into ``r^2`` triangles (``:217-265``), with ``r`` chosen per macro;
+    the multipliers are ``face_space`` on every macroface.  Macros whose ear
+    triangulation and refinement agree have the same local topology, so they
+    form a **bucket** that shares one layout (pattern, band ordering, assembly
+    plan) and one batch; the geometry (barycentric gradients, element
+    measures), the constraints ``M 1 / 1^T M 1`` and the couplings are per
+    macro.  Inside a bucket every macro's boundary nodes sit at the same
+    fractions of its sides, so a side's coupling is the bucket's unit-length
+    template times the side length, with the macroface orientation as a signed
+    column map (``fem/scalar/triangle.py:160-205``).
+    """
+
+    def __init__(self, macro: meshing.MacroMesh, refinements, degree: int = 1,
+                 face_space: meshing.FaceSpace = meshing.FaceSpace.uniform(1, 1), order: int = 4, device: int = 0):
+        self.macro = macro
+        self.degree = int(degree)
+        self.space = face_space
+        self.order = max(int(order), self.degree + 2)
+        self.M = len(macro.cells)
+        self.refinements = np.asarray(refinements, dtype=np.int64)
+        if self.refinements.shape != (self.M,) or self.refinements.min() < 1:
+            raise ValueError("one refinement >= 1 per macro")
+        fs = face_space.size
+        self.trace_size = len(macro.faces) * fs
+        groups = {}
+        for c, cell in enumerate(macro.cells):
+            tris = meshing.ear_split(macro.points[cell])
+            key = (len(cell), int(self.refinements[c]), tuple(tris.ravel().tolist()))
+            groups.setdefault(key, []).append(c)
+        self.qbary, self.qw = elements.triangle_rule(self.order)
+        self.vals, self.der = elements.triangle_tables(self.degree, self.qbary)
+        self.buckets = [PolygonBucket(self, groups[key], key, device) for key in sorted(groups)]
+        self.bucket_of = np.empty(self.M, dtype=np.int64)
+        self.slot_of = np.empty(self.M, dtype=np.int64)
+        for bi, bk in enumerate(self.buckets):
+            self.bucket_of[bk.cells] = bi
+            self.slot_of[bk.cells] = np.arange(len(bk.cells))
+
+    def quadrature_points(self):
+        """Per macro ``(ne_c, nq, 2)`` physical quadrature points (a list)."""
+        out = [None] * self.M
+        for bk in self.buckets:
+            for r, c in enumerate(bk.cells):
+                out[c] = bk.qpts[r]
+        return out
+
+    def global_indices(self):
+        """Per macro ``[trace_dofs, coarse_dof]`` (a list: widths differ)."""
+        out = [None] * self.M
+        for bk in self.buckets:
+            for r, c in enumerate(bk.cells):
+                out[c] = np.concatenate((bk.trace_dofs[r], [self.trace_size + c])).astype(np.int64)
+        return out
+
+    def kernel_offsets(self):
+        return self.trace_size + np.arange(self.M + 1, dtype=np.int64)
+
+
+class PolygonBucket:
+    """Macros of one local topology: a shared layout, per-macro geometry."""
+
+    def __init__(self, prob: PolygonProblem, cells, key, device: int = 0):
+        k, r, flat = key
+        tris = np.asarray(flat, dtype=np.int64).reshape(-1, 3)
+        macro, fs = prob.macro, prob.space.size
+        self.cells = np.asarray(cells, dtype=np.int64)
+        self.k, self.r = k, r
+        Mb = len(cells)
+        verts0 = macro.points[macro.cells[cells[0]]]
+        pts0, tri_cells = meshing.refine_triangles(verts0, tris, r)
+        mesh = meshing.local_mesh(pts0, tri_cells, prob.degree)
+        self.mesh = mesh
+        n, ne, m = mesh.n, len(mesh.cells), k * fs
+        self.n, self.m = n, m
+        lengths0 = np.linalg.norm(np.roll(verts0, -1, axis=0) - verts0, axis=1)
+        template = np.concatenate(
+            [meshing.side_coupling(mesh, verts0[s], verts0[(s + 1) % k], prob.space) / lengths0[s]
+             for s in range(k)], axis=1)
+        rev_map, rev_scale = prob.space.reversal()
+        nq = len(prob.qw)
+        self.glam = np.empty((Mb, ne, 3, 2))
+        self.meas = np.empty((Mb, ne))
+        self.qpts = np.empty((Mb, ne, nq, 2))
+        self.constraints = np.empty((Mb, n))
+        self.colmap = np.empty((Mb, m), dtype=np.int32)
+        self.colscale = np.empty((Mb, m))
+        self.trace_dofs = np.empty((Mb, m), dtype=np.int64)
+        for i, c in enumerate(cells):
+            verts = macro.points[macro.cells[c]]
+            pts = pts0 if i == 0 else meshing.refine_triangles(verts, tris, r)[0]
+            g, ms = elements.barycentric_gradients(pts, mesh.cells)
+            if np.any(ms <= 0):
+                raise ValueError(f"macro {c}: inverted element in its ear refinement")
+            self.glam[i], self.meas[i] = g, ms
+            self.qpts[i] = np.einsum("qi,tij->tqj", prob.qbary, pts[mesh.cells])
+            mass1 = np.zeros(n)
+            if prob.degree == 1:
+                np.add.at(mass1, mesh.dofs.ravel(), np.repeat(ms / 3.0, 3))
+            else:
+                np.add.at(mass1, mesh.dofs[:, 3:].ravel(), np.repeat(ms / 3.0, 3))
+            self.constraints[i] = mass1 / mass1.sum()
+            lengths = np.linalg.norm(np.roll(verts, -1, axis=0) - verts, axis=1)
+            for s in range(k):
+                cols = slice(s * fs, (s + 1) * fs)
+                if macro.signs[c][s] > 0:
+                    self.colmap[i, cols] = s * fs + np.arange(fs)
+                    self.colscale[i, cols] = lengths[s]
+                else:  # second owner: the face's stored direction is reversed, sign -1
+                    self.colmap[i, cols] = s * fs + rev_map
+                    self.colscale[i, cols] = -rev_scale * lengths[s]
+                self.trace_dofs[i, cols] = macro.cell_faces[c][s] * fs + np.arange(fs)
+        indptr, indices = element_pattern(n, mesh.dofs)
+        tangent = verts0[1] - verts0[0]
+        dist = np.abs(meshing._cross(tangent, mesh.nodes - verts0[0]))
+        start = np.flatnonzero(dist <= 1e-12 * float(np.sqrt(tangent @ tangent)))
+        perm = choose_permutation(indptr, indices, start, mesh.nodes - verts0[0], tangent)
+        self.layout = LocalLayout(
+            n, perm, indptr, indices, m=m, k=1, coupling=template, constraint=self.constraints[0],
+            assembly=dict(dofs=mesh.dofs, dim=2, qweights=prob.qw, dtable=prob.der, vtable=prob.vals,
+                          grad_lambda=self.glam[0], measure=self.meas[0]),
+            device=device)
+
+    def batch(self, precision: str = "fp64", max_columns: int = 0):
+        b = LocalBatch(self.layout, len(self.cells), precision=precision, max_columns=max_columns)
+        b.set_coupling(colmap=self.colmap, colscale=self.colscale, constraints=self.constraints)
+        b.set_geometry(self.glam, self.meas)
+        return b

@@ -107,6 +107,17 @@ class LocalBatch:
                   _ptr(kn), _stream())
         torch.cuda.current_stream().synchronize()
 
+    def set_geometry(self, grad_lambda, measure):
+        """Per-macro element geometry ``(M, elements, dim+1, dim)`` barycentric
+        gradients and ``(M, elements)`` measures, for macros that share this
+        layout's topology but not its shape."""
+        g = self._dev(grad_lambda, torch.float64).contiguous()
+        ms = self._dev(measure, torch.float64).contiguous()
+        if g.shape[0] != self.macros or ms.shape != (self.macros, g.shape[1]):
+            raise ValueError("grad_lambda (M, elements, dim+1, dim) and measure (M, elements) required")
+        _lib.call("mhm_batch_set_geometry", self.handle, _ptr(g), _ptr(ms), _stream())
+        torch.cuda.current_stream().synchronize()
+
     # -------------------------------------------------------------- numerics
     def factor(self, check: bool = True):
         _lib.call("mhm_batch_factor", self.handle, _stream())

@@ -16,6 +16,11 @@
   nx = 16, r = 4 (n = 729, bandwidth 189, factored by the wide-band kernel
   and solved by the block solve); r = 2 (n = 125, bandwidth 65) is the small
   size that fits the LDS-window factor and the wavefront solve.
+* ``config4``: Voronoi macros of 2048 seeded points in the unit square (edges
+  under a twentieth of the seed spacing collapsed), each ear-split and refined
+  to about h = 1/1024 within 200-3000 P1 DOFs, discontinuous P1 multipliers per
+  macroface, log K a Gaussian field (correlation length 0.02, variance 2),
+  f = 1.  Macros of equal local topology form buckets (``PolygonProblem``).
 
 Fields are generated here (seeded) because the reference holds no random
 field generator for these cases.
@@ -28,7 +33,7 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import meshing
-from .problems import CubeGridProblem, SquareGridProblem
+from .problems import CubeGridProblem, PolygonProblem, SquareGridProblem
 
 
 @dataclass
@@ -98,6 +103,66 @@ def config3(nx: int = 16, refinement: int = 4, seed: int = 20261204, device: int
     prob = CubeGridProblem(nx, refinement, 2, split=2, order=4, device=device)
     kq = sphere_field(prob.quadrature_points(), 0.02, 0.2, seed, 1e-3, 1.0)
     return Case("config3", prob, kq, None, 1.0)
+
+
+def polygon_refinements(macro, h: float, dof_range=(200, 3000),
+                        levels=(4, 6, 8, 10, 12, 14, 16, 19, 22, 26, 30, 35, 40, 46)):
+    """Per-macro ear refinement for a target element size ``h``: the level
+    closest to ``sqrt(area) / h`` among those giving ``dof_range`` P1 DOFs for
+    the macro's vertex count (a fan of k - 2 ears refined r times has
+    (k-2)(r+1)(r+2)/2 - (k-3)(r+1) nodes); a few levels keep the buckets few."""
+    out = np.empty(len(macro.cells), dtype=np.int64)
+    for c, cell in enumerate(macro.cells):
+        xy = macro.points[cell]
+        k = len(cell)
+        area = 0.5 * float(np.sum(meshing._cross(xy - xy[0], np.roll(xy, -1, axis=0) - xy[0])))
+        nodes = {r: (k - 2) * (r + 1) * (r + 2) // 2 - (k - 3) * (r + 1) for r in levels}
+        ok = [r for r in levels if dof_range[0] <= nodes[r] <= dof_range[1]] or list(levels)
+        target = np.sqrt(area) / h
+        out[c] = min(ok, key=lambda r: abs(r - target))
+    return out
+
+
+def config4(npts: int = 2048, seed: int = 20261207, h: float = 1.0 / 1024, dof_range=(200, 3000),
+            device: int = 0) -> Case:
+    """Voronoi macros of ``npts`` seeded points in the unit square, each ear-split
+    and refined to about ``h`` within ``dof_range`` P1 DOFs, discontinuous P1
+    multipliers per macroface (m = 2 per side), log K a Gaussian field
+    (exponential covariance, correlation length 0.02, variance 2), f = 1."""
+    macro = meshing.voronoi_square(npts, seed)
+    prob = PolygonProblem(macro, polygon_refinements(macro, h, dof_range), 1, meshing.FaceSpace.uniform(1, 1),
+                          order=4, device=device)
+    q = prob.quadrature_points()
+    flat = np.concatenate([x.reshape(-1, 2) for x in q])
+    logk = gaussian_field(flat, 0.02, 2.0, seed + 1)
+    kq, off = [], 0
+    for x in q:
+        size = x.shape[0] * x.shape[1]
+        kq.append(np.exp(logk[off:off + size]).reshape(x.shape[:2]))
+        off += size
+    return Case("config4", prob, kq, None, 1.0)
+
+
+def run_polygon_stage(case: Case, precision: str = "fp64"):
+    """The stage over every bucket of a polygon case; per-macro host lists."""
+    prob = case.problem
+    out = {k: [None] * prob.M for k in ("blocks", "loads", "source", "lifts", "retained")}
+    iters = np.zeros(prob.M, dtype=np.int64)
+    for bk in prob.buckets:
+        with bk.batch(precision) as b:
+            kq = np.stack([case.kq[c] for c in bk.cells])
+            b.assemble(kq, None, case.f_const)
+            b.factor()
+            blocks, loads = b.condense()
+            src, lifts, ret = b.responses()
+            vals = dict(blocks=blocks.cpu().numpy(), loads=loads.cpu().numpy(), source=src.cpu().numpy(),
+                        lifts=lifts.cpu().numpy(), retained=ret.cpu().numpy())
+            for r, c in enumerate(bk.cells):
+                for key in out:
+                    out[key][c] = vals[key][r]
+            iters[bk.cells] = b.iterations()
+    out["iterations"] = iters
+    return out
 
 
 def run_stage(case: Case, precision: str = "fp64", macros: slice | None = None):
