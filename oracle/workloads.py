@@ -88,9 +88,13 @@ def build_polygon_problems(case, macros=None):
     (``:217-265``), ``scalar_operators`` and ``trace_coupling`` as the primal
     Darcy provider (``_legacy/models/darcy/primal.py:55-78``)."""
     prob = case.problem
-    pm = PolygonMesh(prob.macro.points, [list(cell) for cell in prob.macro.cells])
-    sp = prob.space
-    skel = SkeletonSpace(pm, [FaceSpace(sp.breaks, sp.degrees)] * len(pm.faces))
+    cached = getattr(prob, "_oracle_skeleton", None)
+    if cached is None:  # mesh-only, built once per problem
+        pm = PolygonMesh(prob.macro.points, [list(cell) for cell in prob.macro.cells])
+        sp = prob.space
+        cached = (pm, SkeletonSpace(pm, [FaceSpace(sp.breaks, sp.degrees)] * len(pm.faces)))
+        prob._oracle_skeleton = cached
+    pm, skel = cached
     out = []
     for c in (range(len(pm.cells)) if macros is None else macros):
         fine = pm.submesh(c, int(prob.refinements[c]))
@@ -156,6 +160,44 @@ def cube_spec(case, macros):
                 fq=None if case.fq is None else case.fq[macros], f_const=case.f_const)
 
 
+def polygon_spec(case, macros):
+    """Picklable description of selected macros of a polygon (uneven) case."""
+    prob = case.problem
+    macros = [int(c) for c in macros]
+    return dict(kind="polygon", points=prob.macro.points, cells=[list(map(int, c)) for c in prob.macro.cells],
+                refinements=prob.refinements, degree=prob.degree, order=prob.order,
+                breaks=tuple(prob.space.breaks), degrees=tuple(prob.space.degrees), macros=macros,
+                kq=[case.kq[c] for c in macros], fq=None if case.fq is None else [case.fq[c] for c in macros],
+                f_const=case.f_const)
+
+
+def _polygon_case(spec):
+    """A minimal case over the spec's macros (kq/fq indexed by macro id)."""
+    key = ("polygon", len(spec["cells"]), float(spec["points"].sum()))
+    if key not in _MACRO_CACHE:
+        class P:
+            pass
+        prob = P()
+
+        class Macro:
+            pass
+        prob.macro = Macro()
+        prob.macro.points, prob.macro.cells = spec["points"], spec["cells"]
+        prob.refinements, prob.degree, prob.order = spec["refinements"], spec["degree"], spec["order"]
+        prob.space = FaceSpace(spec["breaks"], spec["degrees"])
+        _MACRO_CACHE[key] = prob
+    prob = _MACRO_CACHE[key]
+
+    class C:
+        pass
+    case = C()
+    case.problem = prob
+    case.kq = {c: spec["kq"][r] for r, c in enumerate(spec["macros"])}
+    case.fq = None if spec["fq"] is None else {c: spec["fq"][r] for r, c in enumerate(spec["macros"])}
+    case.f_const = spec["f_const"]
+    return case
+
+
 class _SpecCase:
     """Minimal case object rebuilt from a spec inside a worker process."""
 
@@ -174,6 +216,14 @@ def condense_spec(spec):
     Same scope as the GPU stage: local mesh, operators, trace coupling,
     augmented SuperLU solve and condensed block/load per macro.
     """
+    if spec.get("kind") == "polygon":
+        problems, skel = build_polygon_problems(_polygon_case(spec), spec["macros"])
+        blocks, loads = [], []
+        for c, p, _ in problems:
+            _, blk, ld = p.condense().global_contribution([skel.size + c])
+            blocks.append(blk)
+            loads.append(ld)
+        return spec["macros"], blocks, loads
     if spec.get("kind") == "cube":
         # the macro list is re-indexed 0..len-1 against the spec's own kq/fq rows
         full_kq = {c: spec["kq"][r] for r, c in enumerate(spec["macros"])}

@@ -43,49 +43,72 @@ def _groups(keys):
 
 
 class HybridScatter:
-    """Fixed scatter plan of one macro mesh (indices ``(M, w)`` per cell in cell order)."""
+    """Fixed scatter plan of one macro mesh.
 
-    def __init__(self, indices, trace_size: int, kernel_offsets, device=0):
-        idx = np.asarray(indices, dtype=np.int64)
-        if idx.ndim != 2:
-            raise ValueError("indices must be (cells, width)")
+    ``indices`` is ``(M, w)`` per cell in cell order, or a list of per-cell
+    index arrays of different widths (uneven macros).  The values gathered are
+    a flat device buffer of the cells' row-major blocks and one of their
+    loads; ``positions`` / ``load_positions`` give where each cell's block and
+    load start in those buffers (default: cell order, back to back), so
+    blocks produced bucket by bucket need no reordering.
+    """
+
+    def __init__(self, indices, trace_size: int, kernel_offsets, device=0, positions=None, load_positions=None):
+        fixed = isinstance(indices, np.ndarray) and indices.ndim == 2
+        rows_of = [np.asarray(r, dtype=np.int64) for r in (indices if not fixed else list(indices))]
+        if not fixed and not all(r.ndim == 1 for r in rows_of):
+            raise ValueError("indices must be (cells, width) or a list of 1-D index arrays")
         offs = np.asarray(kernel_offsets, dtype=np.int64)
         self.size = int(offs[-1])
         self.trace_size = int(trace_size)
-        M, w = idx.shape
+        M = len(rows_of)
+        widths = np.array([len(r) for r in rows_of], dtype=np.int64)
         if len(offs) != M + 1 or offs[0] != trace_size or np.any(np.diff(offs) < 0):
             raise ValueError("kernel_offsets must be an ordered partition starting at trace_size")
-        if np.any(idx < 0) or np.any(idx >= self.size):
+        allidx = np.concatenate(rows_of) if M else np.zeros(0, dtype=np.int64)
+        if np.any(allidx < 0) or np.any(allidx >= self.size):
             raise ValueError("contribution indices must be valid global integers")
-        for c in range(M):
-            row = idx[c]
-            if len(np.unique(row)) != w:
+        for c, row in enumerate(rows_of):
+            if len(np.unique(row)) != len(row):
                 raise ValueError("contribution indices must be distinct")
             if not np.array_equal(np.sort(row[row >= trace_size]), np.arange(offs[c], offs[c + 1])):
                 raise ValueError("contribution coarse indices must match its ordered cell partition")
-        if not np.array_equal(np.unique(idx), np.arange(self.size)):
+        if not np.array_equal(np.unique(allidx), np.arange(self.size)):
             raise ValueError("contributions must cover the contiguous global numbering")
-        self.M, self.w = M, w
-        self.indices = idx
+        bpos = (np.concatenate(([0], np.cumsum(widths * widths)[:-1])) if positions is None
+                else np.asarray(positions, dtype=np.int64))
+        lpos = (np.concatenate(([0], np.cumsum(widths)[:-1])) if load_positions is None
+                else np.asarray(load_positions, dtype=np.int64))
+        self.M = M
+        self.w = int(widths[0]) if fixed else None
+        self.indices = np.stack(rows_of) if fixed else rows_of
         self.device = torch.device("cuda", device) if torch.cuda.is_available() else None
-        rows = np.repeat(idx, w, axis=1).ravel()
-        cols = np.tile(idx, (1, w)).ravel()
-        keys, vptr, vcode = _groups(cols * self.size + rows)
+        # the COO stream in the reference's order: cells in order, each block row-major
+        rows = np.concatenate([np.repeat(r, len(r)) for r in rows_of])
+        cols = np.concatenate([np.tile(r, len(r)) for r in rows_of])
+        codes = np.concatenate([bpos[c] + np.arange(widths[c] ** 2) for c in range(M)])
+        keys, vptr, order = _groups(cols * self.size + rows)
+        vcode = codes[order]
         self.csc_indices = (keys % self.size).astype(np.int32)
         colcount = np.bincount((keys // self.size).astype(np.int64), minlength=self.size)
         self.csc_indptr = np.zeros(self.size + 1, dtype=np.int32)
         self.csc_indptr[1:] = np.cumsum(colcount)
-        _, lptr, lcode = _groups(idx.ravel())
+        lcodes = np.concatenate([lpos[c] + np.arange(widths[c]) for c in range(M)])
+        _, lptr, lorder = _groups(allidx)
+        lcode = lcodes[lorder]
         self._host = dict(vptr=vptr, vcode=vcode, lptr=lptr, lcode=lcode)
         if self.device is not None:
             self.vptr = torch.as_tensor(vptr, device=self.device)
             self.vcode = torch.as_tensor(vcode, device=self.device)
             self.lptr = torch.as_tensor(lptr, device=self.device)
             self.lcode = torch.as_tensor(lcode, device=self.device)
-            self.d_indices = torch.as_tensor(idx, device=self.device)
+            if fixed:
+                self.d_indices = torch.as_tensor(self.indices, device=self.device)
 
     def coordinates(self):
-        """Row-major COO ``(rows, cols)`` of the whole stream on the GPU."""
+        """Row-major COO ``(rows, cols)`` of the whole stream on the GPU (fixed width)."""
+        if self.w is None:
+            raise ValueError("device coordinates need one contribution width")
         rows = torch.empty(self.M * self.w * self.w, dtype=torch.int64, device=self.device)
         cols = torch.empty_like(rows)
         _lib.call("mhm_scatter_coordinates", _ptr(self.d_indices), self.w, self.M, _ptr(rows), _ptr(cols),

@@ -314,3 +314,68 @@ class PolygonBucket:
         b.set_coupling(colmap=self.colmap, colscale=self.colscale, constraints=self.constraints)
         b.set_geometry(self.glam, self.meas)
         return b
+
+
+class BucketedStage:
+    """The local stage over every bucket of a ``PolygonProblem``.
+
+    Buckets are independent batches; a bucket of a few dozen macros leaves most
+    of the GPU idle while its band solves walk their rows, so the buckets are
+    issued largest first, round-robin over ``streams`` HIP streams (default
+    priority), and joined before the global gather.  ``plan`` (optional) is a
+    ``HybridScatter`` whose block/load positions follow the bucket order.
+    """
+
+    def __init__(self, prob: PolygonProblem, precision: str = "fp64", streams: int = 4, max_columns: int = 0):
+        import torch
+
+        self.prob = prob
+        work = [len(bk.cells) * bk.n * (bk.layout.bandwidth + 1) for bk in prob.buckets]
+        self.order = sorted(range(len(prob.buckets)), key=lambda i: -work[i])
+        self.batches = {i: prob.buckets[i].batch(precision, max_columns=max_columns) for i in self.order}
+        self.streams = [torch.cuda.Stream() for _ in range(max(1, int(streams)))]
+
+    def positions(self):
+        """Block and load start of every macro in the flat buffers of ``run``."""
+        prob = self.prob
+        bpos = np.zeros(prob.M, dtype=np.int64)
+        lpos = np.zeros(prob.M, dtype=np.int64)
+        ob = ol = 0
+        for i in self.order:
+            bk = prob.buckets[i]
+            w = bk.m + 1
+            bpos[bk.cells] = ob + np.arange(len(bk.cells)) * w * w
+            lpos[bk.cells] = ol + np.arange(len(bk.cells)) * w
+            ob += len(bk.cells) * w * w
+            ol += len(bk.cells) * w
+        return bpos, lpos
+
+    def run(self, kqs, fqs=None, f_const: float = 0.0, check: bool = True):
+        """Assemble, factor and condense every bucket (``kqs[i]``: the bucket's
+        ``(macros, elements, q)`` coefficients); returns the flat blocks and
+        loads in bucket order on the current stream."""
+        import torch
+
+        main = torch.cuda.current_stream()
+        for s in self.streams:
+            s.wait_stream(main)
+        bl, ld = [], []
+        for j, i in enumerate(self.order):
+            b = self.batches[i]
+            with torch.cuda.stream(self.streams[j % len(self.streams)]):
+                b.assemble(kqs[i], None if fqs is None else fqs[i], f_const)
+                b.factor(check=False)
+                x, y = b.condense(check=False)
+                bl.append(x.reshape(-1))
+                ld.append(y.reshape(-1))
+        for s in self.streams:
+            main.wait_stream(s)
+        if check:
+            for i in self.order:
+                self.batches[i].raise_on_failure(f"bucket {i}")
+        return torch.cat(bl), torch.cat(ld)
+
+    def close(self):
+        for b in self.batches.values():
+            b.close()
+        self.batches = {}

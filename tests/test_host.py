@@ -97,3 +97,38 @@ def test_level_order_handles_rcm_fallback():
     indices = np.array([0, 1, 0, 1, 2, 2])
     with pytest.raises(ValueError):
         layout.level_order(indptr, indices, [0])   # node 2 unreachable
+
+
+def test_ragged_scatter_plan_in_bucket_order_matches_reference():
+    """Uneven macros: per-cell widths differ and the blocks sit bucket by
+    bucket in the device buffer; the gathered values equal the reference's
+    assembly of the same contributions (core/contributions.py:77-195)."""
+    from oracle.local import assemble_hybrid_contributions
+
+    prob = pw.config4(npts=20, h=1 / 64, dof_range=(30, 900)).problem
+    idx = prob.global_indices()
+    rng = np.random.default_rng(4)
+    blocks = [rng.standard_normal((len(i), len(i))) for i in idx]
+    loads = [rng.standard_normal(len(i)) for i in idx]
+    # device buffers in bucket order
+    order = np.concatenate([bk.cells for bk in prob.buckets])
+    wid = np.array([len(i) for i in idx])
+    bpos = np.zeros(prob.M, dtype=np.int64)
+    lpos = np.zeros(prob.M, dtype=np.int64)
+    bpos[order] = np.concatenate(([0], np.cumsum(wid[order] ** 2)[:-1]))
+    lpos[order] = np.concatenate(([0], np.cumsum(wid[order])[:-1]))
+    flat_b = np.concatenate([blocks[c].ravel() for c in order])
+    flat_l = np.concatenate([loads[c] for c in order])
+    plan = hybrid.HybridScatter(idx, prob.trace_size, prob.kernel_offsets(), device=0, positions=bpos,
+                                load_positions=lpos)
+    host = plan._host
+    vals = np.array([np.sum(flat_b[host["vcode"][host["vptr"][g]:host["vptr"][g + 1]]])
+                     for g in range(len(plan.csc_indices))])
+    rhs = np.array([np.sum(flat_l[host["lcode"][host["lptr"][g]:host["lptr"][g + 1]]])
+                    for g in range(plan.size)])
+    mat, orhs, _ = assemble_hybrid_contributions([(idx[c], blocks[c], loads[c]) for c in range(prob.M)],
+                                                 prob.trace_size, prob.kernel_offsets())
+    from scipy import sparse
+    got = sparse.csc_matrix((vals, plan.csc_indices, plan.csc_indptr), shape=(plan.size, plan.size))
+    assert np.array_equal(got.toarray(), mat.toarray())
+    assert np.array_equal(rhs, orhs)
