@@ -1576,8 +1576,6 @@ __device__ __forceinline__ double rhs_at(const LayoutK &L, const BatchK &B, cons
 #define RES_ROWS 64
 
 struct ResShared {
-  double tot[4][64];  // per column: ||r||^2, ||b||^2, constraint row (hi, lo)
-  double bad[64];
   const double *colptr[64];
   int colstride[64];
   double colscale[64], mu[64];
@@ -1644,27 +1642,29 @@ __global__ void __launch_bounds__(256)
   const double cs = B.scal[(size_t)mac * NSCAL + S_CS];
   T *Y = y + (size_t)mac * n * B.ldy;
   const bool cols_here = c0 < ncols;  // uniform per wavefront
-  auto rhs_value = [&](int i, int t) -> double {
-    const int c = min(c0 + t, 63);
-    return sh.colptr[c][(size_t)i * sh.colstride[c]] * sh.colscale[c];
+  // the chunk's right-hand side and outputs go through an LDS stage of
+  // RES_ROWS rows (stride ldr): loaded and stored cooperatively, so the
+  // block's global accesses are row-contiguous instead of one row per lane
+  double *stg = xr + (size_t)ring * ldr;
+  const FastDiv divc(ncols);
+  auto load_chunk = [&](int i0) {
+    for (int e = tid; e < RES_ROWS * ncols; e += 256) {
+      const int q = divc(e), c = e - q * ncols;
+      const int i = i0 + q;
+      stg[q * ldr + c] = i < n ? sh.colptr[c][(size_t)i * sh.colstride[c]] * sh.colscale[c] : 0.0;
+    }
+  };
+  auto store_chunk = [&](int i0) {
+    for (int e = tid; e < RES_ROWS * ncols; e += 256) {
+      const int q = divc(e), c = e - q * ncols;
+      const int i = i0 + q;
+      if (i < n) Y[(size_t)i * B.ldy + c] = T(stg[q * ldr + c]);
+    }
   };
   double rr[G], chi[G], clo[G];
   bool finite = true;
 #pragma unroll
   for (int t = 0; t < G; ++t) rr[t] = chi[t] = clo[t] = 0.0;
-  if (it == 0) {
-    // x = 0: r = b; ||b||^2 is kept for the later passes of this solve
-    if (cols_here)
-      for (int i = lane; i < n; i += 64) {
-#pragma unroll
-        for (int t = 0; t < G; ++t) {
-          const double v = rhs_value(i, t);
-          finite &= isfinite(v);
-          rr[t] += v * v;
-          if (!check_only && c0 + t < ncols) Y[(size_t)i * B.ldy + c0 + t] = T(ds[i] * v);
-        }
-      }
-  } else {
   // ring rows entering before chunk i0: [i0 + b, i0 + RES_ROWS + b) (all of
   // [0, RES_ROWS + b) before the first); the next chunk's rows are loaded into
   // registers during the current chunk and committed after it
@@ -1674,7 +1674,7 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
     for (int v = 0; v < RPF; ++v) {
       const int e = tid + 256 * v;
-      const int q = e / ncols, c = e - q * ncols;
+      const int q = divc(e), c = e - q * ncols;
       xpf[v] = (lo + q < hi) ? x[(size_t)(lo + q) * ncols + c] : 0.0;
     }
   };
@@ -1682,67 +1682,80 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
     for (int v = 0; v < RPF; ++v) {
       const int e = tid + 256 * v;
-      const int q = e / ncols, c = e - q * ncols;
+      const int q = divc(e), c = e - q * ncols;
       if (lo + q < hi) xr[((lo + q) % ring) * ldr + c] = xpf[v];
     }
   };
   if (use_ring) {
     __syncthreads();  // zero fill done
     for (int e = tid; e < min(n, RES_ROWS + b) * ncols; e += 256) {
-      const int q = e / ncols, c = e - q * ncols;
+      const int q = divc(e), c = e - q * ncols;
       xr[q * ldr + c] = x[(size_t)q * ncols + c];  // rows < ring: slot = row
     }
     ring_issue(RES_ROWS + b, min(n, 2 * RES_ROWS + b));
-    __syncthreads();
   }
   for (int i0 = 0; i0 < n; i0 += RES_ROWS) {
+    __syncthreads();  // the previous chunk's stage is written out (and the ring committed)
+    load_chunk(i0);
+    __syncthreads();
     const int rb = use_ring ? (max(0, i0 - b) / ring) * ring : 0;  // ring slot of row j: j - rb (- ring)
     const int i = i0 + lane;
     if (cols_here && i < n) {
       double hi[G], lo[G];
 #pragma unroll
       for (int t = 0; t < G; ++t) {
-        hi[t] = rhs_value(i, t);
+        hi[t] = c0 + t < ncols ? stg[lane * ldr + c0 + t] : 0.0;
         lo[t] = 0.0;
       }
-      const int s1 = rowptr[i + 1];
-      for (int sb = rowptr[i]; sb < s1; sb += 4) {
-        // the chunk's CSR slots are loaded together, then applied in order
-        double av[4];
-        int jv[4];
+      if (it > 0) {
+        const int s1 = rowptr[i + 1];
+        for (int sb = rowptr[i]; sb < s1; sb += 4) {
+          // the chunk's CSR slots are loaded together, then applied in order
+          double av[4];
+          int jv[4];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          const bool v = sb + u < s1;
-          av[u] = v ? -a[sb + u] : 0.0;
-          jv[u] = v ? cols[sb + u] : i;
-        }
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          if (sb + u >= s1) break;
-          double xv[G];
-          if (use_ring) {
-            int slot = jv[u] - rb;
-            slot = slot >= ring ? slot - ring : slot;
-            const double *src = xr + slot * ldr + c0;
-#pragma unroll
-            for (int t = 0; t < G; ++t) xv[t] = src[t];
-          } else {
-            const double *src = x + (size_t)jv[u] * ncols;
-#pragma unroll
-            for (int t = 0; t < G; ++t) xv[t] = c0 + t < ncols ? src[c0 + t] : 0.0;
+          for (int u = 0; u < 4; ++u) {
+            const bool v = sb + u < s1;
+            av[u] = v ? -a[sb + u] : 0.0;
+            jv[u] = v ? cols[sb + u] : i;
           }
 #pragma unroll
-          for (int t = 0; t < G; ++t) dd_fma(hi[t], lo[t], av[u], xv[t]);
-        }
-      }
-      if (L.k) {
-        const double ci = cs * constraint_at(L, B, mac, i);
-        const double *xi = x + (size_t)i * ncols;
+          for (int u = 0; u < 4; ++u) {
+            if (sb + u >= s1) break;
+            double xv[G];
+            if (use_ring) {
+              int slot = jv[u] - rb;
+              slot = slot >= ring ? slot - ring : slot;
+              const double *src = xr + slot * ldr + c0;
 #pragma unroll
-        for (int t = 0; t < G; ++t) {
-          const int c = min(c0 + t, 63);
-          dd_fma(hi[t], lo[t], -ci, sh.mu[c]);
-          dd_fma(chi[t], clo[t], ci, c0 + t < ncols ? xi[c0 + t] : 0.0);
+              for (int t = 0; t < G; ++t) xv[t] = src[t];
+            } else {
+              const double *src = x + (size_t)jv[u] * ncols;
+#pragma unroll
+              for (int t = 0; t < G; ++t) xv[t] = c0 + t < ncols ? src[c0 + t] : 0.0;
+            }
+#pragma unroll
+            for (int t = 0; t < G; ++t) dd_fma(hi[t], lo[t], av[u], xv[t]);
+          }
+        }
+        if (L.k) {
+          const double ci = cs * constraint_at(L, B, mac, i);
+          double xi[G];
+          if (use_ring) {
+            int slot = i - rb;
+            slot = slot >= ring ? slot - ring : slot;
+#pragma unroll
+            for (int t = 0; t < G; ++t) xi[t] = xr[slot * ldr + c0 + t];
+          } else {
+#pragma unroll
+            for (int t = 0; t < G; ++t) xi[t] = c0 + t < ncols ? x[(size_t)i * ncols + c0 + t] : 0.0;
+          }
+#pragma unroll
+          for (int t = 0; t < G; ++t) {
+            const int c = min(c0 + t, 63);
+            dd_fma(hi[t], lo[t], -ci, sh.mu[c]);
+            dd_fma(chi[t], clo[t], ci, xi[t]);
+          }
         }
       }
 #pragma unroll
@@ -1750,8 +1763,12 @@ __global__ void __launch_bounds__(256)
         const double r = hi[t] + lo[t];
         finite &= isfinite(r);
         rr[t] += r * r;
-        if (!check_only && c0 + t < ncols) Y[(size_t)i * B.ldy + c0 + t] = T(ds[i] * r);
+        if (!check_only && c0 + t < ncols) stg[lane * ldr + c0 + t] = ds[i] * r;
       }
+    }
+    if (!check_only) {
+      __syncthreads();
+      store_chunk(i0);
     }
     if (use_ring && i0 + RES_ROWS < n) {
       // rows of the next chunk into the slots of rows [i0 - b, i0 - b + RES_ROWS)
@@ -1759,11 +1776,12 @@ __global__ void __launch_bounds__(256)
       __syncthreads();  // the chunk's ring reads are done
       ring_commit(lo, hi);
       ring_issue(hi, min(n, i0 + 3 * RES_ROWS + b));
-      __syncthreads();
     }
   }
-  }
-  // per-column sums over the lanes (each wavefront owns its columns)
+  // per-column sums over the lanes (each wavefront owns its columns), left in
+  // the stage: tot[k*64 + c] = ||r||^2, ||b||^2, constraint row (hi, lo), bad
+  double *tot = stg;
+  __syncthreads();  // the last chunk's stage is written out
   const double badw = wave_sum(finite ? 0.0 : 1.0);
 #pragma unroll
   for (int t = 0; t < G; ++t) {
@@ -1779,23 +1797,23 @@ __global__ void __launch_bounds__(256)
       l = l + ol + ee;
     }
     if (lane == 0 && cols_here && c0 + t < ncols && t < cq) {
-      sh.tot[0][c0 + t] = r2;
-      sh.tot[1][c0 + t] = b2;
+      tot[0 * 64 + c0 + t] = r2;
+      tot[1 * 64 + c0 + t] = b2;
       if (it == 0 && !check_only) bnorm[(size_t)mac * ncols + c0 + t] = r2;
-      sh.tot[2][c0 + t] = h;
-      sh.tot[3][c0 + t] = l;
-      sh.bad[c0 + t] = badw;
+      tot[2 * 64 + c0 + t] = h;
+      tot[3 * 64 + c0 + t] = l;
+      tot[4 * 64 + c0 + t] = badw;
     }
   }
   __syncthreads();
   if (!mine) return;
   const int c = tid;
-  double R2 = sh.tot[0][c], B2 = sh.tot[1][c];
-  const double BAD = sh.bad[c];
+  double R2 = tot[0 * 64 + c], B2 = tot[1 * 64 + c];
+  const double BAD = tot[4 * 64 + c];
   double rn = 0.0;
   if (L.k) {
     const double bn = rhs ? rhs[((size_t)mac * N + n) * ncols + c] : 0.0;
-    rn = bn - (sh.tot[2][c] + sh.tot[3][c]);
+    rn = bn - (tot[2 * 64 + c] + tot[3 * 64 + c]);
     R2 += rn * rn;
     B2 += bn * bn;
   }
@@ -1816,6 +1834,76 @@ __global__ void __launch_bounds__(256)
   }
 }
 
+// Per-column sums over the rows of the refinement kernels go in fixed chunks
+// of SADDLE_ROWS rows.
+#define SADDLE_ROWS 32
+
+// First pass of a refined solve (x = 0, so r = b): y = D b, ||b||^2 per column
+// (kept for the later passes), the constraint entry of b (the bordered
+// correction's right-hand side) and the column state, without the residual's
+// CSR walk and x ring.  Block per macro, thread (row group, column) with
+// row-contiguous accesses; the per-column sums go in fixed chunks of
+// SADDLE_ROWS rows in row order, then in chunk order, so a column's bits do
+// not depend on the other columns of the call.
+template <typename T>
+__global__ void __launch_bounds__(256)
+    k_rhs_init(LayoutK L, BatchK B, int ncols, const double *__restrict__ rhs, T *__restrict__ y,
+               double *__restrict__ beta, int *__restrict__ colstate, const double *__restrict__ coupling_rm,
+               double *__restrict__ bnorm) {
+  extern __shared__ double part[];  // nchunk x ncols
+  __shared__ int bad;
+  const int mac = blockIdx.x, tid = threadIdx.x;
+  const int n = L.n, m = L.m, N = B.N;
+  const int lanes = 256 / ncols;
+  const int g = tid / ncols, c = tid - g * ncols;
+  const int nchunk = (n + SADDLE_ROWS - 1) / SADDLE_ROWS;
+  if (tid == 0) bad = 0;
+  __syncthreads();
+  const double *ds = B.dscale + (size_t)mac * n;
+  T *Y = y + (size_t)mac * n * B.ldy;
+  const double *src;
+  int stride = 1;
+  double scale = 1.0;
+  if (rhs) {
+    src = rhs + (size_t)mac * N * ncols + c;
+    stride = ncols;
+  } else if (c == 0) {
+    src = B.load + (size_t)mac * n;
+  } else if (c > m) {
+    src = B.az + (size_t)mac * n;
+  } else if (B.bcoup) {
+    src = B.bcoup + ((size_t)mac * m + (c - 1)) * n;
+  } else {
+    src = coupling_rm + (B.colmap ? B.colmap[(size_t)mac * m + c - 1] : c - 1);
+    stride = m;
+    if (B.colscale) scale = B.colscale[(size_t)mac * m + c - 1];
+  }
+  bool finite = true;
+  if (g < lanes)
+    for (int k = g; k < nchunk; k += lanes) {
+      double s2 = 0.0;
+      const int i1 = min(n, (k + 1) * SADDLE_ROWS);
+#pragma unroll 4
+      for (int i = k * SADDLE_ROWS; i < i1; ++i) {
+        const double v = src[(size_t)i * stride] * scale;
+        finite &= isfinite(v);
+        s2 += v * v;
+        Y[(size_t)i * B.ldy + c] = T(ds[i] * v);
+      }
+      part[k * ncols + c] = s2;
+    }
+  if (!finite) bad = 1;
+  __syncthreads();
+  if (tid < ncols) {
+    double s2 = 0.0;
+    for (int k = 0; k < nchunk; ++k) s2 += part[k * ncols + tid];
+    bnorm[(size_t)mac * ncols + tid] = s2;
+    colstate[(size_t)mac * ncols + tid] = 0;
+    beta[(size_t)mac * ncols + tid] = rhs ? rhs[((size_t)mac * N + n) * ncols + tid] : 0.0;
+  }
+  if (tid == 0 && bad) atomicOr(&B.status[mac], MHM_STATUS_NONFINITE);
+}
+
 // Column width per wavefront of the residual (compile-time G >= ceil(ncols/4)).
 static const int kResWidths[] = {2, 4, 6, 9, 12, 16};
 
@@ -1833,7 +1921,6 @@ static int res_width(int ncols) {
 // summed in row order, then the chunk sums in chunk order: the result does not
 // depend on how many columns share the call (a column's bits are the same
 // alone and in a batch).
-#define SADDLE_ROWS 32
 
 template <typename T>
 __global__ void __launch_bounds__(256)
@@ -3065,10 +3152,13 @@ static int launch_residual(mhm_batch *b, int ncols, const double *rhs, const dou
   const int G = res_width(ncols);
   const int ldr = 4 * G + 1;  // odd row stride (in doubles) spreads the ring rows over the LDS banks
   int ring = RES_ROWS + 2 * L.b;
-  size_t shm = (size_t)ring * ldr * sizeof(double);
-  if (shm + sizeof(ResShared) > (size_t)lds_limit(b->layout->device)) {
+  const size_t stage = (size_t)RES_ROWS * ldr * sizeof(double);  // the chunk's rhs / output stage
+  size_t shm = (size_t)ring * ldr * sizeof(double) + stage;
+  // the first pass (x = 0, r = b) reads no x: no ring, so the launch is not
+  // held to the ring's occupancy
+  if (it == 0 || shm + sizeof(ResShared) > (size_t)lds_limit(b->layout->device)) {
     ring = 0;
-    shm = 0;
+    shm = stage;
   }
 #define MHM_RES(g)                                                                                          \
   case g:                                                                                                   \
@@ -3114,7 +3204,17 @@ static int run_solve(mhm_batch *b, const double *rhs, double *X, int ncols, int 
   for (int it = 0; it <= max_refine; ++it) {
     {
       TimerSpan span(MHM_TIMER_RESIDUAL, st);
-      if (int rc = launch_residual<T>(b, ncols, rhs, X, rtol, fwd, it, 0, st)) return rc;
+      if (it == 0) {
+        const size_t shm = saddle_lds(L.n, ncols) / 2;
+        if (shm > 65536)
+          HIP_TRY(hipFuncSetAttribute((const void *)k_rhs_init<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      (int)shm));
+        hipLaunchKernelGGL((k_rhs_init<T>), dim3(b->M), dim3(256), shm, st, L, b->bk, ncols, rhs, (T *)b->y,
+                           b->beta, b->colstate, b->layout->coupling_rm, b->bnorm);
+        LAUNCH_CHECK("k_rhs_init");
+      } else if (int rc = launch_residual<T>(b, ncols, rhs, X, rtol, fwd, it, 0, st)) {
+        return rc;
+      }
     }
     if (it == 0 && uw_inline) {
       hipLaunchKernelGGL((k_load_uw<T>), grid2(L.n, 256, b->M), dim3(256), 0, st, L, b->bk, (T *)b->y, ncols);
