@@ -252,6 +252,7 @@ struct LayoutK {
   const int *slot_ptr, *slot_code, *load_ptr, *load_code;
   const double *glam, *measure, *qw, *dtab, *vtab;
   const int *bslot;  // bslot[j*W + r] = CSR slot of A(j + r, j), or -1
+  const int *srow;   // row of every CSR slot
 };
 
 // Per-macro scalars kept in scal[mac * NSCAL + ...].
@@ -550,7 +551,7 @@ __global__ void __launch_bounds__(128)
 // Block per macro: max|A| and max|C| (constraint scaling of
 // core/condensation.py:160-166), Jacobi scaling of Ahat, band fill.
 template <typename T>
-__global__ void __launch_bounds__(256) k_prepare(LayoutK L, BatchK B, T *__restrict__ band) {
+__global__ void __launch_bounds__(256) k_prepare(LayoutK L, BatchK B, T *__restrict__ band, T *__restrict__ sval) {
   __shared__ double red[4];
   const int mac = blockIdx.x;
   const double *a = B.aval + (size_t)mac * L.nnz;
@@ -578,10 +579,21 @@ __global__ void __launch_bounds__(256) k_prepare(LayoutK L, BatchK B, T *__restr
     B.dscale[(size_t)mac * L.n + i] = 1.0 / sqrt(d);
   }
   __syncthreads();  // dscale of every row written
+  const double *ds = B.dscale + (size_t)mac * L.n;
+  if (sval) {
+    // the scaled, pinned operator in CSR slot order, ds[i] (A(i, j) + sigma
+    // at the pin) ds[j]: the LDS-window factor gathers its band from it
+    T *sv = sval + (size_t)mac * L.nnz;
+    const int pslot = L.pin >= 0 ? L.diagslot[L.pin] : -1;
+    for (int e = threadIdx.x; e < L.nnz; e += 256) {
+      double v = a[e];
+      if (e == pslot) v += sigma;
+      sv[e] = T(ds[L.srow[e]] * v * ds[L.cols[e]]);
+    }
+  } else {
   // band[j*W + r] = ds[j+r] A(j+r, j) ds[j] through the band slot map, one
   // coalesced pass (positions outside the pattern or past row n - 1 are 0)
   T *bd = band + (size_t)mac * L.n * L.W;
-  const double *ds = B.dscale + (size_t)mac * L.n;
   const int W = L.W, dj = 256 / W, dr = 256 - dj * W;
   int j = threadIdx.x / W, r = threadIdx.x - j * W;
   for (int e = threadIdx.x; e < L.n * W; e += 256) {
@@ -599,6 +611,7 @@ __global__ void __launch_bounds__(256) k_prepare(LayoutK L, BatchK B, T *__restr
       r -= W;
       ++j;
     }
+  }
   }
   if (bad) atomicOr(&B.status[mac], bad);
   if (threadIdx.x == 0) {
@@ -803,12 +816,18 @@ static size_t factor_panel_lds(int b, size_t tsize) {
   return (S * panel_ldw(b, tsize) + (size_t)FPANEL * panel_ldq(b)) * tsize;
 }
 
+static int lds_limit(int device);
+static bool factor_panel_ok(int b, size_t tsize, int device) {
+  return FPANEL + b <= 127 && factor_panel_lds(b, tsize) <= (size_t)lds_limit(device);
+}
+
 template <typename T>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
     k_factor_panel(int n, int b, int ldw, int ldq, T *__restrict__ band, T *__restrict__ rband,
-                   int *__restrict__ status, double tiny) {
+                   int *__restrict__ status, double tiny, const T *__restrict__ sval, int nnz,
+                   const int *__restrict__ bslot) {
   constexpr int NB = FPANEL;
-  constexpr int NPF = 8;  // NB * W <= 256 * NPF, so W <= 128
+  constexpr int NPF = 10;  // NB * W <= 192 * NPF, so W <= 120
   typedef typename Mfma16<T>::v4 V4;
   extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
   const int W = b + 1;
@@ -820,11 +839,17 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
   const int mac = blockIdx.x, tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   T *Lg = band + (size_t)mac * n * W;
   T *Rg = rband ? rband + (size_t)mac * n * W : nullptr;
+  // the scaled, pinned operator (CSR slot order, written by k_prepare) at band
+  // position (j, r) is sv[bslot[j*W + r]]; slot -1 (outside the pattern or
+  // past row n - 1) reads as 0
+  const T *sv = sval + (size_t)mac * nnz;
+  auto slot_of = [&](int j, int r) -> int { return (j < n && j + r < n) ? bslot[(size_t)j * W + r] : -1; };
   const FastDiv divW(W);
   if (tid == 0) bad = 0;
   for (int e = tid; e < S * W; e += 256) {  // columns 0 .. S-1 in slots 0 .. S-1
     const int c = divW(e), r = e - c * W;
-    win[c * ldw + r] = (c + r < n) ? Lg[e] : T(0);
+    const int q = slot_of(c, r);
+    win[c * ldw + r] = q >= 0 ? sv[q] : T(0);
   }
   for (int e = tid; e < NB * ldq; e += 256) LpT[e] = T(0);
   if (Rg)  // entries L(j, j-r) with j - r < 0
@@ -840,14 +865,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
   for (int j0 = 0; j0 < n; j0 += NB) {
     const int s0 = j0 % S;
     const int np = min(NB, n - j0);
-    // columns j0 + S .. j0 + S + NB - 1 enter after this panel, into its slots
-#pragma unroll
-    for (int v = 0; v < NPF; ++v) {
-      const int e = tid + 256 * v;
-      const int t = divW(e);
-      pf[v] = (e < NB * W && j0 + S + t + (e - t * W) < n) ? Lg[(size_t)(j0 + S) * W + e] : T(0);
-    }
-    // (a) panel factorisation, wavefront 0
+    // (a) panel factorisation, wavefront 0; meanwhile wavefronts 1-3 gather
+    // the columns j0 + S .. j0 + S + NB - 1, which enter after this panel
     if (wave == 0) {
       const int q0 = lane, q1 = lane + 64;
       T a0[NB], a1[NB];
@@ -883,6 +902,14 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
         }
       }
       if (failed && lane == 0) bad = MHM_STATUS_PIVOT;
+    } else {
+#pragma unroll
+      for (int v = 0; v < NPF; ++v) {
+        const int e = tid - 64 + 192 * v;
+        const int t = divW(e);
+        const int q = e < NB * W ? slot_of(j0 + S + t, e - t * W) : -1;
+        pf[v] = q >= 0 ? sv[q] : T(0);
+      }
     }
     __syncthreads();
     // (b) the panel to HBM: column band (contiguous) and row band
@@ -933,8 +960,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
     // (d) entering columns take the panel's slots
 #pragma unroll
     for (int v = 0; v < NPF; ++v) {
-      const int e = tid + 256 * v;
-      if (e < NB * W) {
+      const int e = tid - 64 + 192 * v;
+      if (wave > 0 && e < NB * W) {
         const int t = divW(e), r = e - t * W;
         int slot = s0 + t;
         slot = slot >= S ? slot - S : slot;
@@ -2268,6 +2295,7 @@ struct mhm_batch {
   size_t tsize = 8;
   BatchK bk{};
   void *band = nullptr, *rband = nullptr, *y = nullptr;  // rband: row-major band (wavefront solve)
+  void *sval = nullptr;  // scaled operator in CSR slot order (LDS-window factor only)
   double *xstd = nullptr, *beta = nullptr, *colnorm = nullptr, *bnorm = nullptr;
   double *rhsu = nullptr, *xu = nullptr, *srcscale = nullptr;
   double *glam = nullptr, *meas = nullptr;  // per-macro element geometry (uneven macros)
@@ -2792,6 +2820,10 @@ int mhm_layout_create(int32_t device, const mhm_layout_desc *d, mhm_layout **out
         if (j <= i) bslot[(size_t)j * k.W + (i - j)] = s;
       }
     if ((rc = upload(L->owned, bslot.data(), bslot.size(), &k.bslot))) return cleanup(rc);
+    std::vector<int> srow((size_t)d->nnz);
+    for (int i = 0; i < d->n; ++i)
+      for (int s = d->rowptr[i]; s < d->rowptr[i + 1]; ++s) srow[s] = i;
+    if ((rc = upload(L->owned, srow.data(), srow.size(), &k.srow))) return cleanup(rc);
     for (int i0 = 0; i0 < d->n; i0 += GATHER_ROWS)
       L->gather_slots = std::max(L->gather_slots, d->rowptr[std::min(d->n, i0 + GATHER_ROWS)] - d->rowptr[i0]);
   }
@@ -2901,6 +2933,10 @@ int mhm_batch_create(const mhm_layout *L, int32_t macros, int32_t precision, int
     }
     if ((rc = dev_alloc(b->owned, &p, M * n * b->ldy * tsize))) return cleanup(rc);
     b->y = p;
+    if (factor_panel_ok(lk.b, tsize, dev)) {
+      if ((rc = dev_alloc(b->owned, &p, M * lk.nnz * tsize))) return cleanup(rc);
+      b->sval = p;
+    }
   }
   HIP_TRY(hipMemset(k.status, 0, M * sizeof(int)));
   HIP_TRY(hipMemset(k.iters, 0, M * sizeof(int)));
@@ -3092,19 +3128,22 @@ static int run_factor(mhm_batch *b, hipStream_t st) {
   const mhm_layout *lay = b->layout;
   const LayoutK &L = lay->k;
   hipLaunchKernelGGL(k_fill_int, dim3((b->M + 255) / 256), dim3(256), 0, st, b->bk.status, 0, (size_t)b->M);
-  hipLaunchKernelGGL((k_prepare<T>), dim3(b->M), dim3(256), 0, st, L, b->bk, (T *)b->band);
+  const size_t pshm = factor_panel_lds(L.b, sizeof(T));
+  const bool panel = factor_panel_ok(L.b, sizeof(T), lay->device);
+  hipLaunchKernelGGL((k_prepare<T>), dim3(b->M), dim3(256), 0, st, L, b->bk, (T *)b->band,
+                     panel ? (T *)b->sval : (T *)nullptr);
   LAUNCH_CHECK("k_prepare");
   const size_t shm = factor_lds(L.b, sizeof(T));
   const double tiny = (double)L.n * (sizeof(T) == 8 ? 2.220446049250313e-16 : 1.1920929e-07);
   {
   TimerSpan span(MHM_TIMER_FACTOR, st);
-  const size_t pshm = factor_panel_lds(L.b, sizeof(T));
-  if (FPANEL + L.b <= 127 && pshm <= (size_t)lds_limit(lay->device)) {
+  if (panel) {
     if (pshm > 65536)
       HIP_TRY(hipFuncSetAttribute((const void *)k_factor_panel<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                   (int)pshm));
     hipLaunchKernelGGL((k_factor_panel<T>), dim3(b->M), dim3(256), pshm, st, L.n, L.b, panel_ldw(L.b, sizeof(T)),
-                       panel_ldq(L.b), (T *)b->band, (T *)b->rband, b->bk.status, tiny);
+                       panel_ldq(L.b), (T *)b->band, (T *)b->rband, b->bk.status, tiny,
+                       (const T *)b->sval, L.nnz, L.bslot);
   } else if (factor_wide_ok(L.b, sizeof(T), lds_limit(lay->device))) {
     const size_t wshm = factor_wide_lds(L.b, sizeof(T));
     if (wshm > 65536)
