@@ -83,6 +83,9 @@ __device__ __forceinline__ void dd_add(double &hi, double &lo, double v) {
   lo += e;
 }
 
+// Compiler-only barrier: memory accesses are not moved across it.
+#define MEM_FENCE() asm volatile("" ::: "memory")
+
 // Cross-lane hand-off through LDS inside one wavefront: the wavefront-scope
 // release/acquire fences keep the compiler from reusing or reordering LDS
 // values across it (lanes read what other lanes wrote), the wave barrier keeps
@@ -767,8 +770,8 @@ __global__ void __launch_bounds__(NT)
 //      C = C - L_I L_J^T in four 16x16x4 MFMA steps with the C fragment read
 //      from and written back to the window (ldw makes those accesses
 //      conflict-free in binary64, ldq the operand reads);
-//  (d) the NB columns that enter next, prefetched into registers at the start
-//      of the panel, take the panel's slots.
+//  (d) the NB columns that enter next, gathered by wavefronts 1-3 during (a),
+//      take the panel's slots.
 // Three block barriers per panel instead of two per column.
 #define FPANEL 16
 
@@ -1360,7 +1363,6 @@ constexpr int wave_chunk() {
   return sizeof(T) * MAXW > 256 ? 4 : 8;
 }
 
-#define MEM_FENCE() asm volatile("" ::: "memory")
 
 template <typename T, int N>
 __device__ __forceinline__ void pin(T (&a)[N]) {
@@ -1515,6 +1517,193 @@ __global__ void __launch_bounds__(64 * SOLVE_WAVES) __attribute__((amdgpu_waves_
   band_sweep<T, MAXW, 1>(n, W, ldy, on, cl, lane, band + (size_t)mac * n * W, Y, stage);
   // a lane reads back only the rows it wrote itself
   band_sweep<T, MAXW, -1>(n, W, ldy, on, cl, lane, rband + (size_t)mac * n * W, Y, stage);
+}
+
+// ----------------------------------------------------------------- blocked solve on the matrix cores
+// L z = y, then L^T x = z in binary64, one wavefront per macro, in blocks of 16
+// rows; dinv holds the inverse of every 16 x 16 diagonal block of L
+// (k_diag_inverse, after the factor).  With the column band
+// (band[j*W + r] = L(j+r, j)) for the forward and the row band
+// (rband[i*W + r] = L(i, i-r)) for the backward sweep, a block I is
+//   forward:  z_I = Dinv_I (y_I - L(I, J) z_J),        J = the 4 nk rows above I
+//   backward: x_I = Dinv_I^T (z_I - L(K, I)^T x_K),   K = the 4 nk rows below I
+// with nk = ceil(b / 4).  The coupling product is nk v_mfma_f64_16x16x4f64
+// steps per 16-column tile (A = L, 16 rows x 4 window rows; B = 4 window rows
+// x 16 columns), the diagonal block four more (B = y_I - product).  In binary64 the MFMA result
+// layout (row 4r + lane/16 of the tile in register r) is the B operand layout
+// of the k-step of rows 4r .. 4r+3, so a solved block enters the register
+// window of the next block as it is: window slot t holds one k-step for every
+// column tile, the newest four slots the block just solved.  Every lane loads
+// distinct band values (16 consecutive entries per 16-lane group); the band
+// operands of the next block are loaded during this block's products, the
+// right-hand side and the diagonal inverse at the start of the block and used
+// after them.  Each element of y is read and written by the same lane in both
+// sweeps.
+#define SOLVE_MFMA_WAVES 4
+
+// Loads outside the band or the matrix read this zero instead of branching, so
+// every load of the sweep is issued and the memory counter waits stay exact.
+// (A global-space variable, never written: a constant-space one would turn the
+// selected pointer into a flat address.)
+__device__ double kZeroF64[1];
+
+// Inverses of the 16 x 16 diagonal blocks of L (rows and columns 16 q ..
+// 16 q + 15, identity past row n - 1), row-major at dinv[(mac * nblk + q) * 256]:
+// 16 lanes per block, column sigma of the inverse in lane sigma by
+// column-oriented forward substitution from the column band.
+__global__ void __launch_bounds__(256) k_diag_inverse(int n, int b, int macros, const double *__restrict__ band,
+                                                     double *__restrict__ dinv) {
+  const int nblk = (n + 15) / 16;
+  const long gb = (long)blockIdx.x * 16 + (threadIdx.x >> 4);
+  const int sigma = threadIdx.x & 15;
+  if (gb >= (long)macros * nblk) return;
+  const int mac = (int)(gb / nblk), q = (int)(gb - (long)mac * nblk);
+  const int W = b + 1, i0 = 16 * q, np = min(16, n - i0);
+  const double *col = band + ((size_t)mac * n + i0) * W;  // column i0 of L
+  double *dst = dinv + (size_t)gb * 256;
+  double acc[16];
+#pragma unroll
+  for (int rho = 0; rho < 16; ++rho) acc[rho] = rho == sigma ? 1.0 : 0.0;
+#pragma unroll
+  for (int tau = 0; tau < 16; ++tau) {
+    MEM_FENCE();  // one column of the block in registers at a time
+    const double *c = col + (size_t)tau * W;  // L(i0 + tau + r, i0 + tau) = c[r]
+    const double z = tau < np ? acc[tau] / c[0] : acc[tau];
+    dst[tau * 16 + sigma] = z;
+#pragma unroll
+    for (int rho = tau + 1; rho < 16; ++rho)
+      if (rho < np && rho - tau <= b) acc[rho] -= c[rho - tau] * z;
+  }
+}
+
+template <int MAXK, int NCT, int DIR>
+__device__ __forceinline__ void mfma_sweep(int n, int b, int ldy, int ncols, const double *__restrict__ Lb,
+                                           const double *__restrict__ Di, double *__restrict__ Y, int lane) {
+  typedef Mfma16<double>::v4 V4;
+  const int W = b + 1, nblk = (n + 15) / 16;
+  const int a16 = lane & 15, g = lane >> 4;
+  // operand A of window slot t for block q: row a16 of the block against
+  // window row g of slot t, zero outside the band and the matrix (loaded
+  // values are not touched before the product, so the loads stay in flight)
+  auto loadA = [&](int q, int t) -> double {
+    const int i0 = 16 * q;
+    if (DIR > 0) {
+      const int j = i0 - 4 * (MAXK - t) + g, rr = a16 + 4 * (MAXK - t) - g;
+      return *((j >= 0 && rr <= b && i0 + a16 < n) ? Lb + (size_t)j * W + rr : kZeroF64);
+    }
+    const int i = i0 + 16 + 4 * (MAXK - 1 - t) + g, rr = 16 + 4 * (MAXK - 1 - t) + g - a16;
+    return *((i < n && rr <= b) ? Lb + (size_t)i * W + rr : kZeroF64);
+  };
+  double A[MAXK];
+  double win[MAXK][NCT];
+#pragma unroll
+  for (int t = 0; t < MAXK; ++t) {
+#pragma unroll
+    for (int ct = 0; ct < NCT; ++ct) win[t][ct] = 0.0;
+  }
+  // (slots below MAXK - ceil(b / 4) lie outside the band and read zeros)
+  {
+    const int q = DIR > 0 ? 0 : nblk - 1;
+#pragma unroll
+    for (int t = 0; t < MAXK; ++t) A[t] = loadA(q, t);
+  }
+  for (int s = 0; s < nblk; ++s) {
+    const int q = DIR > 0 ? s : nblk - 1 - s;
+    const bool more = s + 1 < nblk;
+    const int qn = more ? (DIR > 0 ? q + 1 : q - 1) : q;  // the last block reloads its own operands
+    const int i0 = 16 * q;
+    // right-hand side rows and the diagonal inverse of this block
+    double rhs[NCT][4], dv[4];
+#pragma unroll
+    for (int ct = 0; ct < NCT; ++ct) {
+      const int col = 16 * ct + a16;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = i0 + 4 * r + g;
+        rhs[ct][r] = *((row < n && col < ncols) ? Y + (size_t)row * ldy + col : kZeroF64);
+      }
+    }
+    const double *dq = Di + (size_t)q * 256;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) dv[k] = DIR > 0 ? dq[a16 * 16 + 4 * k + g] : dq[(4 * k + g) * 16 + a16];
+    // these loads issue before the next block's operands (the memory counter
+    // drains in order), so waiting for them leaves those in flight
+    MEM_FENCE();
+    // coupling product; each slot's operand is reloaded for the next block
+    // once its products are issued
+    V4 acc[NCT];
+#pragma unroll
+    for (int ct = 0; ct < NCT; ++ct) acc[ct] = V4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int t = 0; t < MAXK; ++t) {
+#pragma unroll
+      for (int ct = 0; ct < NCT; ++ct) acc[ct] = Mfma16<double>::step(A[t], win[t][ct], acc[ct]);
+      A[t] = loadA(qn, t);
+    }
+    // diagonal block
+    V4 x[NCT];
+#pragma unroll
+    for (int ct = 0; ct < NCT; ++ct) {
+      x[ct] = V4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int k = 0; k < 4; ++k) x[ct] = Mfma16<double>::step(dv[k], rhs[ct][k] - acc[ct][k], x[ct]);
+    }
+#pragma unroll
+    for (int ct = 0; ct < NCT; ++ct) {
+      const int col = 16 * ct + a16;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = i0 + 4 * r + g;
+        if (row < n && col < ncols) Y[(size_t)row * ldy + col] = x[ct][r];
+      }
+    }
+    // window: drop the four oldest slots, the block enters as the newest
+#pragma unroll
+    for (int t = 0; t + 4 < MAXK; ++t) {
+#pragma unroll
+      for (int ct = 0; ct < NCT; ++ct) win[t][ct] = win[t + 4][ct];
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+#pragma unroll
+      for (int ct = 0; ct < NCT; ++ct) win[DIR > 0 ? MAXK - 4 + r : MAXK - 1 - r][ct] = x[ct][r];
+    }
+  }
+}
+
+template <int MAXK, int NCT>
+__global__ void __launch_bounds__(64 * SOLVE_MFMA_WAVES) __attribute__((amdgpu_waves_per_eu(2)))
+    k_solve_mfma(int n, int b, int ldy, int ncols, int macros, const double *__restrict__ band,
+                 const double *__restrict__ rband, const double *__restrict__ dinv, double *__restrict__ y,
+                 const int *__restrict__ colstate, int *__restrict__ iters) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int mac = __builtin_amdgcn_readfirstlane(blockIdx.x * SOLVE_MFMA_WAVES + wave);
+  if (mac >= macros) return;
+  if (colstate) {
+    const bool pending = lane < ncols && colstate[(size_t)mac * ncols + lane] == 0;
+    if (!__any(pending)) return;
+  }
+  if (iters && lane == 0) iters[mac] += 1;
+  const size_t nW = (size_t)n * (b + 1);
+  const double *Di = dinv + (size_t)mac * ((n + 15) / 16) * 256;
+  double *Y = y + (size_t)mac * n * ldy;
+  mfma_sweep<MAXK, NCT, 1>(n, b, ldy, ncols, band + mac * nW, Di, Y, lane);
+  mfma_sweep<MAXK, NCT, -1>(n, b, ldy, ncols, rband + mac * nW, Di, Y, lane);
+}
+
+// Register window of the matrix-core solve: MAXK k-steps x NCT column tiles
+// (at most 52 doubles per lane).
+static void solve_mfma_shape(int b, int ncols, int *maxk, int *nct) {
+  static const int kMaxK[] = {8, 12, 17, 24, 28};
+  *maxk = *nct = 0;
+  if (ncols < 1 || ncols > 64 || b < 1) return;
+  const int t = (ncols + 15) / 16, nk = (b + 3) / 4;
+  for (int m : kMaxK)
+    if (nk <= m && m * t <= 52) {
+      *maxk = m;
+      *nct = t;
+      return;
+    }
 }
 
 // D e_p and D c into columns c0, c0+1 of the workspace.
@@ -2296,6 +2485,7 @@ struct mhm_batch {
   BatchK bk{};
   void *band = nullptr, *rband = nullptr, *y = nullptr;  // rband: row-major band (wavefront solve)
   void *sval = nullptr;  // scaled operator in CSR slot order (LDS-window factor only)
+  double *dinv = nullptr;  // inverses of the 16 x 16 diagonal blocks (binary64 matrix-core solve)
   double *xstd = nullptr, *beta = nullptr, *colnorm = nullptr, *bnorm = nullptr;
   double *rhsu = nullptr, *xu = nullptr, *srcscale = nullptr;
   double *glam = nullptr, *meas = nullptr;  // per-macro element geometry (uneven macros)
@@ -2647,6 +2837,26 @@ static void launch_wave(const mhm_batch *b, int ncols, const int *colstate, int 
 template <typename T>
 static int launch_solve(const mhm_batch *b, int ncols, const int *colstate, int *iters, hipStream_t st) {
   const LayoutK &L = b->layout->k;
+  if constexpr (sizeof(T) == 8) {
+    int maxk, nct;
+    solve_mfma_shape(L.b, ncols, &maxk, &nct);
+    if (b->dinv && b->rband && maxk) {
+      const dim3 grid((unsigned)((b->M + SOLVE_MFMA_WAVES - 1) / SOLVE_MFMA_WAVES));
+#define MHM_MFMA(K, C)                                                                                      \
+  if (maxk == K && nct == C) {                                                                              \
+    hipLaunchKernelGGL((k_solve_mfma<K, C>), grid, dim3(64 * SOLVE_MFMA_WAVES), 0, st, L.n, L.b, b->ldy,     \
+                       ncols, b->M, (const double *)b->band, (const double *)b->rband,                     \
+                       (const double *)b->dinv, (double *)b->y, colstate, iters);                          \
+    LAUNCH_CHECK("k_solve_mfma");                                                                           \
+    return MHM_OK;                                                                                          \
+  }
+      MHM_MFMA(8, 1) MHM_MFMA(8, 2) MHM_MFMA(8, 3) MHM_MFMA(8, 4)
+      MHM_MFMA(12, 1) MHM_MFMA(12, 2) MHM_MFMA(12, 3) MHM_MFMA(12, 4)
+      MHM_MFMA(17, 1) MHM_MFMA(17, 2) MHM_MFMA(17, 3)
+      MHM_MFMA(24, 1) MHM_MFMA(24, 2) MHM_MFMA(28, 1)
+#undef MHM_MFMA
+    }
+  }
   switch (b->rband ? wave_width(L.b, ncols) : 0) {
 #define MHM_WAVE(w)                                \
   case w:                                          \
@@ -2936,6 +3146,9 @@ int mhm_batch_create(const mhm_layout *L, int32_t macros, int32_t precision, int
     if (factor_panel_ok(lk.b, tsize, dev)) {
       if ((rc = dev_alloc(b->owned, &p, M * lk.nnz * tsize))) return cleanup(rc);
       b->sval = p;
+      if (precision == MHM_FP64 && b->rband &&
+          (rc = dev_alloc(b->owned, &b->dinv, M * (size_t)((n + 15) / 16) * 256)))
+        return cleanup(rc);
     }
   }
   HIP_TRY(hipMemset(k.status, 0, M * sizeof(int)));
@@ -3162,6 +3375,12 @@ static int run_factor(mhm_batch *b, hipStream_t st) {
                        (T *)b->rband, b->bk.status, tiny);
   }
   LAUNCH_CHECK("k_factor");
+  if (sizeof(T) == 8 && b->dinv) {
+    const size_t blocks = (size_t)b->M * ((L.n + 15) / 16);
+    hipLaunchKernelGGL(k_diag_inverse, dim3((unsigned)((blocks + 15) / 16)), dim3(256), 0, st, L.n, L.b, b->M,
+                       (const double *)b->band, b->dinv);
+    LAUNCH_CHECK("k_diag_inverse");
+  }
   }
   // u = Ahat^-1 e_p and w = Ahat^-1 c ride along with the first solve
   b->uw_pending = L.k > 0;
