@@ -1548,31 +1548,45 @@ __global__ void __launch_bounds__(64 * SOLVE_WAVES) __attribute__((amdgpu_waves_
 __device__ double kZeroF64[1];
 
 // Inverses of the 16 x 16 diagonal blocks of L (rows and columns 16 q ..
-// 16 q + 15, identity past row n - 1), row-major at dinv[(mac * nblk + q) * 256]:
-// 16 lanes per block, column sigma of the inverse in lane sigma by
-// column-oriented forward substitution from the column band.
+// 16 q + 15, identity past row n - 1), row-major at dinv[(mac * nblk + q) * 256].
+// A workgroup takes 16 blocks: thread (block, tau) stages column tau of its
+// block (16 contiguous band entries) and 1 / L(tau, tau) in LDS, then lane
+// sigma of the block's 16 forms column sigma of the inverse by column-oriented
+// forward substitution from LDS.
 __global__ void __launch_bounds__(256) k_diag_inverse(int n, int b, int macros, const double *__restrict__ band,
                                                      double *__restrict__ dinv) {
+  __shared__ double sL[16][16][17];  // [block][column tau][L(tau + r, tau), r < 16; 1 / L(tau, tau)]
   const int nblk = (n + 15) / 16;
-  const long gb = (long)blockIdx.x * 16 + (threadIdx.x >> 4);
-  const int sigma = threadIdx.x & 15;
-  if (gb >= (long)macros * nblk) return;
-  const int mac = (int)(gb / nblk), q = (int)(gb - (long)mac * nblk);
-  const int W = b + 1, i0 = 16 * q, np = min(16, n - i0);
-  const double *col = band + ((size_t)mac * n + i0) * W;  // column i0 of L
+  const int lb = threadIdx.x >> 4, sigma = threadIdx.x & 15;
+  const long gb = (long)blockIdx.x * 16 + lb;
+  const bool on = gb < (long)macros * nblk;
+  const int mac = on ? (int)(gb / nblk) : 0, q = on ? (int)(gb - (long)mac * nblk) : 0;
+  const int W = b + 1, i0 = 16 * q, np = on ? min(16, n - i0) : 0;
+  {
+    const int tau = sigma;  // this thread stages column tau
+    const double *c = band + ((size_t)mac * n + i0 + min(tau, max(np - 1, 0))) * W;
+    double *dstc = sL[lb][tau];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const bool in = tau < np && tau + r < np && r <= b;
+      dstc[r] = in ? c[r] : (r == 0 ? 1.0 : 0.0);  // identity past the matrix
+    }
+    dstc[16] = tau < np ? 1.0 / c[0] : 1.0;
+  }
+  __syncthreads();
+  if (!on) return;
   double *dst = dinv + (size_t)gb * 256;
   double acc[16];
 #pragma unroll
   for (int rho = 0; rho < 16; ++rho) acc[rho] = rho == sigma ? 1.0 : 0.0;
 #pragma unroll
   for (int tau = 0; tau < 16; ++tau) {
-    MEM_FENCE();  // one column of the block in registers at a time
-    const double *c = col + (size_t)tau * W;  // L(i0 + tau + r, i0 + tau) = c[r]
-    const double z = tau < np ? acc[tau] / c[0] : acc[tau];
+    if ((tau & 3) == 0) MEM_FENCE();  // four columns of the block in registers at a time
+    const double *c = sL[lb][tau];
+    const double z = acc[tau] * c[16];
     dst[tau * 16 + sigma] = z;
 #pragma unroll
-    for (int rho = tau + 1; rho < 16; ++rho)
-      if (rho < np && rho - tau <= b) acc[rho] -= c[rho - tau] * z;
+    for (int rho = tau + 1; rho < 16; ++rho) acc[rho] -= c[rho - tau] * z;
   }
 }
 
