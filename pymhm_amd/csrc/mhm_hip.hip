@@ -764,7 +764,9 @@ __global__ void __launch_bounds__(NT)
 //      unchanged).  The panel goes to LDS transposed, LpT[t * ldq + q] =
 //      L(j0 + q, j0 + t), rows PR .. ldq - 1 zero;
 //  (b) the panel's columns go to HBM as the column band (contiguous) and the
-//      row band (rband[R*W + r] = L(R, R-r), runs of NB per row);
+//      row band (rband[R*W + r] = L(R, R-r), runs of NB per row), and with
+//      dblk set its 16 x 16 diagonal block, row-major, to
+//      dblk[(mac * nblocks + j0 / 16) * 256];
 //  (c) the rank-NB update of the b trailing columns on the matrix cores:
 //      16 x 16 tiles of the lower triangle, round-robin over the wavefronts,
 //      C = C - L_I L_J^T in four 16x16x4 MFMA steps with the C fragment read
@@ -828,7 +830,7 @@ template <typename T>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
     k_factor_panel(int n, int b, int ldw, int ldq, T *__restrict__ band, T *__restrict__ rband,
                    int *__restrict__ status, double tiny, const T *__restrict__ sval, int nnz,
-                   const int *__restrict__ bslot) {
+                   const int *__restrict__ bslot, T *__restrict__ dblk) {
   constexpr int NB = FPANEL;
   constexpr int NPF = 10;  // NB * W <= 192 * NPF, so W <= 120
   typedef typename Mfma16<T>::v4 V4;
@@ -926,6 +928,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
         const int R = j0 + q, r = q - t;
         if (t < np && r >= 0 && r <= b && R < n) Rg[(size_t)R * W + r] = LpT[t * ldq + q];
       }
+    if (dblk)  // the diagonal block, row-major (identity-padded past row n - 1)
+      dblk[((size_t)mac * ((n + NB - 1) / NB) + j0 / NB) * (NB * NB) + tid] = LpT[(tid & 15) * ldq + (tid >> 4)];
     // (c) trailing update on the matrix cores: tile k = (I, J), J <= I < nt,
     // columns qc0 = NB + 16 J, rows qr0 = NB + 16 I, for wavefront k mod 4
     for (int k = wave; k < ntile; k += 4) {
@@ -1521,8 +1525,8 @@ __global__ void __launch_bounds__(64 * SOLVE_WAVES) __attribute__((amdgpu_waves_
 
 // ----------------------------------------------------------------- blocked solve on the matrix cores
 // L z = y, then L^T x = z in binary64, one wavefront per macro, in blocks of 16
-// rows; dinv holds the inverse of every 16 x 16 diagonal block of L
-// (k_diag_inverse, after the factor).  With the column band
+// rows; dinv holds the inverse of every 16 x 16 diagonal block of L (the
+// factor writes the blocks, k_diag_inverse inverts them in place).  With the column band
 // (band[j*W + r] = L(j+r, j)) for the forward and the row band
 // (rband[i*W + r] = L(i, i-r)) for the backward sweep, a block I is
 //   forward:  z_I = Dinv_I (y_I - L(I, J) z_J),        J = the 4 nk rows above I
@@ -1547,35 +1551,30 @@ __global__ void __launch_bounds__(64 * SOLVE_WAVES) __attribute__((amdgpu_waves_
 // selected pointer into a flat address.)
 __device__ double kZeroF64[1];
 
-// Inverses of the 16 x 16 diagonal blocks of L (rows and columns 16 q ..
-// 16 q + 15, identity past row n - 1), row-major at dinv[(mac * nblk + q) * 256].
-// A workgroup takes 16 blocks: thread (block, tau) stages column tau of its
-// block (16 contiguous band entries) and 1 / L(tau, tau) in LDS, then lane
-// sigma of the block's 16 forms column sigma of the inverse by column-oriented
-// forward substitution from LDS.
-__global__ void __launch_bounds__(256) k_diag_inverse(int n, int b, int macros, const double *__restrict__ band,
-                                                     double *__restrict__ dinv) {
+// Inverses of the 16 x 16 diagonal blocks of L in place: dinv[(mac * nblk +
+// q) * 256] holds the block (rows and columns 16 q .. 16 q + 15, row-major,
+// identity past row n - 1, written by the LDS-window factor) and receives its
+// inverse.  A workgroup takes 16 blocks: thread (block, rho) stages row rho
+// of its block and 1 / L(rho, rho) in LDS, then lane sigma of the block's 16
+// forms column sigma of the inverse by column-oriented forward substitution.
+__global__ void __launch_bounds__(256) k_diag_inverse(long nblocks, double *__restrict__ dinv) {
   __shared__ double sL[16][16][17];  // [block][column tau][L(tau + r, tau), r < 16; 1 / L(tau, tau)]
-  const int nblk = (n + 15) / 16;
   const int lb = threadIdx.x >> 4, sigma = threadIdx.x & 15;
   const long gb = (long)blockIdx.x * 16 + lb;
-  const bool on = gb < (long)macros * nblk;
-  const int mac = on ? (int)(gb / nblk) : 0, q = on ? (int)(gb - (long)mac * nblk) : 0;
-  const int W = b + 1, i0 = 16 * q, np = on ? min(16, n - i0) : 0;
+  const bool on = gb < nblocks;
+  double *blk = dinv + (size_t)(on ? gb : 0) * 256;
   {
-    const int tau = sigma;  // this thread stages column tau
-    const double *c = band + ((size_t)mac * n + i0 + min(tau, max(np - 1, 0))) * W;
-    double *dstc = sL[lb][tau];
+    const int rho = sigma;  // this thread stages row rho
+    double row[16];
 #pragma unroll
-    for (int r = 0; r < 16; ++r) {
-      const bool in = tau < np && tau + r < np && r <= b;
-      dstc[r] = in ? c[r] : (r == 0 ? 1.0 : 0.0);  // identity past the matrix
-    }
-    dstc[16] = tau < np ? 1.0 / c[0] : 1.0;
+    for (int t = 0; t < 16; ++t) row[t] = on ? blk[rho * 16 + t] : (t == rho ? 1.0 : 0.0);
+#pragma unroll
+    for (int t = 0; t < 16; ++t)
+      if (t <= rho) sL[lb][t][rho - t] = row[t];
+    sL[lb][rho][16] = 1.0 / row[rho];
   }
   __syncthreads();
   if (!on) return;
-  double *dst = dinv + (size_t)gb * 256;
   double acc[16];
 #pragma unroll
   for (int rho = 0; rho < 16; ++rho) acc[rho] = rho == sigma ? 1.0 : 0.0;
@@ -1584,7 +1583,7 @@ __global__ void __launch_bounds__(256) k_diag_inverse(int n, int b, int macros, 
     if ((tau & 3) == 0) MEM_FENCE();  // four columns of the block in registers at a time
     const double *c = sL[lb][tau];
     const double z = acc[tau] * c[16];
-    dst[tau * 16 + sigma] = z;
+    blk[tau * 16 + sigma] = z;
 #pragma unroll
     for (int rho = tau + 1; rho < 16; ++rho) acc[rho] -= c[rho - tau] * z;
   }
@@ -3370,7 +3369,7 @@ static int run_factor(mhm_batch *b, hipStream_t st) {
                                   (int)pshm));
     hipLaunchKernelGGL((k_factor_panel<T>), dim3(b->M), dim3(256), pshm, st, L.n, L.b, panel_ldw(L.b, sizeof(T)),
                        panel_ldq(L.b), (T *)b->band, (T *)b->rband, b->bk.status, tiny,
-                       (const T *)b->sval, L.nnz, L.bslot);
+                       (const T *)b->sval, L.nnz, L.bslot, sizeof(T) == 8 ? (T *)b->dinv : (T *)nullptr);
   } else if (factor_wide_ok(L.b, sizeof(T), lds_limit(lay->device))) {
     const size_t wshm = factor_wide_lds(L.b, sizeof(T));
     if (wshm > 65536)
@@ -3391,8 +3390,8 @@ static int run_factor(mhm_batch *b, hipStream_t st) {
   LAUNCH_CHECK("k_factor");
   if (sizeof(T) == 8 && b->dinv) {
     const size_t blocks = (size_t)b->M * ((L.n + 15) / 16);
-    hipLaunchKernelGGL(k_diag_inverse, dim3((unsigned)((blocks + 15) / 16)), dim3(256), 0, st, L.n, L.b, b->M,
-                       (const double *)b->band, b->dinv);
+    hipLaunchKernelGGL(k_diag_inverse, dim3((unsigned)((blocks + 15) / 16)), dim3(256), 0, st, (long)blocks,
+                       b->dinv);
     LAUNCH_CHECK("k_diag_inverse");
   }
   }
