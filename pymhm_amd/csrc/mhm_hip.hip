@@ -60,7 +60,11 @@ static int fail(int code, const std::string &msg) {
   } while (0)
 
 // ----------------------------------------------------------------- device helpers
+// The error-free transformations below are exact only with every operation
+// rounded on its own: contraction into fused multiply-adds is off in them (the
+// one fma of dd_fma is explicit).
 __device__ __forceinline__ void two_sum(double a, double b, double &s, double &e) {
+#pragma clang fp contract(off)
   s = a + b;
   double bb = s - a;
   e = (a - (s - bb)) + (b - bb);
@@ -68,6 +72,7 @@ __device__ __forceinline__ void two_sum(double a, double b, double &s, double &e
 
 // Accumulate the exact product a*b into the double-double (hi, lo).
 __device__ __forceinline__ void dd_fma(double &hi, double &lo, double a, double b) {
+#pragma clang fp contract(off)
   double p = a * b;
   double pe = fma(a, b, -p);
   double s, e;
@@ -77,10 +82,24 @@ __device__ __forceinline__ void dd_fma(double &hi, double &lo, double a, double 
 }
 
 __device__ __forceinline__ void dd_add(double &hi, double &lo, double v) {
+#pragma clang fp contract(off)
   double s, e;
   two_sum(hi, v, s, e);
   hi = s;
   lo += e;
+}
+
+// A zero in global memory, never written: loads that select between a real
+// address and this one stay global loads (a zero in LDS or constant memory
+// would turn the selected pointer into a flat address), and loads outside a
+// band or a matrix can read it instead of branching.
+__device__ double kZeroF64[1];
+
+// A pointer known to address global memory (one read back from LDS would
+// otherwise be accessed as a flat address).
+template <typename P>
+__device__ __forceinline__ const __attribute__((address_space(1))) P *global_ptr(const P *p) {
+  return (const __attribute__((address_space(1))) P *)p;
 }
 
 // Compiler-only barrier: memory accesses are not moved across it.
@@ -1545,11 +1564,6 @@ __global__ void __launch_bounds__(64 * SOLVE_WAVES) __attribute__((amdgpu_waves_
 // sweeps.
 #define SOLVE_MFMA_WAVES 4
 
-// Loads outside the band or the matrix read this zero instead of branching, so
-// every load of the sweep is issued and the memory counter waits stay exact.
-// (A global-space variable, never written: a constant-space one would turn the
-// selected pointer into a flat address.)
-__device__ double kZeroF64[1];
 
 // Inverses of the 16 x 16 diagonal blocks of L in place: dinv[(mac * nblk +
 // q) * 256] holds the block (rows and columns 16 q .. 16 q + 15, row-major,
@@ -1788,21 +1802,28 @@ __device__ __forceinline__ double rhs_at(const LayoutK &L, const BatchK &B, cons
 // r = b - S x in double-double, per-column acceptance ||r|| <= rtol ||b||
 // (linalg/linear.py:132-152; the max|b| column scaling of the reference
 // cancels from both sides), scaled correction rhs D r into y, constraint-row
-// residual into beta.  it == 0: x = 0 (initial solve).  fwd_tol > 0 also
-// requires forward convergence of the refinement.  check_only: set
-// MHM_STATUS_RESIDUAL for columns that miss the criterion.
-// Block per macro.  Rows go in chunks of RES_ROWS; wavefront q takes columns
-// [q*cq, q*cq + cq) (cq = ceil(ncols / 4), at most G; columns past ncols are
-// zero padding) and lane r row i0 + r of the chunk.  x rows stream from HBM
-// once into an LDS ring of RES_ROWS + 2b rows (a chunk's CSR columns lie in
-// rows [i0 - b, i0 + RES_ROWS + b)); a thread walks its row's CSR slots
-// once, each slot's value and column index applied to its G double-double
-// accumulators.  Column c's right-hand side is colptr[c][i * colstride[c]] *
+// residual into beta, for x from a refinement pass (it >= 1; the first pass
+// is k_rhs_init).  fwd_tol > 0 also requires forward convergence of the
+// refinement.  check_only: set MHM_STATUS_RESIDUAL for columns that miss the
+// criterion.  ||b||^2 comes from bnorm (kept by k_rhs_init).
+// Block per macro, RES_WAVES wavefronts, lane c = column c (ncols <= 64).
+// Rows go in chunks of RES_ROWS; wavefront w takes rows i0 + w, i0 + w +
+// RES_WAVES, ... of the chunk, one at a time.  The chunk's CSR slots (values,
+// column indices, row offsets) are staged in LDS with the right-hand side; a
+// row's slots are the same for the whole wavefront, so their values and
+// column indices are broadcast LDS reads (four slots at a time) and the x row
+// of a slot is one contiguous LDS read; each (row, column) entry is
+// accumulated in slot order.
+// x rows stream from HBM once into an LDS ring of RES_ROWS + 2b rows (a
+// chunk's CSR columns lie in rows [i0 - b, i0 + RES_ROWS + b)); the chunk's
+// right-hand side and outputs go through an LDS stage with row-contiguous
+// global access.  Column c's right-hand side is colptr[c][i * colstride[c]] *
 // colscale[c] (explicit rhs, f, A Z, or the coupling through the per-macro
-// column map and scale).  Each thread keeps its rows' norm partials; they are
-// summed over the lanes once at the end in a fixed order.  Without a ring (no
-// LDS room) x is read from HBM directly.
-#define RES_ROWS 64
+// column map and scale).  The per-column sums of every wavefront are combined
+// in wavefront order at the end.  Without a ring (no LDS room) x is read from
+// HBM directly.
+#define RES_ROWS 32
+#define RES_WAVES 8
 
 struct ResShared {
   const double *colptr[64];
@@ -1812,21 +1833,31 @@ struct ResShared {
   double zero;
 };
 
-template <typename T, int G>
-__global__ void __launch_bounds__(256)
+// LDS of the residual past the x ring: the chunk's right-hand side rows, its
+// CSR values, column indices and row offsets (at most slots per chunk).
+static size_t res_stage(int ncols, int slots) {
+  return ((size_t)RES_ROWS * ncols + slots) * sizeof(double) + ((size_t)slots + RES_ROWS + 1) * sizeof(int);
+}
+
+template <typename T, bool RING>
+__global__ void __launch_bounds__(64 * RES_WAVES)
     k_residual(LayoutK L, BatchK B, int ncols, const double *__restrict__ rhs, const double *__restrict__ X,
                double rtol, double fwd_tol, int it, T *__restrict__ y, double *__restrict__ beta,
                int *__restrict__ colstate, const double *__restrict__ colnorm, int check_only,
                const int *__restrict__ rowptr, const int *__restrict__ cols, const double *__restrict__ aval,
-               const double *__restrict__ dscale, const double *__restrict__ coupling_rm, int ring, int ldr,
-               double *__restrict__ bnorm) {
+               const double *__restrict__ dscale, const double *__restrict__ coupling_rm, int ring,
+               int slots, double *__restrict__ bnorm) {
+  constexpr int NT = 64 * RES_WAVES;
   extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
   __shared__ ResShared sh;
+  const int ldr = ncols;
   double *xr = reinterpret_cast<double *>(smem_raw);  // ring x ldr
   const int mac = blockIdx.x;
-  const int tid = threadIdx.x;
-  const int wave = tid >> 6, lane = tid & 63;
-  const int n = L.n, N = B.N, m = L.m, b = L.b;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int n = L.n, N = B.N, b = L.b;
+  const bool on = lane < ncols;
+  const int cl = on ? lane : 0;  // lanes past ncols read column 0 and store nothing
   int *state = colstate + (size_t)mac * ncols;
   // converged columns were accepted by the residual of their current X
   const bool mine = tid < ncols && state[tid] == 0;
@@ -1834,7 +1865,7 @@ __global__ void __launch_bounds__(256)
   const double *x = X + (size_t)mac * N * ncols;
   if (tid < 64) {
     const int c = tid;
-    const double *ptr = &sh.zero;
+    const double *ptr = kZeroF64;
     int stride = 0;
     double scale = 1.0;
     if (c < ncols) {
@@ -1844,65 +1875,56 @@ __global__ void __launch_bounds__(256)
         stride = ncols;
       } else if (c == 0) {
         ptr = B.load + (size_t)mac * n;
-      } else if (c > m) {
+      } else if (c > L.m) {
         ptr = B.az + (size_t)mac * n;
       } else if (B.bcoup) {
-        ptr = B.bcoup + ((size_t)mac * m + (c - 1)) * n;
+        ptr = B.bcoup + ((size_t)mac * L.m + (c - 1)) * n;
       } else {
-        ptr = coupling_rm + (B.colmap ? B.colmap[(size_t)mac * m + c - 1] : c - 1);
-        stride = m;
-        if (B.colscale) scale = B.colscale[(size_t)mac * m + c - 1];
+        ptr = coupling_rm + (B.colmap ? B.colmap[(size_t)mac * L.m + c - 1] : c - 1);
+        stride = L.m;
+        if (B.colscale) scale = B.colscale[(size_t)mac * L.m + c - 1];
       }
     }
     sh.colptr[c] = ptr;
     sh.colstride[c] = stride;
     sh.colscale[c] = scale;
-    sh.mu[c] = (c < ncols && L.k && it > 0) ? x[(size_t)n * ncols + c] : 0.0;
+    sh.mu[c] = (c < ncols && L.k) ? x[(size_t)n * ncols + c] : 0.0;
     if (c == 0) sh.zero = 0.0;
   }
   if (!__syncthreads_or(mine)) return;
-  const int cq = (ncols + 3) / 4;
-  const int c0 = wave * cq;  // this wavefront's first column
-  const bool use_ring = ring > 0 && it > 0;
-  if (use_ring)
-    for (int e = tid; e < ring * ldr; e += 256) xr[e] = 0.0;  // padding columns stay zero
+  constexpr bool use_ring = RING;  // an instance per case: LDS and global reads stay apart
   const double *a = aval + (size_t)mac * L.nnz;
   const double *ds = dscale + (size_t)mac * n;
   const double cs = B.scal[(size_t)mac * NSCAL + S_CS];
   T *Y = y + (size_t)mac * n * B.ldy;
-  const bool cols_here = c0 < ncols;  // uniform per wavefront
-  // the chunk's right-hand side and outputs go through an LDS stage of
-  // RES_ROWS rows (stride ldr): loaded and stored cooperatively, so the
-  // block's global accesses are row-contiguous instead of one row per lane
   double *stg = xr + (size_t)ring * ldr;
+  double *ca = stg + (size_t)RES_ROWS * ldr;      // the chunk's CSR values (negated)
+  int *cj = reinterpret_cast<int *>(ca + slots);  // ... byte offsets of their x rows (ring or X)
+  int *crp = cj + slots;                          // ... row offsets from the chunk's first slot
   const FastDiv divc(ncols);
   auto load_chunk = [&](int i0) {
-    for (int e = tid; e < RES_ROWS * ncols; e += 256) {
+    for (int e = tid; e < RES_ROWS * ncols; e += NT) {
       const int q = divc(e), c = e - q * ncols;
       const int i = i0 + q;
-      stg[q * ldr + c] = i < n ? sh.colptr[c][(size_t)i * sh.colstride[c]] * sh.colscale[c] : 0.0;
+      stg[q * ldr + c] = i < n ? global_ptr(sh.colptr[c])[(size_t)i * sh.colstride[c]] * sh.colscale[c] : 0.0;
     }
   };
   auto store_chunk = [&](int i0) {
-    for (int e = tid; e < RES_ROWS * ncols; e += 256) {
+    for (int e = tid; e < RES_ROWS * ncols; e += NT) {
       const int q = divc(e), c = e - q * ncols;
       const int i = i0 + q;
       if (i < n) Y[(size_t)i * B.ldy + c] = T(stg[q * ldr + c]);
     }
   };
-  double rr[G], chi[G], clo[G];
-  bool finite = true;
-#pragma unroll
-  for (int t = 0; t < G; ++t) rr[t] = chi[t] = clo[t] = 0.0;
   // ring rows entering before chunk i0: [i0 + b, i0 + RES_ROWS + b) (all of
   // [0, RES_ROWS + b) before the first); the next chunk's rows are loaded into
   // registers during the current chunk and committed after it
-  constexpr int RPF = G;  // RES_ROWS * ncols <= RES_ROWS * 4G = 256 * G
+  constexpr int RPF = RES_ROWS * 64 / NT;
   double xpf[RPF];
   auto ring_issue = [&](int lo, int hi) {
 #pragma unroll
     for (int v = 0; v < RPF; ++v) {
-      const int e = tid + 256 * v;
+      const int e = tid + NT * v;
       const int q = divc(e), c = e - q * ncols;
       xpf[v] = (lo + q < hi) ? x[(size_t)(lo + q) * ncols + c] : 0.0;
     }
@@ -1910,89 +1932,76 @@ __global__ void __launch_bounds__(256)
   auto ring_commit = [&](int lo, int hi) {
 #pragma unroll
     for (int v = 0; v < RPF; ++v) {
-      const int e = tid + 256 * v;
+      const int e = tid + NT * v;
       const int q = divc(e), c = e - q * ncols;
       if (lo + q < hi) xr[((lo + q) % ring) * ldr + c] = xpf[v];
     }
   };
   if (use_ring) {
-    __syncthreads();  // zero fill done
-    for (int e = tid; e < min(n, RES_ROWS + b) * ncols; e += 256) {
+    for (int e = tid; e < min(n, RES_ROWS + b) * ncols; e += NT) {
       const int q = divc(e), c = e - q * ncols;
       xr[q * ldr + c] = x[(size_t)q * ncols + c];  // rows < ring: slot = row
     }
     ring_issue(RES_ROWS + b, min(n, 2 * RES_ROWS + b));
   }
+  double rr = 0.0, chi = 0.0, clo = 0.0;
+  bool finite = true;
   for (int i0 = 0; i0 < n; i0 += RES_ROWS) {
     __syncthreads();  // the previous chunk's stage is written out (and the ring committed)
     load_chunk(i0);
-    __syncthreads();
     const int rb = use_ring ? (max(0, i0 - b) / ring) * ring : 0;  // ring slot of row j: j - rb (- ring)
-    const int i = i0 + lane;
-    if (cols_here && i < n) {
-      double hi[G], lo[G];
-#pragma unroll
-      for (int t = 0; t < G; ++t) {
-        hi[t] = c0 + t < ncols ? stg[lane * ldr + c0 + t] : 0.0;
-        lo[t] = 0.0;
+    // byte offset of x row j: in the ring, or in X
+    auto xoff = [&](int j) -> int {
+      if constexpr (RING) {
+        int slot = j - rb;
+        slot = slot >= ring ? slot - ring : slot;
+        return slot * ldr * (int)sizeof(double);
+      } else {
+        return j * ncols * (int)sizeof(double);
       }
-      if (it > 0) {
-        const int s1 = rowptr[i + 1];
-        for (int sb = rowptr[i]; sb < s1; sb += 4) {
-          // the chunk's CSR slots are loaded together, then applied in order
-          double av[4];
-          int jv[4];
-#pragma unroll
-          for (int u = 0; u < 4; ++u) {
-            const bool v = sb + u < s1;
-            av[u] = v ? -a[sb + u] : 0.0;
-            jv[u] = v ? cols[sb + u] : i;
-          }
-#pragma unroll
-          for (int u = 0; u < 4; ++u) {
-            if (sb + u >= s1) break;
-            double xv[G];
-            if (use_ring) {
-              int slot = jv[u] - rb;
-              slot = slot >= ring ? slot - ring : slot;
-              const double *src = xr + slot * ldr + c0;
-#pragma unroll
-              for (int t = 0; t < G; ++t) xv[t] = src[t];
-            } else {
-              const double *src = x + (size_t)jv[u] * ncols;
-#pragma unroll
-              for (int t = 0; t < G; ++t) xv[t] = c0 + t < ncols ? src[c0 + t] : 0.0;
-            }
-#pragma unroll
-            for (int t = 0; t < G; ++t) dd_fma(hi[t], lo[t], av[u], xv[t]);
-          }
-        }
-        if (L.k) {
-          const double ci = cs * constraint_at(L, B, mac, i);
-          double xi[G];
-          if (use_ring) {
-            int slot = i - rb;
-            slot = slot >= ring ? slot - ring : slot;
-#pragma unroll
-            for (int t = 0; t < G; ++t) xi[t] = xr[slot * ldr + c0 + t];
-          } else {
-#pragma unroll
-            for (int t = 0; t < G; ++t) xi[t] = c0 + t < ncols ? x[(size_t)i * ncols + c0 + t] : 0.0;
-          }
-#pragma unroll
-          for (int t = 0; t < G; ++t) {
-            const int c = min(c0 + t, 63);
-            dd_fma(hi[t], lo[t], -ci, sh.mu[c]);
-            dd_fma(chi[t], clo[t], ci, xi[t]);
-          }
-        }
+    };
+    {
+      const int base = rowptr[i0], cnt = rowptr[min(n, i0 + RES_ROWS)] - base;
+      for (int e = tid; e < cnt; e += NT) {
+        ca[e] = -a[base + e];
+        cj[e] = xoff(cols[base + e]);
       }
+      if (tid <= RES_ROWS) crp[tid] = rowptr[min(n, i0 + tid)] - base;
+    }
+    __syncthreads();
+    // the lane's column in the ring or in X; a slot adds its row's byte offset
+    const char *xcol = RING ? reinterpret_cast<const char *>(xr + cl) : reinterpret_cast<const char *>(x + cl);
+    auto xat = [&](int off) -> double { return *reinterpret_cast<const double *>(xcol + off); };
+    for (int q = wave; q < RES_ROWS; q += RES_WAVES) {
+      const int i = i0 + q;
+      if (i >= n) break;
+      double hi = stg[q * ldr + cl], lo = 0.0;
+      const int s1 = __builtin_amdgcn_readfirstlane(crp[q + 1]);
+      int sb = __builtin_amdgcn_readfirstlane(crp[q]);
+      for (; sb + 4 <= s1; sb += 4) {
+        double av[4], xv[4];
+        int ov[4];
 #pragma unroll
-      for (int t = 0; t < G; ++t) {
-        const double r = hi[t] + lo[t];
+        for (int u = 0; u < 4; ++u) {
+          av[u] = ca[sb + u];
+          ov[u] = cj[sb + u];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) xv[u] = xat(ov[u]);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) dd_fma(hi, lo, av[u], xv[u]);
+      }
+      for (; sb < s1; ++sb) dd_fma(hi, lo, ca[sb], xat(cj[sb]));
+      if (L.k) {
+        const double ci = cs * constraint_at(L, B, mac, i);
+        dd_fma(hi, lo, -ci, sh.mu[cl]);
+        dd_fma(chi, clo, ci, xat(xoff(i)));
+      }
+      const double r = hi + lo;
+      if (on) {
         finite &= isfinite(r);
-        rr[t] += r * r;
-        if (!check_only && c0 + t < ncols) stg[lane * ldr + c0 + t] = ds[i] * r;
+        rr += r * r;
+        if (!check_only) stg[q * ldr + cl] = ds[i] * r;
       }
     }
     if (!check_only) {
@@ -2007,47 +2016,36 @@ __global__ void __launch_bounds__(256)
       ring_issue(hi, min(n, i0 + 3 * RES_ROWS + b));
     }
   }
-  // per-column sums over the lanes (each wavefront owns its columns), left in
-  // the stage: tot[k*64 + c] = ||r||^2, ||b||^2, constraint row (hi, lo), bad
-  double *tot = stg;
+  // per-wavefront column sums over the dead ring and stage: tot[(w*4 + k)*64
+  // + c] = ||r||^2, constraint row (hi, lo), nonfinite flag
+  double *tot = xr;
   __syncthreads();  // the last chunk's stage is written out
-  const double badw = wave_sum(finite ? 0.0 : 1.0);
-#pragma unroll
-  for (int t = 0; t < G; ++t) {
-    const double r2 = wave_sum(rr[t]);
-    // ||b||^2: this pass's r = b when it == 0, else the value kept by that pass
-    const double b2 = it == 0 ? r2 : (c0 + t < ncols ? bnorm[(size_t)mac * ncols + c0 + t] : 0.0);
-    double h = chi[t], l = clo[t];
-    for (int off = 32; off > 0; off >>= 1) {
-      const double oh = __shfl_down(h, off), ol = __shfl_down(l, off);
-      double ss, ee;
-      two_sum(h, oh, ss, ee);
-      h = ss;
-      l = l + ol + ee;
-    }
-    if (lane == 0 && cols_here && c0 + t < ncols && t < cq) {
-      tot[0 * 64 + c0 + t] = r2;
-      tot[1 * 64 + c0 + t] = b2;
-      if (it == 0 && !check_only) bnorm[(size_t)mac * ncols + c0 + t] = r2;
-      tot[2 * 64 + c0 + t] = h;
-      tot[3 * 64 + c0 + t] = l;
-      tot[4 * 64 + c0 + t] = badw;
-    }
-  }
+  tot[(wave * 4 + 0) * 64 + lane] = rr;
+  tot[(wave * 4 + 1) * 64 + lane] = chi;
+  tot[(wave * 4 + 2) * 64 + lane] = clo;
+  tot[(wave * 4 + 3) * 64 + lane] = finite ? 0.0 : 1.0;
   __syncthreads();
   if (!mine) return;
   const int c = tid;
-  double R2 = tot[0 * 64 + c], B2 = tot[1 * 64 + c];
-  const double BAD = tot[4 * 64 + c];
+  double R2 = 0.0, BAD = 0.0, h = 0.0, l = 0.0;
+  for (int w = 0; w < RES_WAVES; ++w) {
+    R2 += tot[(w * 4 + 0) * 64 + c];
+    double ss, ee;
+    two_sum(h, tot[(w * 4 + 1) * 64 + c], ss, ee);
+    h = ss;
+    l = l + tot[(w * 4 + 2) * 64 + c] + ee;
+    BAD += tot[(w * 4 + 3) * 64 + c];
+  }
+  double B2 = bnorm[(size_t)mac * ncols + c];
   double rn = 0.0;
   if (L.k) {
     const double bn = rhs ? rhs[((size_t)mac * N + n) * ncols + c] : 0.0;
-    rn = bn - (tot[2 * 64 + c] + tot[3 * 64 + c]);
+    rn = bn - (h + l);
     R2 += rn * rn;
     B2 += bn * bn;
   }
   const bool resid_ok = sqrt(R2) <= rtol * sqrt(B2) && BAD == 0.0;
-  bool ok = resid_ok && it > 0;  // the initial solve always runs
+  bool ok = resid_ok;
   if (ok && fwd_tol > 0.0) {
     // forward convergence: the last correction is below fwd_tol ||x||, or it
     // no longer halves (refinement has reached the binary64 noise floor)
@@ -2131,16 +2129,6 @@ __global__ void __launch_bounds__(256)
     beta[(size_t)mac * ncols + tid] = rhs ? rhs[((size_t)mac * N + n) * ncols + tid] : 0.0;
   }
   if (tid == 0 && bad) atomicOr(&B.status[mac], MHM_STATUS_NONFINITE);
-}
-
-// Column width per wavefront of the residual (compile-time G >= ceil(ncols/4)).
-static const int kResWidths[] = {2, 4, 6, 9, 12, 16};
-
-static int res_width(int ncols) {
-  const int cq = (ncols + 3) / 4;
-  for (int g : kResWidths)
-    if (cq <= g) return g;
-  return 0;
 }
 
 // x = D y (+ bordered correction when k == 1); it == 0 sets X, later
@@ -2486,6 +2474,7 @@ struct mhm_layout {
   int device = 0;
   LayoutK k{};
   int gather_slots = 0;  // most CSR slots in GATHER_ROWS consecutive rows
+  int res_slots = 0;     // most CSR slots in a residual chunk (RES_ROWS rows from a multiple of RES_ROWS)
   const double *coupling_rm = nullptr;  // template coupling, row-major n x m (band rows)
   std::vector<void *> owned;
   double cmax_host = 0.0;
@@ -3049,6 +3038,8 @@ int mhm_layout_create(int32_t device, const mhm_layout_desc *d, mhm_layout **out
     if ((rc = upload(L->owned, srow.data(), srow.size(), &k.srow))) return cleanup(rc);
     for (int i0 = 0; i0 < d->n; i0 += GATHER_ROWS)
       L->gather_slots = std::max(L->gather_slots, d->rowptr[std::min(d->n, i0 + GATHER_ROWS)] - d->rowptr[i0]);
+    for (int i0 = 0; i0 < d->n; i0 += RES_ROWS)
+      L->res_slots = std::max(L->res_slots, d->rowptr[std::min(d->n, i0 + RES_ROWS)] - d->rowptr[i0]);
   }
   if (d->ne > 0) {
     const size_t ne = d->ne, nb = d->nb, nq = d->nq, dim = d->dim;
@@ -3420,33 +3411,25 @@ template <typename T>
 static int launch_residual(mhm_batch *b, int ncols, const double *rhs, const double *X, double rtol, double fwd,
                            int it, int check_only, hipStream_t st) {
   const LayoutK &L = b->layout->k;
-  const int G = res_width(ncols);
-  const int ldr = 4 * G + 1;  // odd row stride (in doubles) spreads the ring rows over the LDS banks
+  if (it < 1 || ncols < 1 || ncols > 64) return fail(MHM_EINVAL, "residual pass needs it >= 1 and 1..64 columns");
   int ring = RES_ROWS + 2 * L.b;
-  const size_t stage = (size_t)RES_ROWS * ldr * sizeof(double);  // the chunk's rhs / output stage
-  size_t shm = (size_t)ring * ldr * sizeof(double) + stage;
-  // the first pass (x = 0, r = b) reads no x: no ring, so the launch is not
-  // held to the ring's occupancy
-  if (it == 0 || shm + sizeof(ResShared) > (size_t)lds_limit(b->layout->device)) {
+  const int slots = b->layout->res_slots;
+  const size_t stage = res_stage(ncols, slots);
+  const size_t totals = (size_t)RES_WAVES * 4 * 64 * sizeof(double);  // the final per-wavefront sums
+  size_t shm = std::max((size_t)ring * ncols * sizeof(double) + stage, totals);
+  if (shm + sizeof(ResShared) > (size_t)lds_limit(b->layout->device)) {
     ring = 0;
-    shm = stage;
+    shm = std::max(stage, totals);
   }
-#define MHM_RES(g)                                                                                          \
-  case g:                                                                                                   \
-    if (shm > 65536)                                                                                        \
-      HIP_TRY(hipFuncSetAttribute((const void *)k_residual<T, g>, hipFuncAttributeMaxDynamicSharedMemorySize, \
-                                  (int)shm));                                                               \
-    hipLaunchKernelGGL((k_residual<T, g>), dim3(b->M), dim3(256), shm, st, L, b->bk, ncols, rhs, X, rtol, fwd, \
-                       it, (T *)b->y, b->beta, b->colstate, (const double *)b->colnorm, check_only, L.rowptr,  \
-                       L.cols, (const double *)b->bk.aval, (const double *)b->bk.dscale,                     \
-                       b->layout->coupling_rm, ring, ldr, b->bnorm);                                        \
-    break;
-  switch (G) {
-    MHM_RES(2) MHM_RES(4) MHM_RES(6) MHM_RES(9) MHM_RES(12) MHM_RES(16)
-    default:
-      return fail(MHM_EINVAL, "too many columns for the residual kernel");
-  }
-#undef MHM_RES
+  if (shm + sizeof(ResShared) > (size_t)lds_limit(b->layout->device))
+    return fail(MHM_EINVAL, "residual chunk exceeds the LDS budget");
+  auto kern = ring > 0 ? k_residual<T, true> : k_residual<T, false>;
+  if (shm > 65536)
+    HIP_TRY(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+  hipLaunchKernelGGL(kern, dim3(b->M), dim3(64 * RES_WAVES), shm, st, L, b->bk, ncols, rhs, X, rtol, fwd, it,
+                     (T *)b->y, b->beta, b->colstate, (const double *)b->colnorm, check_only, L.rowptr, L.cols,
+                     (const double *)b->bk.aval, (const double *)b->bk.dscale, b->layout->coupling_rm, ring, slots,
+                     b->bnorm);
   LAUNCH_CHECK("k_residual");
   return MHM_OK;
 }
