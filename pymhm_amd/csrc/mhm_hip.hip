@@ -1799,13 +1799,21 @@ __device__ __forceinline__ double rhs_at(const LayoutK &L, const BatchK &B, cons
   return B.az[(size_t)mac * L.n + i];
 }
 
-// r = b - S x in double-double, per-column acceptance ||r|| <= rtol ||b||
-// (linalg/linear.py:132-152; the max|b| column scaling of the reference
-// cancels from both sides), scaled correction rhs D r into y, constraint-row
-// residual into beta, for x from a refinement pass (it >= 1; the first pass
-// is k_rhs_init).  fwd_tol > 0 also requires forward convergence of the
-// refinement.  check_only: set MHM_STATUS_RESIDUAL for columns that miss the
-// criterion.  ||b||^2 comes from bnorm (kept by k_rhs_init).
+// r = b - S x, per-column acceptance ||r|| <= rtol ||b|| (linalg/linear.py:
+// 132-152; the max|b| column scaling of the reference cancels from both
+// sides), scaled correction rhs D r into y, constraint-row residual into beta,
+// for x from a refinement pass (it >= 1; the first pass is k_rhs_init).  Like
+// the reference, in two stages:
+//  FAST: binary64 products and sums for every pending column, with the
+//        rounding bound E = gam_row ||(|b| + |S||x|)|| (gam_con for the
+//        constraint row).  A column that meets the criterion is accepted;
+//        one whose binary64 residual is at least 16 E (so within 1/16 of the
+//        exact one) keeps it as its correction (rdone = 1);
+//  DD:   double-double evaluation for the pending columns left (rdone = 0,
+//        near the rounding floor), and for the final check (check_only: set
+//        MHM_STATUS_RESIDUAL for columns that miss the criterion).
+// fwd_tol > 0 also requires forward convergence of the refinement.  ||b||^2
+// comes from bnorm (kept by k_rhs_init).
 // Block per macro, RES_WAVES wavefronts, lane c = column c (ncols <= 64).
 // Rows go in chunks of RES_ROWS; wavefront w takes rows i0 + w, i0 + w +
 // RES_WAVES, ... of the chunk, one at a time.  The chunk's CSR slots (values,
@@ -1824,6 +1832,7 @@ __device__ __forceinline__ double rhs_at(const LayoutK &L, const BatchK &B, cons
 // HBM directly.
 #define RES_ROWS 32
 #define RES_WAVES 8
+#define RES_CPF 4  // CSR slots per thread of a chunk: at most 64 * RES_WAVES * RES_CPF per chunk
 
 struct ResShared {
   const double *colptr[64];
@@ -1833,20 +1842,21 @@ struct ResShared {
   double zero;
 };
 
-// LDS of the residual past the x ring: the chunk's right-hand side rows, its
-// CSR values, column indices and row offsets (at most slots per chunk).
+// LDS of the residual past the x ring: the chunk's right-hand side rows, row
+// scales and constraint entries, its CSR values, column indices and row
+// offsets (at most slots per chunk).
 static size_t res_stage(int ncols, int slots) {
-  return ((size_t)RES_ROWS * ncols + slots) * sizeof(double) + ((size_t)slots + RES_ROWS + 1) * sizeof(int);
+  return ((size_t)RES_ROWS * (ncols + 2) + slots) * sizeof(double) + ((size_t)slots + RES_ROWS + 1) * sizeof(int);
 }
 
-template <typename T, bool RING>
+template <typename T, bool RING, bool FAST>
 __global__ void __launch_bounds__(64 * RES_WAVES)
     k_residual(LayoutK L, BatchK B, int ncols, const double *__restrict__ rhs, const double *__restrict__ X,
                double rtol, double fwd_tol, int it, T *__restrict__ y, double *__restrict__ beta,
                int *__restrict__ colstate, const double *__restrict__ colnorm, int check_only,
                const int *__restrict__ rowptr, const int *__restrict__ cols, const double *__restrict__ aval,
                const double *__restrict__ dscale, const double *__restrict__ coupling_rm, int ring,
-               int slots, double *__restrict__ bnorm) {
+               int slots, double *__restrict__ bnorm, int *__restrict__ rdone, double gam_row, double gam_con) {
   constexpr int NT = 64 * RES_WAVES;
   extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
   __shared__ ResShared sh;
@@ -1859,8 +1869,10 @@ __global__ void __launch_bounds__(64 * RES_WAVES)
   const bool on = lane < ncols;
   const int cl = on ? lane : 0;  // lanes past ncols read column 0 and store nothing
   int *state = colstate + (size_t)mac * ncols;
-  // converged columns were accepted by the residual of their current X
-  const bool mine = tid < ncols && state[tid] == 0;
+  // converged columns were accepted by the residual of their current X; the
+  // double-double stage takes the pending columns the binary64 stage left
+  int *done = rdone + (size_t)mac * ncols;
+  const bool mine = tid < ncols && state[tid] == 0 && (FAST || check_only || done[tid] == 0);
   if (tid < 64) sh.pend[tid] = mine ? 1 : 0;
   const double *x = X + (size_t)mac * N * ncols;
   if (tid < 64) {
@@ -1898,17 +1910,12 @@ __global__ void __launch_bounds__(64 * RES_WAVES)
   const double cs = B.scal[(size_t)mac * NSCAL + S_CS];
   T *Y = y + (size_t)mac * n * B.ldy;
   double *stg = xr + (size_t)ring * ldr;
-  double *ca = stg + (size_t)RES_ROWS * ldr;      // the chunk's CSR values (negated)
+  double *cds = stg + (size_t)RES_ROWS * ldr;     // the chunk's row scales
+  double *ccn = cds + RES_ROWS;                   // ... scaled constraint entries
+  double *ca = ccn + RES_ROWS;                    // ... CSR values (negated)
   int *cj = reinterpret_cast<int *>(ca + slots);  // ... byte offsets of their x rows (ring or X)
   int *crp = cj + slots;                          // ... row offsets from the chunk's first slot
   const FastDiv divc(ncols);
-  auto load_chunk = [&](int i0) {
-    for (int e = tid; e < RES_ROWS * ncols; e += NT) {
-      const int q = divc(e), c = e - q * ncols;
-      const int i = i0 + q;
-      stg[q * ldr + c] = i < n ? global_ptr(sh.colptr[c])[(size_t)i * sh.colstride[c]] * sh.colscale[c] : 0.0;
-    }
-  };
   auto store_chunk = [&](int i0) {
     for (int e = tid; e < RES_ROWS * ncols; e += NT) {
       const int q = divc(e), c = e - q * ncols;
@@ -1937,6 +1944,66 @@ __global__ void __launch_bounds__(64 * RES_WAVES)
       if (lo + q < hi) xr[((lo + q) % ring) * ldr + c] = xpf[v];
     }
   };
+  // byte offset of x row j for the chunk at i0: in the ring, or in X
+  auto xoff = [&](int i0, int j) -> int {
+    if constexpr (RING) {
+      const int rb = (max(0, i0 - b) / ring) * ring;  // ring slot of row j: j - rb (- ring)
+      int slot = j - rb;
+      slot = slot >= ring ? slot - ring : slot;
+      return slot * ldr * (int)sizeof(double);
+    } else {
+      return j * ncols * (int)sizeof(double);
+    }
+  };
+  // a chunk's right-hand side and CSR slots, loaded into registers one chunk
+  // ahead and committed to the stage at the chunk boundary (the loaded values
+  // are not touched before the commit, so the loads stay in flight meanwhile)
+  constexpr int SPF = RES_ROWS * 64 / NT;  // right-hand side values per thread
+  double spf[SPF], apf[RES_CPF], dpf = 0.0, cpf = 0.0;
+  int jpf[RES_CPF], rpf = 0, ccnt = 0, cbase = 0;
+  auto chunk_issue = [&](int i0) {
+#pragma unroll
+    for (int v = 0; v < SPF; ++v) {
+      const int e = tid + NT * v;
+      const int q = divc(e), c = e - q * ncols;
+      const int i = i0 + q;
+      const bool in = e < RES_ROWS * ncols && i < n;
+      spf[v] = *(in ? global_ptr(sh.colptr[c]) + (size_t)i * sh.colstride[c] : global_ptr(kZeroF64));
+    }
+    cbase = rowptr[i0];
+    ccnt = rowptr[min(n, i0 + RES_ROWS)] - cbase;
+#pragma unroll
+    for (int v = 0; v < RES_CPF; ++v) {
+      const int e = min(tid + NT * v, max(ccnt - 1, 0));  // past the chunk: a repeated slot
+      apf[v] = a[cbase + e];
+      jpf[v] = cols[cbase + e];
+    }
+    rpf = rowptr[min(n, i0 + min(tid, RES_ROWS))];
+    const int ir = min(i0 + min(tid, RES_ROWS - 1), n - 1);
+    dpf = ds[ir];
+    cpf = L.k ? (B.cons ? B.cons[(size_t)mac * n + ir] : L.constraint[ir]) : 0.0;
+  };
+  auto chunk_commit = [&](int i0) {
+#pragma unroll
+    for (int v = 0; v < SPF; ++v) {
+      const int e = tid + NT * v;
+      const int q = divc(e), c = e - q * ncols;
+      if (e < RES_ROWS * ncols) stg[q * ldr + c] = spf[v] * sh.colscale[c];
+    }
+#pragma unroll
+    for (int v = 0; v < RES_CPF; ++v) {
+      const int e = tid + NT * v;
+      if (e < ccnt) {
+        ca[e] = -apf[v];
+        cj[e] = xoff(i0, jpf[v]);
+      }
+    }
+    if (tid <= RES_ROWS) crp[tid] = rpf - cbase;
+    if (tid < RES_ROWS) {
+      cds[tid] = dpf;
+      ccn[tid] = cs * cpf;
+    }
+  };
   if (use_ring) {
     for (int e = tid; e < min(n, RES_ROWS + b) * ncols; e += NT) {
       const int q = divc(e), c = e - q * ncols;
@@ -1944,38 +2011,30 @@ __global__ void __launch_bounds__(64 * RES_WAVES)
     }
     ring_issue(RES_ROWS + b, min(n, 2 * RES_ROWS + b));
   }
-  double rr = 0.0, chi = 0.0, clo = 0.0;
+  chunk_issue(0);
+  chunk_commit(0);
+  double rr = 0.0, chi = 0.0, clo = 0.0;  // FAST: clo = sum |c_i x_i|
+  double tt = 0.0;                         // FAST: sum of (|b| + |S||x|)^2
   bool finite = true;
+  // the lane's column in the ring or in X; a slot adds its row's byte offset
+  const char *xcol = RING ? reinterpret_cast<const char *>(xr + cl) : reinterpret_cast<const char *>(x + cl);
+  auto xat = [&](int off) -> double { return *reinterpret_cast<const double *>(xcol + off); };
+  __syncthreads();
   for (int i0 = 0; i0 < n; i0 += RES_ROWS) {
-    __syncthreads();  // the previous chunk's stage is written out (and the ring committed)
-    load_chunk(i0);
-    const int rb = use_ring ? (max(0, i0 - b) / ring) * ring : 0;  // ring slot of row j: j - rb (- ring)
-    // byte offset of x row j: in the ring, or in X
-    auto xoff = [&](int j) -> int {
-      if constexpr (RING) {
-        int slot = j - rb;
-        slot = slot >= ring ? slot - ring : slot;
-        return slot * ldr * (int)sizeof(double);
-      } else {
-        return j * ncols * (int)sizeof(double);
-      }
-    };
-    {
-      const int base = rowptr[i0], cnt = rowptr[min(n, i0 + RES_ROWS)] - base;
-      for (int e = tid; e < cnt; e += NT) {
-        ca[e] = -a[base + e];
-        cj[e] = xoff(cols[base + e]);
-      }
-      if (tid <= RES_ROWS) crp[tid] = rowptr[min(n, i0 + tid)] - base;
-    }
-    __syncthreads();
-    // the lane's column in the ring or in X; a slot adds its row's byte offset
-    const char *xcol = RING ? reinterpret_cast<const char *>(xr + cl) : reinterpret_cast<const char *>(x + cl);
-    auto xat = [&](int off) -> double { return *reinterpret_cast<const double *>(xcol + off); };
+    const bool more = i0 + RES_ROWS < n;
+    if (more) chunk_issue(i0 + RES_ROWS);
     for (int q = wave; q < RES_ROWS; q += RES_WAVES) {
       const int i = i0 + q;
       if (i >= n) break;
-      double hi = stg[q * ldr + cl], lo = 0.0;
+      double hi = stg[q * ldr + cl], lo = FAST ? fabs(hi) : 0.0;  // FAST: lo = |b| + |S||x|
+      auto term = [&](double av, double xv) {
+        if constexpr (FAST) {
+          hi = fma(av, xv, hi);
+          lo = fma(fabs(av), fabs(xv), lo);
+        } else {
+          dd_fma(hi, lo, av, xv);
+        }
+      };
       const int s1 = __builtin_amdgcn_readfirstlane(crp[q + 1]);
       int sb = __builtin_amdgcn_readfirstlane(crp[q]);
       for (; sb + 4 <= s1; sb += 4) {
@@ -1989,60 +2048,77 @@ __global__ void __launch_bounds__(64 * RES_WAVES)
 #pragma unroll
         for (int u = 0; u < 4; ++u) xv[u] = xat(ov[u]);
 #pragma unroll
-        for (int u = 0; u < 4; ++u) dd_fma(hi, lo, av[u], xv[u]);
+        for (int u = 0; u < 4; ++u) term(av[u], xv[u]);
       }
-      for (; sb < s1; ++sb) dd_fma(hi, lo, ca[sb], xat(cj[sb]));
+      for (; sb < s1; ++sb) term(ca[sb], xat(cj[sb]));
       if (L.k) {
-        const double ci = cs * constraint_at(L, B, mac, i);
-        dd_fma(hi, lo, -ci, sh.mu[cl]);
-        dd_fma(chi, clo, ci, xat(xoff(i)));
+        const double ci = ccn[q];
+        const double xi = xat(xoff(i0, i));
+        term(-ci, sh.mu[cl]);
+        if constexpr (FAST) {
+          chi = fma(ci, xi, chi);
+          clo = fma(fabs(ci), fabs(xi), clo);
+        } else {
+          dd_fma(chi, clo, ci, xi);
+        }
       }
-      const double r = hi + lo;
+      const double r = FAST ? hi : hi + lo;
+      if (FAST && on) tt += lo * lo;
       if (on) {
         finite &= isfinite(r);
         rr += r * r;
-        if (!check_only) stg[q * ldr + cl] = ds[i] * r;
+        if (!check_only) stg[q * ldr + cl] = cds[q] * r;
       }
     }
-    if (!check_only) {
-      __syncthreads();
-      store_chunk(i0);
-    }
-    if (use_ring && i0 + RES_ROWS < n) {
+    __syncthreads();  // the chunk's stage and ring reads are done
+    if (!check_only) store_chunk(i0);
+    if (use_ring && more) {
       // rows of the next chunk into the slots of rows [i0 - b, i0 - b + RES_ROWS)
       const int lo = i0 + RES_ROWS + b, hi = min(n, i0 + 2 * RES_ROWS + b);
-      __syncthreads();  // the chunk's ring reads are done
       ring_commit(lo, hi);
       ring_issue(hi, min(n, i0 + 3 * RES_ROWS + b));
     }
+    __syncthreads();  // the stage is written out, the ring holds the next chunk's rows
+    if (more) {
+      chunk_commit(i0 + RES_ROWS);
+      __syncthreads();
+    }
   }
-  // per-wavefront column sums over the dead ring and stage: tot[(w*4 + k)*64
-  // + c] = ||r||^2, constraint row (hi, lo), nonfinite flag
+  // per-wavefront column sums over the dead ring and stage: tot[(w*5 + k)*64
+  // + c] = ||r||^2, constraint row (hi, lo), nonfinite flag, FAST bound sum
   double *tot = xr;
   __syncthreads();  // the last chunk's stage is written out
-  tot[(wave * 4 + 0) * 64 + lane] = rr;
-  tot[(wave * 4 + 1) * 64 + lane] = chi;
-  tot[(wave * 4 + 2) * 64 + lane] = clo;
-  tot[(wave * 4 + 3) * 64 + lane] = finite ? 0.0 : 1.0;
+  tot[(wave * 5 + 0) * 64 + lane] = rr;
+  tot[(wave * 5 + 1) * 64 + lane] = chi;
+  tot[(wave * 5 + 2) * 64 + lane] = clo;
+  tot[(wave * 5 + 3) * 64 + lane] = finite ? 0.0 : 1.0;
+  tot[(wave * 5 + 4) * 64 + lane] = tt;
   __syncthreads();
   if (!mine) return;
   const int c = tid;
-  double R2 = 0.0, BAD = 0.0, h = 0.0, l = 0.0;
+  double R2 = 0.0, BAD = 0.0, h = 0.0, l = 0.0, T2 = 0.0;
   for (int w = 0; w < RES_WAVES; ++w) {
-    R2 += tot[(w * 4 + 0) * 64 + c];
-    double ss, ee;
-    two_sum(h, tot[(w * 4 + 1) * 64 + c], ss, ee);
-    h = ss;
-    l = l + tot[(w * 4 + 2) * 64 + c] + ee;
-    BAD += tot[(w * 4 + 3) * 64 + c];
+    R2 += tot[(w * 5 + 0) * 64 + c];
+    if (FAST) {
+      h += tot[(w * 5 + 1) * 64 + c];
+      l += tot[(w * 5 + 2) * 64 + c];
+      T2 += tot[(w * 5 + 4) * 64 + c];
+    } else {
+      double ss, ee;
+      two_sum(h, tot[(w * 5 + 1) * 64 + c], ss, ee);
+      h = ss;
+      l = l + tot[(w * 5 + 2) * 64 + c] + ee;
+    }
+    BAD += tot[(w * 5 + 3) * 64 + c];
   }
   double B2 = bnorm[(size_t)mac * ncols + c];
-  double rn = 0.0;
+  double rn = 0.0, E2 = 0.0;
   if (L.k) {
     const double bn = rhs ? rhs[((size_t)mac * N + n) * ncols + c] : 0.0;
-    rn = bn - (h + l);
+    rn = FAST ? bn - h : bn - (h + l);
     R2 += rn * rn;
     B2 += bn * bn;
+    if (FAST) E2 = gam_con * (fabs(bn) + l) * (gam_con * (fabs(bn) + l));
   }
   const bool resid_ok = sqrt(R2) <= rtol * sqrt(B2) && BAD == 0.0;
   bool ok = resid_ok;
@@ -2051,6 +2127,18 @@ __global__ void __launch_bounds__(64 * RES_WAVES)
     // no longer halves (refinement has reached the binary64 noise floor)
     const double *cn = colnorm + ((size_t)mac * ncols + c) * 3;
     ok = cn[0] <= fwd_tol * fwd_tol * cn[1] || cn[0] >= 0.25 * cn[2];
+  }
+  if (FAST) {
+    // accepted, or a correction from the binary64 residual when it is at
+    // least 16 times its rounding bound; the rest go to the double-double stage
+    E2 += gam_row * gam_row * T2;
+    if (ok) {
+      state[c] = 1;
+    } else {
+      beta[(size_t)mac * ncols + c] = rn;
+      done[c] = (BAD == 0.0 && R2 >= 256.0 * E2) ? 1 : 0;
+    }
+    return;
   }
   if (BAD != 0.0) atomicOr(&B.status[mac], MHM_STATUS_NONFINITE);
   if (check_only) {
@@ -2475,6 +2563,7 @@ struct mhm_layout {
   LayoutK k{};
   int gather_slots = 0;  // most CSR slots in GATHER_ROWS consecutive rows
   int res_slots = 0;     // most CSR slots in a residual chunk (RES_ROWS rows from a multiple of RES_ROWS)
+  int maxrow = 0;        // most CSR slots in one row
   const double *coupling_rm = nullptr;  // template coupling, row-major n x m (band rows)
   std::vector<void *> owned;
   double cmax_host = 0.0;
@@ -2492,6 +2581,7 @@ struct mhm_batch {
   double *rhsu = nullptr, *xu = nullptr, *srcscale = nullptr;
   double *glam = nullptr, *meas = nullptr;  // per-macro element geometry (uneven macros)
   int *colstate = nullptr;
+  int *rdone = nullptr;  // per column: the binary64 residual stage supplied the correction
   double *ework = nullptr, *fwork = nullptr;
   int *colmap = nullptr;
   double *colscale = nullptr, *bcoup = nullptr, *cons = nullptr, *kern = nullptr;
@@ -3040,6 +3130,7 @@ int mhm_layout_create(int32_t device, const mhm_layout_desc *d, mhm_layout **out
       L->gather_slots = std::max(L->gather_slots, d->rowptr[std::min(d->n, i0 + GATHER_ROWS)] - d->rowptr[i0]);
     for (int i0 = 0; i0 < d->n; i0 += RES_ROWS)
       L->res_slots = std::max(L->res_slots, d->rowptr[std::min(d->n, i0 + RES_ROWS)] - d->rowptr[i0]);
+    for (int i = 0; i < d->n; ++i) L->maxrow = std::max(L->maxrow, d->rowptr[i + 1] - d->rowptr[i]);
   }
   if (d->ne > 0) {
     const size_t ne = d->ne, nb = d->nb, nq = d->nq, dim = d->dim;
@@ -3130,7 +3221,7 @@ int mhm_batch_create(const mhm_layout *L, int32_t macros, int32_t precision, int
       (rc = dev_alloc(b->owned, &b->xstd, M * R * N)) ||
       (rc = dev_alloc(b->owned, &b->beta, M * ncm)) ||
       (rc = dev_alloc(b->owned, &b->colnorm, M * ncm * 3)) || (rc = dev_alloc(b->owned, &b->bnorm, M * ncm)) ||
-      (rc = dev_alloc(b->owned, &b->colstate, M * ncm)))
+      (rc = dev_alloc(b->owned, &b->colstate, M * ncm)) || (rc = dev_alloc(b->owned, &b->rdone, M * ncm)))
     return cleanup(rc);
   if (max_columns > 0 &&
       ((rc = dev_alloc(b->owned, &b->rhsu, M * max_columns * N)) ||
@@ -3156,6 +3247,7 @@ int mhm_batch_create(const mhm_layout *L, int32_t macros, int32_t precision, int
     }
   }
   HIP_TRY(hipMemset(k.status, 0, M * sizeof(int)));
+  HIP_TRY(hipMemset(b->rdone, 0, M * ncm * sizeof(int)));
   HIP_TRY(hipMemset(k.iters, 0, M * sizeof(int)));
   HIP_TRY(hipMemset(k.scal, 0, M * NSCAL * sizeof(double)));
   *out = b;
@@ -3415,22 +3507,30 @@ static int launch_residual(mhm_batch *b, int ncols, const double *rhs, const dou
   int ring = RES_ROWS + 2 * L.b;
   const int slots = b->layout->res_slots;
   const size_t stage = res_stage(ncols, slots);
-  const size_t totals = (size_t)RES_WAVES * 4 * 64 * sizeof(double);  // the final per-wavefront sums
+  const size_t totals = (size_t)RES_WAVES * 5 * 64 * sizeof(double);  // the final per-wavefront sums
   size_t shm = std::max((size_t)ring * ncols * sizeof(double) + stage, totals);
   if (shm + sizeof(ResShared) > (size_t)lds_limit(b->layout->device)) {
     ring = 0;
     shm = std::max(stage, totals);
   }
-  if (shm + sizeof(ResShared) > (size_t)lds_limit(b->layout->device))
-    return fail(MHM_EINVAL, "residual chunk exceeds the LDS budget");
-  auto kern = ring > 0 ? k_residual<T, true> : k_residual<T, false>;
-  if (shm > 65536)
-    HIP_TRY(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
-  hipLaunchKernelGGL(kern, dim3(b->M), dim3(64 * RES_WAVES), shm, st, L, b->bk, ncols, rhs, X, rtol, fwd, it,
-                     (T *)b->y, b->beta, b->colstate, (const double *)b->colnorm, check_only, L.rowptr, L.cols,
-                     (const double *)b->bk.aval, (const double *)b->bk.dscale, b->layout->coupling_rm, ring, slots,
-                     b->bnorm);
-  LAUNCH_CHECK("k_residual");
+  if (shm + sizeof(ResShared) > (size_t)lds_limit(b->layout->device) || slots > 64 * RES_WAVES * RES_CPF)
+    return fail(MHM_EINVAL, "residual chunk exceeds the LDS or register budget");
+  // rounding bounds of the binary64 stage: a row's sum has at most maxrow + 2
+  // terms, the constraint row n + 1
+  const double u = 0x1p-53;
+  const double kr = b->layout->maxrow + 2.0, kc = L.n + 1.0;
+  const double gam_row = kr * u / (1.0 - kr * u), gam_con = kc * u / (1.0 - kc * u);
+  for (int stage = check_only ? 1 : 0; stage < 2; ++stage) {
+    auto kern = stage == 0 ? (ring > 0 ? k_residual<T, true, true> : k_residual<T, false, true>)
+                           : (ring > 0 ? k_residual<T, true, false> : k_residual<T, false, false>);
+    if (shm > 65536)
+      HIP_TRY(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+    hipLaunchKernelGGL(kern, dim3(b->M), dim3(64 * RES_WAVES), shm, st, L, b->bk, ncols, rhs, X, rtol, fwd, it,
+                       (T *)b->y, b->beta, b->colstate, (const double *)b->colnorm, check_only, L.rowptr, L.cols,
+                       (const double *)b->bk.aval, (const double *)b->bk.dscale, b->layout->coupling_rm, ring, slots,
+                       b->bnorm, b->rdone, gam_row, gam_con);
+    LAUNCH_CHECK("k_residual");
+  }
   return MHM_OK;
 }
 
