@@ -54,7 +54,7 @@ METRIC = "macro solves/s at <=1e-10 rel err vs FP64 oracle (vs plain FP64); p50/
 # rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE and scaled by the counters' reading of
 # a 1 GiB device copy (profiles/traffic_fp64_config2.md); quoted as
 # roofline.traffic when the kernel is the dominant one
-MEASURED_TRAFFIC_PER_MACRO = {"k_factor_panel<double>": 573353 + 1379382}
+MEASURED_TRAFFIC_PER_MACRO = {"k_factor_panel<double>": 144964 + 1502216, "k_solve_mfma<17, 3>": 2705593 + 632995}
 HBM_PEAK_GBS = 8000.0
 PEAK_TFLOPS = {"f64": 78.6, "f32": 157.3}
 
@@ -77,10 +77,21 @@ def factor_kernel(b, tname):
     return f"k_factor_panel<{tname}>" if 16 + b <= 127 else f"k_factor_wide<{tname}>"
 
 
+MFMA_MAXK = (8, 12, 17, 24, 28)
+
+
 def solve_kernel(b, ncols, tname):
-    """The solve kernel for half-bandwidth b and ncols columns: the wavefront
-    solve with the smallest register window >= b + 1, else the block solve
-    with its register slots per thread (the library's tables)."""
+    """The solve kernel for half-bandwidth b and ncols columns: in binary64
+    with the LDS-window factor, the matrix-core solve with the smallest
+    register window of ceil(b / 4) k-steps x ceil(ncols / 16) column tiles
+    (at most 52 values per lane); otherwise the wavefront solve with the
+    smallest register window >= b + 1, else the block solve with its register
+    slots per thread (the library's tables)."""
+    if tname == "double" and 16 + b <= 127 and 1 <= ncols <= 64:
+        nct, nk = -(-ncols // 16), -(-b // 4)
+        k = next((k for k in MFMA_MAXK if nk <= k and k * nct <= 52), None)
+        if k is not None:
+            return f"k_solve_mfma<{k}, {nct}>"
     w = next((w for w in WAVE_WINDOWS if b + 1 <= w), None)
     if w is not None and ncols <= 64:
         return f"k_solve_wave<{tname}, {w}>"
@@ -133,18 +144,22 @@ def rel_err(a, b):
 def per_macro_traffic(prob, precision):
     """Algorithmic HBM bytes and flops per macro of the factor and of one solve.
 
-    Factor: read the prepared band, write the column band and the row band
-    (3 n W ts).  Solve: read both bands, read and write the workspace in both
-    sweeps (2 n W ts + 4 n ldy ts, ldy = R + 2 columns for the first solve).
+    Factor (LDS window): read the scaled operator in CSR slot order (nnz ts),
+    write the column band and the row band (2 n W ts) and, in binary64, the
+    16 x 16 diagonal blocks (ceil(n / 16) 256 ts).  Solve: read both bands (and
+    in binary64 the inverses of the diagonal blocks, once per sweep), read and
+    write the R + 2 workspace columns of the first solve in both sweeps
+    (4 n (R + 2) ts).
     """
     lay = prob.layout
     n, b = lay.n, lay.bandwidth
     W = b + 1
     ts = 8 if precision == "fp64" else 4
     R = 1 + lay.m + lay.k
+    blocks = -(-n // 16) * 256 * ts if precision == "fp64" else 0
     return dict(
-        factor_bytes=3 * n * W * ts, factor_flops=n * b * (b + 1),
-        solve_bytes=2 * n * W * ts + 4 * n * (R + 2) * ts, solve_flops=4 * n * b * R,
+        factor_bytes=lay.nnz * ts + 2 * n * W * ts + blocks, factor_flops=n * b * (b + 1),
+        solve_bytes=2 * n * W * ts + 2 * blocks + 4 * n * (R + 2) * ts, solve_flops=4 * n * b * R,
     )
 
 

@@ -3471,12 +3471,12 @@ static int run_factor(mhm_batch *b, hipStream_t st) {
                        (T *)b->rband, b->bk.status, tiny);
   }
   LAUNCH_CHECK("k_factor");
+  }
   if (sizeof(T) == 8 && b->dinv) {
     const size_t blocks = (size_t)b->M * ((L.n + 15) / 16);
     hipLaunchKernelGGL(k_diag_inverse, dim3((unsigned)((blocks + 15) / 16)), dim3(256), 0, st, (long)blocks,
                        b->dinv);
     LAUNCH_CHECK("k_diag_inverse");
-  }
   }
   // u = Ahat^-1 e_p and w = Ahat^-1 c ride along with the first solve
   b->uw_pending = L.k > 0;
