@@ -31,6 +31,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -2330,6 +2331,15 @@ __global__ void k_fill_int(int *p, int v, size_t count) {
   if (t < count) p[t] = v;
 }
 
+// done[mac] = 1 when every column of the macro has converged (its X is final).
+__global__ void k_mark_done(const int *__restrict__ colstate, int ncols, int macros, int *__restrict__ done) {
+  const int mac = blockIdx.x * blockDim.x + threadIdx.x;
+  if (mac >= macros) return;
+  int all = 1;
+  for (int c = 0; c < ncols; ++c) all &= colstate[(size_t)mac * ncols + c] == 1 ? 1 : 0;
+  done[mac] = all;
+}
+
 // ----------------------------------------------------------------- condensation
 // Block per macro: the condensed block of core/contributions.py:40-47 in the
 // general (corrected retained) form and the reduced load of
@@ -2344,10 +2354,12 @@ __global__ void k_fill_int(int *p, int v, size_t count) {
 
 __global__ void __launch_bounds__(256)
     k_condense(LayoutK L, BatchK B, const double *__restrict__ X, double *__restrict__ blocks,
-               double *__restrict__ loads) {
+               double *__restrict__ loads, const int *__restrict__ done, int mode) {
   extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
   __shared__ double fin[2];
   const int mac = blockIdx.x;
+  // mode 1: only macros marked done, mode 2: only the others, 0: all
+  if (mode != 0 && (done[mac] != 0) != (mode == 1)) return;
   const int tid = threadIdx.x;
   const int n = L.n, m = L.m, k = L.k, N = B.N, R = B.R;
   const int w = m + k;
@@ -2582,6 +2594,9 @@ struct mhm_batch {
   double *glam = nullptr, *meas = nullptr;  // per-macro element geometry (uneven macros)
   int *colstate = nullptr;
   int *rdone = nullptr;  // per column: the binary64 residual stage supplied the correction
+  int *done = nullptr;   // per macro: converged after the first refinement pass (early condensation)
+  hipStream_t side = nullptr;                    // early condensation, beside the caller's stream
+  hipEvent_t ev_mark = nullptr, ev_side = nullptr;
   double *ework = nullptr, *fwork = nullptr;
   int *colmap = nullptr;
   double *colscale = nullptr, *bcoup = nullptr, *cons = nullptr, *kern = nullptr;
@@ -3221,7 +3236,8 @@ int mhm_batch_create(const mhm_layout *L, int32_t macros, int32_t precision, int
       (rc = dev_alloc(b->owned, &b->xstd, M * R * N)) ||
       (rc = dev_alloc(b->owned, &b->beta, M * ncm)) ||
       (rc = dev_alloc(b->owned, &b->colnorm, M * ncm * 3)) || (rc = dev_alloc(b->owned, &b->bnorm, M * ncm)) ||
-      (rc = dev_alloc(b->owned, &b->colstate, M * ncm)) || (rc = dev_alloc(b->owned, &b->rdone, M * ncm)))
+      (rc = dev_alloc(b->owned, &b->colstate, M * ncm)) || (rc = dev_alloc(b->owned, &b->rdone, M * ncm)) ||
+      (rc = dev_alloc(b->owned, &b->done, M)))
     return cleanup(rc);
   if (max_columns > 0 &&
       ((rc = dev_alloc(b->owned, &b->rhsu, M * max_columns * N)) ||
@@ -3248,6 +3264,13 @@ int mhm_batch_create(const mhm_layout *L, int32_t macros, int32_t precision, int
   }
   HIP_TRY(hipMemset(k.status, 0, M * sizeof(int)));
   HIP_TRY(hipMemset(b->rdone, 0, M * ncm * sizeof(int)));
+  if (hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&b->ev_mark, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&b->ev_side, hipEventDisableTiming) != hipSuccess) {
+    if (b->ev_mark) (void)hipEventDestroy(b->ev_mark);
+    if (b->side) (void)hipStreamDestroy(b->side);
+    return cleanup(fail(MHM_EDEVICE, "stream/event creation failed"));
+  }
   HIP_TRY(hipMemset(k.iters, 0, M * sizeof(int)));
   HIP_TRY(hipMemset(k.scal, 0, M * NSCAL * sizeof(double)));
   *out = b;
@@ -3258,6 +3281,9 @@ int mhm_batch_destroy(mhm_batch *b) {
   if (!b) return MHM_OK;
   DeviceGuard guard(b->layout->device);
   (void)hipDeviceSynchronize();
+  if (b->ev_side) (void)hipEventDestroy(b->ev_side);
+  if (b->ev_mark) (void)hipEventDestroy(b->ev_mark);
+  if (b->side) (void)hipStreamDestroy(b->side);
   for (void *p : b->owned) (void)hipFree(p);
   delete b;
   return MHM_OK;
@@ -3538,7 +3564,7 @@ static int launch_residual(mhm_batch *b, int ncols, const double *rhs, const dou
 // binary64 refinement; rhs == NULL selects the standard columns [f, B, A Z].
 template <typename T>
 static int run_solve(mhm_batch *b, const double *rhs, double *X, int ncols, int max_refine,
-                     double rtol, hipStream_t st) {
+                     double rtol, hipStream_t st, const std::function<int()> *after_first_pass = nullptr) {
   const LayoutK &L = b->layout->k;
   const double fwd = b->precision == MHM_MIXED ? 1.0e-12 : 0.0;
   hipLaunchKernelGGL(k_fill_int, dim3((unsigned)(((size_t)b->M * ncols + 255) / 256)), dim3(256), 0, st,
@@ -3570,6 +3596,8 @@ static int run_solve(mhm_batch *b, const double *rhs, double *X, int ncols, int 
         return rc;
       }
     }
+    if (it == 1 && after_first_pass)
+      if (int rc = (*after_first_pass)()) return rc;
     if (it == 0 && uw_inline) {
       hipLaunchKernelGGL((k_load_uw<T>), grid2(L.n, 256, b->M), dim3(256), 0, st, L, b->bk, (T *)b->y, ncols);
       LAUNCH_CHECK("k_load_uw");
@@ -3605,15 +3633,37 @@ int mhm_batch_condense(mhm_batch *b, int32_t max_refine, double rtol, double *bl
   const LayoutK &L = b->layout->k;
   DeviceGuard guard(b->layout->device);
   hipStream_t st = (hipStream_t)stream;
-  int rc = b->precision == MHM_FP64 ? run_solve<double>(b, nullptr, b->xstd, b->R, max_refine, rtol, st)
-                                    : run_solve<float>(b, nullptr, b->xstd, b->R, max_refine, rtol, st);
-  if (rc) return rc;
-  TimerSpan span(MHM_TIMER_CONDENSE, st);
   const size_t cshm = condense_lds(L.m, L.k);
   if (cshm > 65536)
     HIP_TRY(hipFuncSetAttribute((const void *)k_condense, hipFuncAttributeMaxDynamicSharedMemorySize, (int)cshm));
+  // After the first refinement pass almost every macro has converged: those
+  // are condensed on the side stream while the few others finish refining on
+  // the caller's stream, then the caller's stream condenses the rest.
+  bool early = false;
+  const std::function<int()> early_condense = [&]() -> int {
+    hipLaunchKernelGGL(k_mark_done, dim3((b->M + 255) / 256), dim3(256), 0, st, (const int *)b->colstate, b->R,
+                       b->M, b->done);
+    LAUNCH_CHECK("k_mark_done");
+    HIP_TRY(hipEventRecord(b->ev_mark, st));
+    HIP_TRY(hipStreamWaitEvent(b->side, b->ev_mark, 0));
+    hipLaunchKernelGGL(k_condense, dim3(b->M), dim3(256), cshm, b->side, L, b->bk, (const double *)b->xstd, blocks,
+                       loads, (const int *)b->done, 1);
+    LAUNCH_CHECK("k_condense");
+    HIP_TRY(hipEventRecord(b->ev_side, b->side));
+    early = true;
+    return MHM_OK;
+  };
+  int rc = b->precision == MHM_FP64
+               ? run_solve<double>(b, nullptr, b->xstd, b->R, max_refine, rtol, st, &early_condense)
+               : run_solve<float>(b, nullptr, b->xstd, b->R, max_refine, rtol, st, &early_condense);
+  if (rc) {
+    if (early) (void)hipStreamWaitEvent(st, b->ev_side, 0);
+    return rc;
+  }
+  TimerSpan span(MHM_TIMER_CONDENSE, st);
+  if (early) HIP_TRY(hipStreamWaitEvent(st, b->ev_side, 0));
   hipLaunchKernelGGL(k_condense, dim3(b->M), dim3(256), cshm, st, L, b->bk, (const double *)b->xstd, blocks,
-                     loads);
+                     loads, (const int *)b->done, early ? 2 : 0);
   LAUNCH_CHECK("k_condense");
   b->condensed = true;
   return MHM_OK;
