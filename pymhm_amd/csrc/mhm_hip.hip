@@ -2165,7 +2165,7 @@ template <typename T>
 __global__ void __launch_bounds__(256)
     k_rhs_init(LayoutK L, BatchK B, int ncols, const double *__restrict__ rhs, T *__restrict__ y,
                double *__restrict__ beta, int *__restrict__ colstate, const double *__restrict__ coupling_rm,
-               double *__restrict__ bnorm) {
+               double *__restrict__ bnorm, int uw_col) {
   extern __shared__ double part[];  // nchunk x ncols
   __shared__ int bad;
   const int mac = blockIdx.x, tid = threadIdx.x;
@@ -2208,6 +2208,16 @@ __global__ void __launch_bounds__(256)
       }
       part[k * ncols + c] = s2;
     }
+  if (uw_col >= 0) {
+    // D e_p and cs D c, the right-hand sides of u and w, in columns uw_col, uw_col + 1
+    const double csc = B.scal[(size_t)mac * NSCAL + S_CS];
+    for (int i = tid; i < n; i += 256) {
+      const double d = ds[i];
+      T *row = Y + (size_t)i * B.ldy + uw_col;
+      row[0] = T(i == L.pin ? d : 0.0);
+      row[1] = T(d * csc * constraint_at(L, B, mac, i));
+    }
+  }
   if (!finite) bad = 1;
   __syncthreads();
   if (tid < ncols) {
@@ -3590,7 +3600,7 @@ static int run_solve(mhm_batch *b, const double *rhs, double *X, int ncols, int 
           HIP_TRY(hipFuncSetAttribute((const void *)k_rhs_init<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                       (int)shm));
         hipLaunchKernelGGL((k_rhs_init<T>), dim3(b->M), dim3(256), shm, st, L, b->bk, ncols, rhs, (T *)b->y,
-                           b->beta, b->colstate, b->layout->coupling_rm, b->bnorm);
+                           b->beta, b->colstate, b->layout->coupling_rm, b->bnorm, uw_inline ? ncols : -1);
         LAUNCH_CHECK("k_rhs_init");
       } else if (int rc = launch_residual<T>(b, ncols, rhs, X, rtol, fwd, it, 0, st)) {
         return rc;
@@ -3599,8 +3609,6 @@ static int run_solve(mhm_batch *b, const double *rhs, double *X, int ncols, int 
     if (it == 1 && after_first_pass)
       if (int rc = (*after_first_pass)()) return rc;
     if (it == 0 && uw_inline) {
-      hipLaunchKernelGGL((k_load_uw<T>), grid2(L.n, 256, b->M), dim3(256), 0, st, L, b->bk, (T *)b->y, ncols);
-      LAUNCH_CHECK("k_load_uw");
       {
         // every column is pending at it == 0, so no column state is needed
         TimerSpan span(MHM_TIMER_SOLVE, st);
