@@ -2024,7 +2024,9 @@ __global__ void __launch_bounds__(64 * RES_WAVES)
   for (int i0 = 0; i0 < n; i0 += RES_ROWS) {
     const bool more = i0 + RES_ROWS < n;
     if (more) chunk_issue(i0 + RES_ROWS);
-    for (int q = wave; q < RES_ROWS; q += RES_WAVES) {
+    // only the lanes of real columns run the rows (LDS returns data to active
+    // lanes only)
+    for (int q = wave; on && q < RES_ROWS; q += RES_WAVES) {
       const int i = i0 + q;
       if (i >= n) break;
       double hi = stg[q * ldr + cl], lo = FAST ? fabs(hi) : 0.0;  // FAST: lo = |b| + |S||x|
@@ -2151,8 +2153,10 @@ __global__ void __launch_bounds__(64 * RES_WAVES)
 }
 
 // Per-column sums over the rows of the refinement kernels go in fixed chunks
-// of SADDLE_ROWS rows.
+// of SADDLE_ROWS rows; their blocks have SADDLE_THREADS threads (a macro still
+// refining alone spreads its chunks over SADDLE_THREADS / ncols row groups).
 #define SADDLE_ROWS 32
+#define SADDLE_THREADS 1024
 
 // First pass of a refined solve (x = 0, so r = b): y = D b, ||b||^2 per column
 // (kept for the later passes), the constraint entry of b (the bordered
@@ -2162,7 +2166,7 @@ __global__ void __launch_bounds__(64 * RES_WAVES)
 // SADDLE_ROWS rows in row order, then in chunk order, so a column's bits do
 // not depend on the other columns of the call.
 template <typename T>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(SADDLE_THREADS)
     k_rhs_init(LayoutK L, BatchK B, int ncols, const double *__restrict__ rhs, T *__restrict__ y,
                double *__restrict__ beta, int *__restrict__ colstate, const double *__restrict__ coupling_rm,
                double *__restrict__ bnorm, int uw_col) {
@@ -2170,7 +2174,7 @@ __global__ void __launch_bounds__(256)
   __shared__ int bad;
   const int mac = blockIdx.x, tid = threadIdx.x;
   const int n = L.n, m = L.m, N = B.N;
-  const int lanes = 256 / ncols;
+  const int lanes = SADDLE_THREADS / ncols;
   const int g = tid / ncols, c = tid - g * ncols;
   const int nchunk = (n + SADDLE_ROWS - 1) / SADDLE_ROWS;
   if (tid == 0) bad = 0;
@@ -2211,7 +2215,7 @@ __global__ void __launch_bounds__(256)
   if (uw_col >= 0) {
     // D e_p and cs D c, the right-hand sides of u and w, in columns uw_col, uw_col + 1
     const double csc = B.scal[(size_t)mac * NSCAL + S_CS];
-    for (int i = tid; i < n; i += 256) {
+    for (int i = tid; i < n; i += SADDLE_THREADS) {
       const double d = ds[i];
       T *row = Y + (size_t)i * B.ldy + uw_col;
       row[0] = T(i == L.pin ? d : 0.0);
@@ -2239,7 +2243,7 @@ __global__ void __launch_bounds__(256)
 // alone and in a batch).
 
 template <typename T>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(SADDLE_THREADS)
     k_saddle_apply(LayoutK L, BatchK B, int ncols, const T *__restrict__ y, const double *__restrict__ beta,
                    double *__restrict__ X, int it, const int *__restrict__ colstate,
                    double *__restrict__ colnorm) {
@@ -2247,7 +2251,7 @@ __global__ void __launch_bounds__(256)
   __shared__ double sxp[64], smu[64];
   const int mac = blockIdx.x;
   const int tid = threadIdx.x;
-  const int lanes = 256 / ncols;
+  const int lanes = SADDLE_THREADS / ncols;
   const int lane = tid / ncols, c = tid - lane * ncols;
   const bool on = lane < lanes;
   const int n = L.n, N = B.N;
@@ -3599,7 +3603,7 @@ static int run_solve(mhm_batch *b, const double *rhs, double *X, int ncols, int 
         if (shm > 65536)
           HIP_TRY(hipFuncSetAttribute((const void *)k_rhs_init<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                       (int)shm));
-        hipLaunchKernelGGL((k_rhs_init<T>), dim3(b->M), dim3(256), shm, st, L, b->bk, ncols, rhs, (T *)b->y,
+        hipLaunchKernelGGL((k_rhs_init<T>), dim3(b->M), dim3(SADDLE_THREADS), shm, st, L, b->bk, ncols, rhs, (T *)b->y,
                            b->beta, b->colstate, b->layout->coupling_rm, b->bnorm, uw_inline ? ncols : -1);
         LAUNCH_CHECK("k_rhs_init");
       } else if (int rc = launch_residual<T>(b, ncols, rhs, X, rtol, fwd, it, 0, st)) {
@@ -3624,7 +3628,7 @@ static int run_solve(mhm_batch *b, const double *rhs, double *X, int ncols, int 
     if (saddle_lds(L.n, ncols) > 65536)
       HIP_TRY(hipFuncSetAttribute((const void *)k_saddle_apply<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                   (int)saddle_lds(L.n, ncols)));
-    hipLaunchKernelGGL((k_saddle_apply<T>), dim3(b->M), dim3(256), saddle_lds(L.n, ncols), st, L, b->bk, ncols,
+    hipLaunchKernelGGL((k_saddle_apply<T>), dim3(b->M), dim3(SADDLE_THREADS), saddle_lds(L.n, ncols), st, L, b->bk, ncols,
                        (const T *)b->y,
                        (const double *)b->beta, X, it, (const int *)b->colstate, b->colnorm);
     LAUNCH_CHECK("k_saddle_apply");
