@@ -1606,7 +1606,8 @@ __global__ void __launch_bounds__(256) k_diag_inverse(long nblocks, double *__re
 
 template <int MAXK, int NCT, int DIR>
 __device__ __forceinline__ void mfma_sweep(int n, int b, int ldy, int ncols, const double *__restrict__ Lb,
-                                           const double *__restrict__ Di, double *__restrict__ Y, int lane) {
+                                           const double *__restrict__ Di, double *__restrict__ Y, int lane,
+                                           int cbase = 0) {
   typedef Mfma16<double>::v4 V4;
   const int W = b + 1, nblk = (n + 15) / 16;
   const int a16 = lane & 15, g = lane >> 4;
@@ -1644,7 +1645,7 @@ __device__ __forceinline__ void mfma_sweep(int n, int b, int ldy, int ncols, con
     double rhs[NCT][4], dv[4];
 #pragma unroll
     for (int ct = 0; ct < NCT; ++ct) {
-      const int col = 16 * ct + a16;
+      const int col = cbase + 16 * ct + a16;
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int row = i0 + 4 * r + g;
@@ -1678,7 +1679,7 @@ __device__ __forceinline__ void mfma_sweep(int n, int b, int ldy, int ncols, con
     }
 #pragma unroll
     for (int ct = 0; ct < NCT; ++ct) {
-      const int col = 16 * ct + a16;
+      const int col = cbase + 16 * ct + a16;
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int row = i0 + 4 * r + g;
@@ -1717,6 +1718,29 @@ __global__ void __launch_bounds__(64 * SOLVE_MFMA_WAVES) __attribute__((amdgpu_w
   double *Y = y + (size_t)mac * n * ldy;
   mfma_sweep<MAXK, NCT, 1>(n, b, ldy, ncols, band + mac * nW, Di, Y, lane);
   mfma_sweep<MAXK, NCT, -1>(n, b, ldy, ncols, rband + mac * nW, Di, Y, lane);
+}
+
+// The same solve with one wavefront per 16-column tile of a macro (a block per
+// macro): the tiles share only the band, so a macro's latency is that of one
+// tile's products (each wavefront reads the band itself: for refinement passes
+// that leave few macros pending).
+template <int MAXK>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+    k_solve_mfma_tiles(int n, int b, int ldy, int ncols, const double *__restrict__ band,
+                       const double *__restrict__ rband, const double *__restrict__ dinv, double *__restrict__ y,
+                       const int *__restrict__ colstate, int *__restrict__ iters) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int mac = blockIdx.x;
+  if (colstate) {
+    const bool pending = lane < ncols && colstate[(size_t)mac * ncols + lane] == 0;
+    if (!__any(pending)) return;
+  }
+  if (iters && threadIdx.x == 0) iters[mac] += 1;
+  const size_t nW = (size_t)n * (b + 1);
+  const double *Di = dinv + (size_t)mac * ((n + 15) / 16) * 256;
+  double *Y = y + (size_t)mac * n * ldy;
+  mfma_sweep<MAXK, 1, 1>(n, b, ldy, ncols, band + mac * nW, Di, Y, lane, 16 * wave);
+  mfma_sweep<MAXK, 1, -1>(n, b, ldy, ncols, rband + mac * nW, Di, Y, lane, 16 * wave);
 }
 
 // Register window of the matrix-core solve: MAXK k-steps x NCT column tiles
@@ -2962,6 +2986,24 @@ static int launch_solve(const mhm_batch *b, int ncols, const int *colstate, int 
     int maxk, nct;
     solve_mfma_shape(L.b, ncols, &maxk, &nct);
     if (b->dinv && b->rband && maxk) {
+      // refinement passes (a column state) leave few macros pending: one
+      // wavefront per tile shortens each macro's solve; the first solve of
+      // every macro keeps one wavefront per macro (the band read once)
+      int kt = 0;
+      if (colstate)
+        for (int k : {8, 12, 17, 24, 28})
+          if (kt == 0 && (L.b + 3) / 4 <= k) kt = k;
+      const int nct = (ncols + 15) / 16;
+#define MHM_TILES(K)                                                                                         \
+  if (kt == K) {                                                                                             \
+    hipLaunchKernelGGL((k_solve_mfma_tiles<K>), dim3(b->M), dim3(64 * nct), 0, st, L.n, L.b, b->ldy, ncols,   \
+                       (const double *)b->band, (const double *)b->rband, (const double *)b->dinv,         \
+                       (double *)b->y, colstate, iters);                                                   \
+    LAUNCH_CHECK("k_solve_mfma_tiles");                                                                      \
+    return MHM_OK;                                                                                           \
+  }
+      MHM_TILES(8) MHM_TILES(12) MHM_TILES(17) MHM_TILES(24) MHM_TILES(28)
+#undef MHM_TILES
       const dim3 grid((unsigned)((b->M + SOLVE_MFMA_WAVES - 1) / SOLVE_MFMA_WAVES));
 #define MHM_MFMA(K, C)                                                                                      \
   if (maxk == K && nct == C) {                                                                              \
