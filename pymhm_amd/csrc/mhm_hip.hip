@@ -2275,7 +2275,7 @@ __global__ void __launch_bounds__(SADDLE_THREADS)
   __shared__ double sxp[64], smu[64];
   const int mac = blockIdx.x;
   const int tid = threadIdx.x;
-  const int lanes = SADDLE_THREADS / ncols;
+  const int lanes = (int)blockDim.x / ncols;  // SADDLE_THREADS, or 256 for passes with few macros left
   const int lane = tid / ncols, c = tid - lane * ncols;
   const bool on = lane < lanes;
   const int n = L.n, N = B.N;
@@ -3670,7 +3670,9 @@ static int run_solve(mhm_batch *b, const double *rhs, double *X, int ncols, int 
     if (saddle_lds(L.n, ncols) > 65536)
       HIP_TRY(hipFuncSetAttribute((const void *)k_saddle_apply<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                   (int)saddle_lds(L.n, ncols)));
-    hipLaunchKernelGGL((k_saddle_apply<T>), dim3(b->M), dim3(SADDLE_THREADS), saddle_lds(L.n, ncols), st, L, b->bk, ncols,
+    // binary64 refinement passes leave few macros: small blocks dispatch faster
+    const int sthreads = (it > 0 && sizeof(T) == 8) ? 256 : SADDLE_THREADS;
+    hipLaunchKernelGGL((k_saddle_apply<T>), dim3(b->M), dim3(sthreads), saddle_lds(L.n, ncols), st, L, b->bk, ncols,
                        (const T *)b->y,
                        (const double *)b->beta, X, it, (const int *)b->colstate, b->colnorm);
     LAUNCH_CHECK("k_saddle_apply");
