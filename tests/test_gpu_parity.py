@@ -139,6 +139,29 @@ def test_config3_wide_band_stage(config3_wide, precision):
     assert rel(responses(out), responses(ref)) < TOL
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_refinement_below_the_rounding_floor(config2_small, precision):
+    """rtol below the rounding floor keeps every column pending through every
+    pass: the binary64 residual stage hands the columns near the floor to the
+    double-double stage, every pass solves every macro (in binary64 one
+    wavefront per column tile), no macro is condensed early.  The results
+    still meet the oracle bar, and the missed criterion is reported."""
+    from pymhm_amd import _lib
+
+    case, sel, ref = config2_small
+    print(f"config2 {precision}, rtol 1e-17", flush=True)
+    with case.problem.batch(precision) as b:
+        b.assemble(case.kq, None, case.f_const)
+        b.factor()
+        blocks, loads = b.condense(max_refine=3, rtol=1e-17, check=False)
+        solves, status = b.iterations(), b.status()
+        blocks, loads = blocks.cpu().numpy(), loads.cpu().numpy()
+    assert (solves == 4).all()  # the first solve and three refinement passes for every macro
+    assert (status & _lib.STATUS_RESIDUAL).any()
+    assert rel(blocks[sel], ref["blocks"]) < TOL
+    assert rel(loads[sel], ref["loads"]) < TOL
+
+
 def test_results_independent_of_batch_composition(config2_small):
     """A macro's result is bitwise the same alone and inside a larger batch."""
     case, _, _ = config2_small
