@@ -156,7 +156,7 @@ def per_macro_traffic(prob, precision):
     W = b + 1
     ts = 8 if precision == "fp64" else 4
     R = 1 + lay.m + lay.k
-    blocks = -(-n // 16) * 256 * ts if precision == "fp64" else 0
+    blocks = -(-n // 16) * 136 * ts if precision == "fp64" else 0  # packed lower triangles
     return dict(
         factor_bytes=lay.nnz * ts + 2 * n * W * ts + blocks, factor_flops=n * b * (b + 1),
         solve_bytes=2 * n * W * ts + 2 * blocks + 4 * n * (R + 2) * ts, solve_flops=4 * n * b * R,

@@ -785,8 +785,8 @@ __global__ void __launch_bounds__(NT)
 //      L(j0 + q, j0 + t), rows PR .. ldq - 1 zero;
 //  (b) the panel's columns go to HBM as the column band (contiguous) and the
 //      row band (rband[R*W + r] = L(R, R-r), runs of NB per row), and with
-//      dblk set its 16 x 16 diagonal block, row-major, to
-//      dblk[(mac * nblocks + j0 / 16) * 256];
+//      dblk set its 16 x 16 diagonal block, packed (tri16), to
+//      dblk[(mac * nblocks + j0 / 16) * DBLK];
 //  (c) the rank-NB update of the b trailing columns on the matrix cores:
 //      16 x 16 tiles of the lower triangle, round-robin over the wavefronts,
 //      C = C - L_I L_J^T in four 16x16x4 MFMA steps with the C fragment read
@@ -796,6 +796,10 @@ __global__ void __launch_bounds__(NT)
 //      take the panel's slots.
 // Three block barriers per panel instead of two per column.
 #define FPANEL 16
+// A 16 x 16 lower-triangular block, packed by rows: entry (rho, sigma), sigma
+// <= rho, at rho (rho + 1) / 2 + sigma.
+#define DBLK 136
+__host__ __device__ constexpr int tri16(int rho, int sigma) { return rho * (rho + 1) / 2 + sigma; }
 
 template <typename T>
 struct Mfma16;
@@ -948,8 +952,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
         const int R = j0 + q, r = q - t;
         if (t < np && r >= 0 && r <= b && R < n) Rg[(size_t)R * W + r] = LpT[t * ldq + q];
       }
-    if (dblk)  // the diagonal block, row-major (identity-padded past row n - 1)
-      dblk[((size_t)mac * ((n + NB - 1) / NB) + j0 / NB) * (NB * NB) + tid] = LpT[(tid & 15) * ldq + (tid >> 4)];
+    if (dblk && (tid & 15) <= (tid >> 4))  // the diagonal block, packed (identity-padded past row n - 1)
+      dblk[((size_t)mac * ((n + NB - 1) / NB) + j0 / NB) * DBLK + tri16(tid >> 4, tid & 15)] =
+          LpT[(tid & 15) * ldq + (tid >> 4)];
     // (c) trailing update on the matrix cores: tile k = (I, J), J <= I < nt,
     // columns qc0 = NB + 16 J, rows qr0 = NB + 16 I, for wavefront k mod 4
     for (int k = wave; k < ntile; k += 4) {
@@ -1567,9 +1572,9 @@ __global__ void __launch_bounds__(64 * SOLVE_WAVES) __attribute__((amdgpu_waves_
 
 
 // Inverses of the 16 x 16 diagonal blocks of L in place: dinv[(mac * nblk +
-// q) * 256] holds the block (rows and columns 16 q .. 16 q + 15, row-major,
-// identity past row n - 1, written by the LDS-window factor) and receives its
-// inverse.  A workgroup takes 16 blocks: thread (block, rho) stages row rho
+// q) * DBLK] holds the block (rows and columns 16 q .. 16 q + 15, packed lower
+// triangle, identity past row n - 1, written by the LDS-window factor) and
+// receives its inverse, packed the same way.  A workgroup takes 16 blocks: thread (block, rho) stages row rho
 // of its block and 1 / L(rho, rho) in LDS, then lane sigma of the block's 16
 // forms column sigma of the inverse by column-oriented forward substitution.
 __global__ void __launch_bounds__(256) k_diag_inverse(long nblocks, double *__restrict__ dinv) {
@@ -1577,12 +1582,12 @@ __global__ void __launch_bounds__(256) k_diag_inverse(long nblocks, double *__re
   const int lb = threadIdx.x >> 4, sigma = threadIdx.x & 15;
   const long gb = (long)blockIdx.x * 16 + lb;
   const bool on = gb < nblocks;
-  double *blk = dinv + (size_t)(on ? gb : 0) * 256;
+  double *blk = dinv + (size_t)(on ? gb : 0) * DBLK;
   {
     const int rho = sigma;  // this thread stages row rho
     double row[16];
 #pragma unroll
-    for (int t = 0; t < 16; ++t) row[t] = on ? blk[rho * 16 + t] : (t == rho ? 1.0 : 0.0);
+    for (int t = 0; t < 16; ++t) row[t] = on && t <= rho ? blk[tri16(rho, t)] : (t == rho ? 1.0 : 0.0);
 #pragma unroll
     for (int t = 0; t < 16; ++t)
       if (t <= rho) sL[lb][t][rho - t] = row[t];
@@ -1598,7 +1603,7 @@ __global__ void __launch_bounds__(256) k_diag_inverse(long nblocks, double *__re
     if ((tau & 3) == 0) MEM_FENCE();  // four columns of the block in registers at a time
     const double *c = sL[lb][tau];
     const double z = acc[tau] * c[16];
-    blk[tau * 16 + sigma] = z;
+    if (tau >= sigma) blk[tri16(tau, sigma)] = z;  // z = 0 above the diagonal
 #pragma unroll
     for (int rho = tau + 1; rho < 16; ++rho) acc[rho] -= c[rho - tau] * z;
   }
@@ -1652,9 +1657,12 @@ __device__ __forceinline__ void mfma_sweep(int n, int b, int ldy, int ncols, con
         rhs[ct][r] = *((row < n && col < ncols) ? Y + (size_t)row * ldy + col : kZeroF64);
       }
     }
-    const double *dq = Di + (size_t)q * 256;
+    const double *dq = Di + (size_t)q * DBLK;
 #pragma unroll
-    for (int k = 0; k < 4; ++k) dv[k] = DIR > 0 ? dq[a16 * 16 + 4 * k + g] : dq[(4 * k + g) * 16 + a16];
+    for (int k = 0; k < 4; ++k) {  // Dinv[a16][4k+g] forward, Dinv[4k+g][a16] backward (zero above the diagonal)
+      const int kk = 4 * k + g;
+      dv[k] = DIR > 0 ? *(kk <= a16 ? dq + tri16(a16, kk) : kZeroF64) : *(a16 <= kk ? dq + tri16(kk, a16) : kZeroF64);
+    }
     // these loads issue before the next block's operands (the memory counter
     // drains in order), so waiting for them leaves those in flight
     MEM_FENCE();
@@ -1714,7 +1722,7 @@ __global__ void __launch_bounds__(64 * SOLVE_MFMA_WAVES) __attribute__((amdgpu_w
   }
   if (iters && lane == 0) iters[mac] += 1;
   const size_t nW = (size_t)n * (b + 1);
-  const double *Di = dinv + (size_t)mac * ((n + 15) / 16) * 256;
+  const double *Di = dinv + (size_t)mac * ((n + 15) / 16) * DBLK;
   double *Y = y + (size_t)mac * n * ldy;
   mfma_sweep<MAXK, NCT, 1>(n, b, ldy, ncols, band + mac * nW, Di, Y, lane);
   mfma_sweep<MAXK, NCT, -1>(n, b, ldy, ncols, rband + mac * nW, Di, Y, lane);
@@ -1737,7 +1745,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
   }
   if (iters && threadIdx.x == 0) iters[mac] += 1;
   const size_t nW = (size_t)n * (b + 1);
-  const double *Di = dinv + (size_t)mac * ((n + 15) / 16) * 256;
+  const double *Di = dinv + (size_t)mac * ((n + 15) / 16) * DBLK;
   double *Y = y + (size_t)mac * n * ldy;
   mfma_sweep<MAXK, 1, 1>(n, b, ldy, ncols, band + mac * nW, Di, Y, lane, 16 * wave);
   mfma_sweep<MAXK, 1, -1>(n, b, ldy, ncols, rband + mac * nW, Di, Y, lane, 16 * wave);
@@ -3314,7 +3322,7 @@ int mhm_batch_create(const mhm_layout *L, int32_t macros, int32_t precision, int
       if ((rc = dev_alloc(b->owned, &p, M * lk.nnz * tsize))) return cleanup(rc);
       b->sval = p;
       if (precision == MHM_FP64 && b->rband &&
-          (rc = dev_alloc(b->owned, &b->dinv, M * (size_t)((n + 15) / 16) * 256)))
+          (rc = dev_alloc(b->owned, &b->dinv, M * (size_t)((n + 15) / 16) * DBLK)))
         return cleanup(rc);
     }
   }
