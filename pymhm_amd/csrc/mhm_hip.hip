@@ -1871,8 +1871,6 @@ struct ResShared {
   const double *colptr[64];
   int colstride[64];
   double colscale[64], mu[64];
-  int pend[64];
-  double zero;
 };
 
 // LDS of the residual past the x ring: the chunk's right-hand side rows, row
@@ -1906,7 +1904,6 @@ __global__ void __launch_bounds__(64 * RES_WAVES)
   // double-double stage takes the pending columns the binary64 stage left
   int *done = rdone + (size_t)mac * ncols;
   const bool mine = tid < ncols && state[tid] == 0 && (FAST || check_only || done[tid] == 0);
-  if (tid < 64) sh.pend[tid] = mine ? 1 : 0;
   const double *x = X + (size_t)mac * N * ncols;
   if (tid < 64) {
     const int c = tid;
@@ -1934,7 +1931,6 @@ __global__ void __launch_bounds__(64 * RES_WAVES)
     sh.colstride[c] = stride;
     sh.colscale[c] = scale;
     sh.mu[c] = (c < ncols && L.k) ? x[(size_t)n * ncols + c] : 0.0;
-    if (c == 0) sh.zero = 0.0;
   }
   if (!__syncthreads_or(mine)) return;
   constexpr bool use_ring = RING;  // an instance per case: LDS and global reads stay apart
