@@ -2189,66 +2189,112 @@ __global__ void __launch_bounds__(64 * RES_WAVES)
 // First pass of a refined solve (x = 0, so r = b): y = D b, ||b||^2 per column
 // (kept for the later passes), the constraint entry of b (the bordered
 // correction's right-hand side) and the column state, without the residual's
-// CSR walk and x ring.  Block per macro, thread (row group, column) with
-// row-contiguous accesses; the per-column sums go in fixed chunks of
-// SADDLE_ROWS rows in row order, then in chunk order, so a column's bits do
-// not depend on the other columns of the call.
+// CSR walk and x ring.  Block per macro; thread (row, column) over the written
+// width (the columns, then u's and w's when uw_col >= 0), so a wavefront's
+// stores cover whole rows of y.  Rows go in groups of gchunks chunks of
+// SADDLE_ROWS rows, each thread holding up to RHS_PASSES rows of a group in
+// flight (the next group's loads are issued before this group's sums).  The
+// per-column sums go in fixed chunks of SADDLE_ROWS rows in row order, then in
+// chunk order, so a column's bits do not depend on the other columns of the
+// call.
+#define RHS_PASSES 5
+#define RHS_TILE 4608  // doubles of a row group's values kept in LDS for the sums
+
 template <typename T>
-__global__ void __launch_bounds__(SADDLE_THREADS)
+__global__ void __launch_bounds__(SADDLE_THREADS) __attribute__((amdgpu_waves_per_eu(8)))
     k_rhs_init(LayoutK L, BatchK B, int ncols, const double *__restrict__ rhs, T *__restrict__ y,
                double *__restrict__ beta, int *__restrict__ colstate, const double *__restrict__ coupling_rm,
-               double *__restrict__ bnorm, int uw_col) {
-  extern __shared__ double part[];  // nchunk x ncols
+               double *__restrict__ bnorm, int uw_col, int gchunks) {
+  extern __shared__ double part[];  // nchunk x ncols chunk sums, then the group's values
   __shared__ int bad;
   const int mac = blockIdx.x, tid = threadIdx.x;
   const int n = L.n, m = L.m, N = B.N;
-  const int lanes = SADDLE_THREADS / ncols;
-  const int g = tid / ncols, c = tid - g * ncols;
+  const int width = uw_col >= 0 ? ncols + 2 : ncols;
+  const int rp = SADDLE_THREADS / width;
+  const int r = tid / width, c = tid - r * width;
+  const bool act = r < rp;
   const int nchunk = (n + SADDLE_ROWS - 1) / SADDLE_ROWS;
+  const int grows = gchunks * SADDLE_ROWS;
+  const int ngroup = (n + grows - 1) / grows;
+  double *tile = part + (size_t)nchunk * ncols;
   if (tid == 0) bad = 0;
-  __syncthreads();
   const double *ds = B.dscale + (size_t)mac * n;
   T *Y = y + (size_t)mac * n * B.ldy;
-  const double *src;
-  int stride = 1;
+  // column c reads src[i * stride] * scale; u's column is e_pin, w's is the
+  // constraint times cs (its product taken as (d cs) c)
+  const double *src = kZeroF64;
+  int stride = 0;
   double scale = 1.0;
-  if (rhs) {
-    src = rhs + (size_t)mac * N * ncols + c;
-    stride = ncols;
-  } else if (c == 0) {
-    src = B.load + (size_t)mac * n;
-  } else if (c > m) {
-    src = B.az + (size_t)mac * n;
-  } else if (B.bcoup) {
-    src = B.bcoup + ((size_t)mac * m + (c - 1)) * n;
-  } else {
-    src = coupling_rm + (B.colmap ? B.colmap[(size_t)mac * m + c - 1] : c - 1);
-    stride = m;
-    if (B.colscale) scale = B.colscale[(size_t)mac * m + c - 1];
+  const bool pinc = c == ncols, conc = c == ncols + 1;
+  if (c < ncols) {
+    stride = 1;
+    if (rhs) {
+      src = rhs + (size_t)mac * N * ncols + c;
+      stride = ncols;
+    } else if (c == 0) {
+      src = B.load + (size_t)mac * n;
+    } else if (c > m) {
+      src = B.az + (size_t)mac * n;
+    } else if (B.bcoup) {
+      src = B.bcoup + ((size_t)mac * m + (c - 1)) * n;
+    } else {
+      src = coupling_rm + (B.colmap ? B.colmap[(size_t)mac * m + c - 1] : c - 1);
+      stride = m;
+      if (B.colscale) scale = B.colscale[(size_t)mac * m + c - 1];
+    }
+  } else if (conc) {
+    src = B.cons ? B.cons + (size_t)mac * n : L.constraint;
+    stride = 1;
+    scale = B.scal[(size_t)mac * NSCAL + S_CS];
   }
+  const int ycol = c < ncols ? c : uw_col + (c - ncols);
+  double v[RHS_PASSES], d[RHS_PASSES];
+  // unconditional loads of a clamped row (a select of the address becomes a
+  // branch with a wait per pass); values of rows outside the group are unused
+  auto issue = [&](int g) {
+#pragma unroll
+    for (int p = 0; p < RHS_PASSES; ++p) {
+      const int i = min(g * grows + r + p * rp, n - 1);
+      v[p] = global_ptr(src)[(size_t)i * stride];
+      d[p] = global_ptr(ds)[i];
+    }
+  };
   bool finite = true;
-  if (g < lanes)
-    for (int k = g; k < nchunk; k += lanes) {
-      double s2 = 0.0;
-      const int i1 = min(n, (k + 1) * SADDLE_ROWS);
-#pragma unroll 4
-      for (int i = k * SADDLE_ROWS; i < i1; ++i) {
-        const double v = src[(size_t)i * stride] * scale;
-        finite &= isfinite(v);
-        s2 += v * v;
-        Y[(size_t)i * B.ldy + c] = T(ds[i] * v);
+  issue(0);
+  for (int g = 0; g < ngroup; ++g) {
+    // every load of the group has landed: without this the branchy consume
+    // loop waits for vmcnt(0) per pass, i.e. for the previous pass's stores
+    __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0), expcnt and lgkmcnt untouched
+#pragma unroll
+    for (int p = 0; p < RHS_PASSES; ++p) {
+      const int ir = r + p * rp, i = g * grows + ir;
+      if (act && ir < grows && i < n) {
+        if (c < ncols) {
+          const double x = v[p] * scale;
+          finite &= isfinite(x);
+          tile[ir * ncols + c] = x;
+          Y[(size_t)i * B.ldy + ycol] = T(d[p] * x);
+        } else {
+          Y[(size_t)i * B.ldy + ycol] = T(conc ? d[p] * scale * v[p] : (pinc && i == L.pin ? d[p] : 0.0));
+        }
       }
-      part[k * ncols + c] = s2;
     }
-  if (uw_col >= 0) {
-    // D e_p and cs D c, the right-hand sides of u and w, in columns uw_col, uw_col + 1
-    const double csc = B.scal[(size_t)mac * NSCAL + S_CS];
-    for (int i = tid; i < n; i += SADDLE_THREADS) {
-      const double d = ds[i];
-      T *row = Y + (size_t)i * B.ldy + uw_col;
-      row[0] = T(i == L.pin ? d : 0.0);
-      row[1] = T(d * csc * constraint_at(L, B, mac, i));
+    __syncthreads();
+    if (g + 1 < ngroup) issue(g + 1);
+    if (tid < gchunks * ncols) {
+      const int k = tid / ncols, cc = tid - k * ncols;
+      const int i0 = g * grows + k * SADDLE_ROWS;
+      if (i0 < n) {
+        const int rows = min(SADDLE_ROWS, n - i0);
+        double s2 = 0.0;
+        for (int q = 0; q < rows; ++q) {
+          const double x = tile[(k * SADDLE_ROWS + q) * ncols + cc];
+          s2 += x * x;
+        }
+        part[(size_t)(i0 / SADDLE_ROWS) * ncols + cc] = s2;
+      }
     }
+    __syncthreads();
   }
   if (!finite) bad = 1;
   __syncthreads();
@@ -2260,6 +2306,15 @@ __global__ void __launch_bounds__(SADDLE_THREADS)
     beta[(size_t)mac * ncols + tid] = rhs ? rhs[((size_t)mac * N + n) * ncols + tid] : 0.0;
   }
   if (tid == 0 && bad) atomicOr(&B.status[mac], MHM_STATUS_NONFINITE);
+}
+
+// Chunks per k_rhs_init row group: the most (up to 4) whose values fit
+// RHS_TILE and whose rows take at most RHS_PASSES passes of the block.
+static int rhs_group_chunks(int ncols, int width) {
+  const int rp = SADDLE_THREADS / width;
+  int g = 4;
+  while (g > 1 && (g * SADDLE_ROWS * ncols > RHS_TILE || (g * SADDLE_ROWS + rp - 1) / rp > RHS_PASSES)) --g;
+  return g;
 }
 
 // x = D y (+ bordered correction when k == 1); it == 0 sets X, later
@@ -3258,7 +3313,8 @@ int mhm_batch_create(const mhm_layout *L, int32_t macros, int32_t precision, int
   const int widest = std::max(std::max(1 + lk.m + lk.k, 2), max_columns);
   if (!solve_supported(lk.b, widest, tsize, limit) || !solve_supported(lk.b, 2, tsize, limit))
     return fail(MHM_EINVAL, "solve window exceeds the register or LDS budget");
-  if (saddle_lds(lk.n, widest) > (size_t)limit)
+  if (std::max(saddle_lds(lk.n, widest), saddle_lds(lk.n, widest) / 2 + RHS_TILE * sizeof(double)) >
+      (size_t)limit)
     return fail(MHM_EINVAL, "too many local rows for the refinement's per-column sums");
   if (factor_lds(lk.b, tsize) > (size_t)limit &&
       !(FPANEL + lk.b <= 127 && factor_panel_lds(lk.b, tsize) <= (size_t)limit) &&
@@ -3645,12 +3701,13 @@ static int run_solve(mhm_batch *b, const double *rhs, double *X, int ncols, int 
     {
       TimerSpan span(MHM_TIMER_RESIDUAL, st);
       if (it == 0) {
-        const size_t shm = saddle_lds(L.n, ncols) / 2;
+        const int gch = rhs_group_chunks(ncols, uw_inline ? ncols + 2 : ncols);
+        const size_t shm = saddle_lds(L.n, ncols) / 2 + (size_t)gch * SADDLE_ROWS * ncols * sizeof(double);
         if (shm > 65536)
           HIP_TRY(hipFuncSetAttribute((const void *)k_rhs_init<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                       (int)shm));
         hipLaunchKernelGGL((k_rhs_init<T>), dim3(b->M), dim3(SADDLE_THREADS), shm, st, L, b->bk, ncols, rhs, (T *)b->y,
-                           b->beta, b->colstate, b->layout->coupling_rm, b->bnorm, uw_inline ? ncols : -1);
+                           b->beta, b->colstate, b->layout->coupling_rm, b->bnorm, uw_inline ? ncols : -1, gch);
         LAUNCH_CHECK("k_rhs_init");
       } else if (int rc = launch_residual<T>(b, ncols, rhs, X, rtol, fwd, it, 0, st)) {
         return rc;
