@@ -2286,11 +2286,18 @@ __global__ void __launch_bounds__(SADDLE_THREADS) __attribute__((amdgpu_waves_pe
       const int i0 = g * grows + k * SADDLE_ROWS;
       if (i0 < n) {
         const int rows = min(SADDLE_ROWS, n - i0);
+        const double *t = tile + k * SADDLE_ROWS * ncols + cc;
         double s2 = 0.0;
-        for (int q = 0; q < rows; ++q) {
-          const double x = tile[(k * SADDLE_ROWS + q) * ncols + cc];
-          s2 += x * x;
+        int q = 0;
+        for (; q + 4 <= rows; q += 4) {  // four reads in flight, summed in row order
+          const double x0 = t[q * ncols], x1 = t[(q + 1) * ncols], x2 = t[(q + 2) * ncols],
+                       x3 = t[(q + 3) * ncols];
+          s2 += x0 * x0;
+          s2 += x1 * x1;
+          s2 += x2 * x2;
+          s2 += x3 * x3;
         }
+        for (; q < rows; ++q) s2 += t[q * ncols] * t[q * ncols];
         part[(size_t)(i0 / SADDLE_ROWS) * ncols + cc] = s2;
       }
     }
@@ -2317,18 +2324,20 @@ static int rhs_group_chunks(int ncols, int width) {
   return g;
 }
 
-// x = D y (+ bordered correction when k == 1); it == 0 sets X, later
-// iterations add the correction and record ||delta||^2, ||x||^2 and the
-// previous ||delta||^2 in colnorm[3 * col].
+// x = D y (+ bordered correction when k == 1); the first pass (REFINE false)
+// sets X, refinement passes add the correction and record ||delta||^2,
+// ||x||^2 and the previous ||delta||^2 in colnorm[3 * col].
 // Per-column sums over the rows go in fixed chunks of SADDLE_ROWS rows, each
 // summed in row order, then the chunk sums in chunk order: the result does not
 // depend on how many columns share the call (a column's bits are the same
 // alone and in a batch).
 
-template <typename T>
-__global__ void __launch_bounds__(SADDLE_THREADS)
+#define SAP_ROWS 3
+
+template <typename T, bool REFINE>
+__global__ void __launch_bounds__(SADDLE_THREADS) __attribute__((amdgpu_waves_per_eu(8)))
     k_saddle_apply(LayoutK L, BatchK B, int ncols, const T *__restrict__ y, const double *__restrict__ beta,
-                   double *__restrict__ X, int it, const int *__restrict__ colstate,
+                   double *__restrict__ X, const int *__restrict__ colstate,
                    double *__restrict__ colnorm) {
   extern __shared__ double part[];  // 2 x nchunk x ncols
   __shared__ double sxp[64], smu[64];
@@ -2339,7 +2348,7 @@ __global__ void __launch_bounds__(SADDLE_THREADS)
   const bool on = lane < lanes;
   const int n = L.n, N = B.N;
   const int nchunk = (n + SADDLE_ROWS - 1) / SADDLE_ROWS;
-  const bool skip = !on || (it > 0 && colstate[(size_t)mac * ncols + c]);
+  const bool skip = !on || (REFINE && colstate[(size_t)mac * ncols + c]);
   double *x = X + (size_t)mac * N * ncols;
   const double *ds = B.dscale + (size_t)mac * n;
   const T *Y = y + (size_t)mac * n * B.ldy;
@@ -2375,18 +2384,40 @@ __global__ void __launch_bounds__(SADDLE_THREADS)
     for (int k = lane; k < nchunk; k += lanes) {
       double dn = 0.0, xn = 0.0;
       const int i1 = min(n, (k + 1) * SADDLE_ROWS);
-#pragma unroll 4
-      for (int i = k * SADDLE_ROWS; i < i1; ++i) {
-        double v = ds[i] * (double)Y[(size_t)i * B.ldy + c];
-        if (L.k) v += sigma * xp * u[i] - mu * w[i];
-        double *xi = x + (size_t)i * ncols + c;
-        if (it > 0) {
-          const double nv = *xi + v;
-          *xi = nv;
-          dn += v * v;
-          xn += nv * nv;
-        } else {
-          *xi = v;
+      // SAP_ROWS rows in flight: unconditional loads of clamped rows, one
+      // wait, then the stores (a per-row loop waits for the previous row's
+      // stores before its loads land)
+      for (int ib = k * SADDLE_ROWS; ib < i1; ib += SAP_ROWS) {
+        double yv[SAP_ROWS], dv[SAP_ROWS], uv[SAP_ROWS], wv[SAP_ROWS], xv[SAP_ROWS];
+#pragma unroll
+        for (int q = 0; q < SAP_ROWS; ++q) {
+          // 32-bit offsets from the macro's bases (at most n * ldy elements)
+          const unsigned i = (unsigned)min(ib + q, n - 1);
+          yv[q] = (double)global_ptr(Y)[i * (unsigned)B.ldy + c];
+          dv[q] = global_ptr(ds)[i];
+          if (L.k) {
+            uv[q] = global_ptr(u)[i];
+            wv[q] = global_ptr(w)[i];
+          }
+          if (REFINE) xv[q] = global_ptr(x)[i * (unsigned)ncols + c];
+        }
+        __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
+#pragma unroll
+        for (int q = 0; q < SAP_ROWS; ++q) {
+          const int i = ib + q;
+          if (i < i1) {
+            double v = dv[q] * yv[q];
+            if (L.k) v += sigma * xp * uv[q] - mu * wv[q];
+            double *xi = x + (size_t)i * ncols + c;
+            if (REFINE) {
+              const double nv = xv[q] + v;
+              *xi = nv;
+              dn += v * v;
+              xn += nv * nv;
+            } else {
+              *xi = v;
+            }
+          }
         }
       }
       part[k * ncols + c] = dn;
@@ -2394,21 +2425,21 @@ __global__ void __launch_bounds__(SADDLE_THREADS)
     }
     if (L.k && lane == 0) {
       double *xm = x + (size_t)n * ncols + c;
-      if (it > 0)
+      if (REFINE)
         *xm += mu;
       else
         *xm = mu;
     }
   }
   __syncthreads();
-  if (tid < ncols && !(it > 0 && colstate[(size_t)mac * ncols + tid])) {
+  if (tid < ncols && !(REFINE && colstate[(size_t)mac * ncols + tid])) {
     double D2 = 0.0, X2 = 0.0;
     for (int k = 0; k < nchunk; ++k) {
       D2 += part[k * ncols + tid];
       X2 += part[(nchunk + k) * ncols + tid];
     }
     double *cn = colnorm + ((size_t)mac * ncols + tid) * 3;
-    if (it > 0) {
+    if (REFINE) {
       cn[2] = cn[0];
       cn[0] = D2;
       cn[1] = X2;
@@ -3728,14 +3759,17 @@ static int run_solve(mhm_batch *b, const double *rhs, double *X, int ncols, int 
       TimerSpan span(MHM_TIMER_SOLVE, st);
       if (int rc = launch_solve<T>(b, ncols, b->colstate, b->bk.iters, st)) return rc;
     }
+    const void *sap = it > 0 ? (const void *)k_saddle_apply<T, true> : (const void *)k_saddle_apply<T, false>;
     if (saddle_lds(L.n, ncols) > 65536)
-      HIP_TRY(hipFuncSetAttribute((const void *)k_saddle_apply<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  (int)saddle_lds(L.n, ncols)));
+      HIP_TRY(hipFuncSetAttribute(sap, hipFuncAttributeMaxDynamicSharedMemorySize, (int)saddle_lds(L.n, ncols)));
     // binary64 refinement passes leave few macros: small blocks dispatch faster
     const int sthreads = (it > 0 && sizeof(T) == 8) ? 256 : SADDLE_THREADS;
-    hipLaunchKernelGGL((k_saddle_apply<T>), dim3(b->M), dim3(sthreads), saddle_lds(L.n, ncols), st, L, b->bk, ncols,
-                       (const T *)b->y,
-                       (const double *)b->beta, X, it, (const int *)b->colstate, b->colnorm);
+    if (it > 0)
+      hipLaunchKernelGGL((k_saddle_apply<T, true>), dim3(b->M), dim3(sthreads), saddle_lds(L.n, ncols), st, L, b->bk,
+                         ncols, (const T *)b->y, (const double *)b->beta, X, (const int *)b->colstate, b->colnorm);
+    else
+      hipLaunchKernelGGL((k_saddle_apply<T, false>), dim3(b->M), dim3(sthreads), saddle_lds(L.n, ncols), st, L, b->bk,
+                         ncols, (const T *)b->y, (const double *)b->beta, X, (const int *)b->colstate, b->colnorm);
     LAUNCH_CHECK("k_saddle_apply");
   }
   if (int rc = launch_residual<T>(b, ncols, rhs, X, rtol, 0.0, 1, 1, st)) return rc;
