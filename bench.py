@@ -54,7 +54,7 @@ METRIC = "macro solves/s at <=1e-10 rel err vs FP64 oracle (vs plain FP64); p50/
 # rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE and scaled by the counters' reading of
 # a 1 GiB device copy (profiles/traffic_fp64_config2.md); quoted as
 # roofline.traffic when the kernel is the dominant one
-MEASURED_TRAFFIC_PER_MACRO = {"k_factor_panel<double>": 144856 + 1502334, "k_solve_mfma<17, 3>": 2705026 + 632842}
+MEASURED_TRAFFIC_PER_MACRO = {"k_factor_panel<double>": 146749 + 1445008, "k_solve_mfma<17, 3>": 2562785 + 631958}
 HBM_PEAK_GBS = 8000.0
 PEAK_TFLOPS = {"f64": 78.6, "f32": 157.3}
 
@@ -68,7 +68,7 @@ WORKLOADS = {
         f"m={p.m} P0 multipliers (2x2 cells per face), K=1e-3 in r=0.02 spheres at 20% volume, f=1")),
 }
 WAVE_WINDOWS = (8, 16, 24, 32, 48, 64, 68, 80, 96)
-BLOCK_SLOTS = (4, 8, 12, 16, 24, 32, 64)
+BLOCK_COLS = (2, 4, 7, 8, 12, 16)  # columns per wavefront of the block solve
 
 
 def factor_kernel(b, tname):
@@ -85,8 +85,8 @@ def solve_kernel(b, ncols, tname):
     with the LDS-window factor, the matrix-core solve with the smallest
     register window of ceil(b / 4) k-steps x ceil(ncols / 16) column tiles
     (at most 52 values per lane); otherwise the wavefront solve with the
-    smallest register window >= b + 1, else the block solve with its register
-    slots per thread (the library's tables)."""
+    smallest register window >= b + 1, else the block solve with its columns
+    per wavefront (the library's tables)."""
     if tname == "double" and 16 + b <= 127 and 1 <= ncols <= 64:
         nct, nk = -(-ncols // 16), -(-b // 4)
         k = next((k for k in MFMA_MAXK if nk <= k and k * nct <= 52), None)
@@ -95,9 +95,7 @@ def solve_kernel(b, ncols, tname):
     w = next((w for w in WAVE_WINDOWS if b + 1 <= w), None)
     if w is not None and ncols <= 64:
         return f"k_solve_wave<{tname}, {w}>"
-    nt = 256 if b >= 12 else 64
-    slots = -(-(b + 1) // (nt // ncols))
-    return f"k_solve<{tname}, {nt}, {next(s for s in BLOCK_SLOTS if slots <= s)}>"
+    return f"k_solve_blk<{tname}, {next(c for c in BLOCK_COLS if ncols <= 4 * c)}>"
 
 
 _LIMITS = None
@@ -146,17 +144,19 @@ def per_macro_traffic(prob, precision):
 
     Factor (LDS window): read the scaled operator in CSR slot order (nnz ts),
     write the column band and the row band (2 n W ts) and, in binary64, the
-    16 x 16 diagonal blocks (ceil(n / 16) 256 ts).  Solve: read both bands (and
-    in binary64 the inverses of the diagonal blocks, once per sweep), read and
+    packed lower triangles of the 16 x 16 diagonal blocks (ceil(n / 16) 136 ts).
+    Solve: read both bands (and in binary64 the packed inverses of the diagonal
+    blocks, once per sweep), read and
     write the R + 2 workspace columns of the first solve in both sweeps
-    (4 n (R + 2) ts).
+    (4 n (R + 2) ts).  The wide-band factor writes no diagonal blocks.
     """
     lay = prob.layout
     n, b = lay.n, lay.bandwidth
     W = b + 1
     ts = 8 if precision == "fp64" else 4
     R = 1 + lay.m + lay.k
-    blocks = -(-n // 16) * 136 * ts if precision == "fp64" else 0  # packed lower triangles
+    panel = 16 + b <= 127
+    blocks = -(-n // 16) * 136 * ts if precision == "fp64" and panel else 0  # packed lower triangles
     return dict(
         factor_bytes=lay.nnz * ts + 2 * n * W * ts + blocks, factor_flops=n * b * (b + 1),
         solve_bytes=2 * n * W * ts + 2 * blocks + 4 * n * (R + 2) * ts, solve_flops=4 * n * b * R,
