@@ -2196,16 +2196,16 @@ __global__ void __launch_bounds__(64 * RES_WAVES)
 // flight (the next group's loads are issued before this group's sums).  The
 // per-column sums go in fixed chunks of SADDLE_ROWS rows in row order, then in
 // chunk order, so a column's bits do not depend on the other columns of the
-// call.
-#define RHS_PASSES 5
-#define RHS_TILE 4608  // doubles of a row group's values kept in LDS for the sums
+// call.  The macro's row scales are staged in LDS once.
+#define RHS_PASSES 7
+#define RHS_TILE 6400  // doubles of a row group's values kept in LDS for the sums
 
 template <typename T>
 __global__ void __launch_bounds__(SADDLE_THREADS) __attribute__((amdgpu_waves_per_eu(8)))
     k_rhs_init(LayoutK L, BatchK B, int ncols, const double *__restrict__ rhs, T *__restrict__ y,
                double *__restrict__ beta, int *__restrict__ colstate, const double *__restrict__ coupling_rm,
                double *__restrict__ bnorm, int uw_col, int gchunks) {
-  extern __shared__ double part[];  // nchunk x ncols chunk sums, then the group's values
+  extern __shared__ double part[];  // nchunk x ncols chunk sums, the group's values, the row scales
   __shared__ int bad;
   const int mac = blockIdx.x, tid = threadIdx.x;
   const int n = L.n, m = L.m, N = B.N;
@@ -2217,6 +2217,7 @@ __global__ void __launch_bounds__(SADDLE_THREADS) __attribute__((amdgpu_waves_pe
   const int grows = gchunks * SADDLE_ROWS;
   const int ngroup = (n + grows - 1) / grows;
   double *tile = part + (size_t)nchunk * ncols;
+  double *sds = tile + (size_t)grows * ncols;
   if (tid == 0) bad = 0;
   const double *ds = B.dscale + (size_t)mac * n;
   T *Y = y + (size_t)mac * n * B.ldy;
@@ -2248,7 +2249,7 @@ __global__ void __launch_bounds__(SADDLE_THREADS) __attribute__((amdgpu_waves_pe
     scale = B.scal[(size_t)mac * NSCAL + S_CS];
   }
   const int ycol = c < ncols ? c : uw_col + (c - ncols);
-  double v[RHS_PASSES], d[RHS_PASSES];
+  double v[RHS_PASSES];
   // unconditional loads of a clamped row (a select of the address becomes a
   // branch with a wait per pass); values of rows outside the group are unused
   auto issue = [&](int g) {
@@ -2256,11 +2257,12 @@ __global__ void __launch_bounds__(SADDLE_THREADS) __attribute__((amdgpu_waves_pe
     for (int p = 0; p < RHS_PASSES; ++p) {
       const int i = min(g * grows + r + p * rp, n - 1);
       v[p] = global_ptr(src)[(size_t)i * stride];
-      d[p] = global_ptr(ds)[i];
     }
   };
   bool finite = true;
   issue(0);
+  for (int t = tid; t < n; t += SADDLE_THREADS) sds[t] = global_ptr(ds)[t];
+  __syncthreads();
   for (int g = 0; g < ngroup; ++g) {
     // every load of the group has landed: without this the branchy consume
     // loop waits for vmcnt(0) per pass, i.e. for the previous pass's stores
@@ -2269,13 +2271,14 @@ __global__ void __launch_bounds__(SADDLE_THREADS) __attribute__((amdgpu_waves_pe
     for (int p = 0; p < RHS_PASSES; ++p) {
       const int ir = r + p * rp, i = g * grows + ir;
       if (act && ir < grows && i < n) {
+        const double d = sds[i];
         if (c < ncols) {
           const double x = v[p] * scale;
           finite &= isfinite(x);
           tile[ir * ncols + c] = x;
-          Y[(size_t)i * B.ldy + ycol] = T(d[p] * x);
+          Y[(size_t)i * B.ldy + ycol] = T(d * x);
         } else {
-          Y[(size_t)i * B.ldy + ycol] = T(conc ? d[p] * scale * v[p] : (pinc && i == L.pin ? d[p] : 0.0));
+          Y[(size_t)i * B.ldy + ycol] = T(conc ? d * scale * v[p] : (pinc && i == L.pin ? d : 0.0));
         }
       }
     }
@@ -2315,11 +2318,11 @@ __global__ void __launch_bounds__(SADDLE_THREADS) __attribute__((amdgpu_waves_pe
   if (tid == 0 && bad) atomicOr(&B.status[mac], MHM_STATUS_NONFINITE);
 }
 
-// Chunks per k_rhs_init row group: the most (up to 4) whose values fit
+// Chunks per k_rhs_init row group: the most (up to 8) whose values fit
 // RHS_TILE and whose rows take at most RHS_PASSES passes of the block.
 static int rhs_group_chunks(int ncols, int width) {
   const int rp = SADDLE_THREADS / width;
-  int g = 4;
+  int g = 8;
   while (g > 1 && (g * SADDLE_ROWS * ncols > RHS_TILE || (g * SADDLE_ROWS + rp - 1) / rp > RHS_PASSES)) --g;
   return g;
 }
@@ -3344,7 +3347,7 @@ int mhm_batch_create(const mhm_layout *L, int32_t macros, int32_t precision, int
   const int widest = std::max(std::max(1 + lk.m + lk.k, 2), max_columns);
   if (!solve_supported(lk.b, widest, tsize, limit) || !solve_supported(lk.b, 2, tsize, limit))
     return fail(MHM_EINVAL, "solve window exceeds the register or LDS budget");
-  if (std::max(saddle_lds(lk.n, widest), saddle_lds(lk.n, widest) / 2 + RHS_TILE * sizeof(double)) >
+  if (std::max(saddle_lds(lk.n, widest), saddle_lds(lk.n, widest) / 2 + (RHS_TILE + lk.n) * sizeof(double)) >
       (size_t)limit)
     return fail(MHM_EINVAL, "too many local rows for the refinement's per-column sums");
   if (factor_lds(lk.b, tsize) > (size_t)limit &&
@@ -3733,7 +3736,7 @@ static int run_solve(mhm_batch *b, const double *rhs, double *X, int ncols, int 
       TimerSpan span(MHM_TIMER_RESIDUAL, st);
       if (it == 0) {
         const int gch = rhs_group_chunks(ncols, uw_inline ? ncols + 2 : ncols);
-        const size_t shm = saddle_lds(L.n, ncols) / 2 + (size_t)gch * SADDLE_ROWS * ncols * sizeof(double);
+        const size_t shm = saddle_lds(L.n, ncols) / 2 + ((size_t)gch * SADDLE_ROWS * ncols + L.n) * sizeof(double);
         if (shm > 65536)
           HIP_TRY(hipFuncSetAttribute((const void *)k_rhs_init<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                       (int)shm));
