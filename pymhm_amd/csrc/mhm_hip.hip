@@ -2351,6 +2351,8 @@ __global__ void __launch_bounds__(SADDLE_THREADS) __attribute__((amdgpu_waves_pe
   const bool on = lane < lanes;
   const int n = L.n, N = B.N;
   const int nchunk = (n + SADDLE_ROWS - 1) / SADDLE_ROWS;
+  if (REFINE && !__syncthreads_or(tid < ncols && colstate[(size_t)mac * ncols + tid] == 0))
+    return;  // every column of the macro has converged (uniform over the block)
   const bool skip = !on || (REFINE && colstate[(size_t)mac * ncols + c]);
   double *x = X + (size_t)mac * N * ncols;
   const double *ds = B.dscale + (size_t)mac * n;
