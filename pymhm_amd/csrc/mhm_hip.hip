@@ -2102,18 +2102,18 @@ __global__ void __launch_bounds__(64 * RES_WAVES)
       }
     }
     __syncthreads();  // the chunk's stage and ring reads are done
+    // Every load in flight is a prefetch consumed here: wait for them once,
+    // before this chunk's stores, so that no later wait covers the stores or
+    // the next prefetch.  store_chunk reads and chunk_commit rewrites the same
+    // stage elements per thread, in that order.
+    __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0), expcnt and lgkmcnt untouched
+    const int lo = i0 + RES_ROWS + b, hi = min(n, i0 + 2 * RES_ROWS + b);
+    // rows of the next chunk into the slots of rows [i0 - b, i0 - b + RES_ROWS)
+    if (use_ring && more) ring_commit(lo, hi);
     if (!check_only) store_chunk(i0);
-    if (use_ring && more) {
-      // rows of the next chunk into the slots of rows [i0 - b, i0 - b + RES_ROWS)
-      const int lo = i0 + RES_ROWS + b, hi = min(n, i0 + 2 * RES_ROWS + b);
-      ring_commit(lo, hi);
-      ring_issue(hi, min(n, i0 + 3 * RES_ROWS + b));
-    }
-    __syncthreads();  // the stage is written out, the ring holds the next chunk's rows
-    if (more) {
-      chunk_commit(i0 + RES_ROWS);
-      __syncthreads();
-    }
+    if (more) chunk_commit(i0 + RES_ROWS);
+    if (use_ring && more) ring_issue(hi, min(n, i0 + 3 * RES_ROWS + b));
+    __syncthreads();  // the ring and the stage hold the next chunk
   }
   // per-wavefront column sums over the dead ring and stage: tot[(w*5 + k)*64
   // + c] = ||r||^2, constraint row (hi, lo), nonfinite flag, FAST bound sum
