@@ -932,12 +932,23 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
       }
       if (failed && lane == 0) bad = MHM_STATUS_PIVOT;
     } else {
+      // all slot indices, then all values: unconditional loads at clamped
+      // positions, masked afterwards (a branch per load waits for each load,
+      // the previous value's included)
+      int qv[NPF];
 #pragma unroll
       for (int v = 0; v < NPF; ++v) {
         const int e = tid - 64 + 192 * v;
         const int t = divW(e);
-        const int q = e < NB * W ? slot_of(j0 + S + t, e - t * W) : -1;
-        pf[v] = q >= 0 ? sv[q] : T(0);
+        const int j = j0 + S + t, r = e - t * W;
+        const bool in = e < NB * W && j < n && j + r < n;
+        const int raw = global_ptr(bslot)[in ? (size_t)j * W + r : 0];
+        qv[v] = in ? raw : -1;
+      }
+#pragma unroll
+      for (int v = 0; v < NPF; ++v) {
+        const T val = global_ptr(sv)[qv[v] >= 0 ? qv[v] : 0];
+        pf[v] = qv[v] >= 0 ? val : T(0);
       }
     }
     __syncthreads();
