@@ -1,7 +1,7 @@
 /*
  * mhm_hip.h - C ABI of the MI355X (gfx950) MHM local-problem stage.
  *
- * One library, libmhm_hip.so, built from pymhm_amd/csrc/mhm_hip.hip.  Every
+ * One library, libmhm_hip.so, built from the .hip sources in pymhm_amd/csrc.  Every
  * entry point returns an int status (MHM_OK on success); mhm_last_error()
  * returns the message of the last failure on the calling thread.  Array
  * arguments named d_* are device pointers on the handle's device; the caller
@@ -50,7 +50,9 @@ enum { MHM_FP64 = 0, MHM_MIXED = 1 };
 enum {
   MHM_STATUS_PIVOT = 1,    /* non-positive or tiny pivot in the band factor      */
   MHM_STATUS_RESIDUAL = 2, /* some column missed the residual criterion          */
-  MHM_STATUS_NONFINITE = 4 /* non-finite operator, pivot or solution             */
+  MHM_STATUS_NONFINITE = 4, /* non-finite operator, pivot or solution            */
+  MHM_STATUS_COEFFICIENT = 8 /* last assembly: diffusion not finite and positive, or
+                                source not finite, at some quadrature point        */
 };
 
 typedef struct mhm_layout mhm_layout;
@@ -215,7 +217,11 @@ int mhm_batch_reconstruct(mhm_batch *batch, const double *d_trace, const double 
                           double *d_fields, int32_t ncases, void *stream);
 
 /* Synchronise the stream and copy the per-macro status bits to host memory
- * (host_status[macros]); also returns the bitwise OR in *any. */
+ * (host_status[macros]); also returns the bitwise OR in *any.  The bits are
+ * those of the current factors (PIVOT, NONFINITE operator; kept until the next
+ * mhm_batch_factor), of the last solving call (RESIDUAL, NONFINITE data; each
+ * mhm_batch_condense/_solve/_sources starts from none) and of the last
+ * assembly (COEFFICIENT). */
 int mhm_batch_status(mhm_batch *batch, int32_t *host_status, int32_t *any, void *stream);
 
 /* Per-macro refinement counts of the last solve (host_counts[macros]); synchronises. */
@@ -234,6 +240,53 @@ int mhm_scatter_coordinates(const int64_t *d_indices, int32_t width, int32_t cou
  * summation of the COO -> CSC conversion (core/contributions.py:175-185). */
 int mhm_ordered_gather(const int64_t *d_ptr, const int64_t *d_code, const double *d_src,
                        int64_t count, double *d_out, double *d_abs_out, void *stream);
+
+/* ---------------------------------------------------------------- dense batched LU
+ * General real matrices, the reference's resident factor object
+ * BatchedFactorization(matrices, rtol, atol) (src/pymhm/execution/cuda.py:165-288):
+ * row then column max-abs equilibration (:198-207), LU with partial pivoting
+ * and the rank test of _factor_batch (:114-136), solves with up to two
+ * binary64 corrections and per-column acceptance (:223-255, _apply :257-263).
+ * Arrays are row-major like numpy: matrices [batch][n][n], right-hand sides
+ * and solutions [batch][n][nrhs].  Per-matrix status bits go to the caller's
+ * device array d_status[batch] (cleared by each call); the calls enqueue work
+ * on `stream` and do not synchronise unless stated. */
+enum {
+  MHM_DENSE_NONFINITE_MATRIX = 1, /* -> ValueError "matrices must be finite ..."    */
+  MHM_DENSE_ZERO_ROW = 2,         /* -> LinearSolveError "zero row"                 */
+  MHM_DENSE_ZERO_COLUMN = 4,      /* -> LinearSolveError "zero column"              */
+  MHM_DENSE_SINGULAR = 8,         /* exactly zero pivot: "singular pivot"            */
+  MHM_DENSE_RANK = 16,            /* min|U_jj| <= eps n max|LU|: "rank deficient"    */
+  MHM_DENSE_NONFINITE_RHS = 32,   /* -> ValueError "rhs must be finite ..."          */
+  MHM_DENSE_NONFINITE_RESULT = 64,/* "nonfinite coefficients"                        */
+  MHM_DENSE_RESIDUAL = 128        /* "residual criterion failed"                     */
+};
+
+typedef struct mhm_dense mhm_dense;
+
+/* Copy and factor `batch` matrices on `device`; factor status to d_status.
+ * Replaces BatchedFactorization.__init__ (cuda.py:178-215). */
+int mhm_dense_create(int32_t device, int32_t batch, int32_t n, const double *d_matrices, int32_t *d_status,
+                     void *stream, mhm_dense **out);
+
+/* Solve d_rhs[batch][n][nrhs] into d_x against the resident factors with the
+ * acceptance max(atol / s, rtol ||b / s||) per column.  A matrix whose factor
+ * failed is skipped (its d_x is left as it was).  Grows the solve workspace
+ * when nrhs exceeds every earlier call's (that call synchronises the stream).
+ * Replaces BatchedFactorization.solve (cuda.py:223-255). */
+int mhm_dense_solve(mhm_dense *h, const double *d_rhs, int32_t nrhs, double rtol, double atol, double *d_x,
+                    int32_t *d_status, void *stream);
+
+/* Synchronise and release.  Replaces BatchedFactorization.close (cuda.py:265-273). */
+int mhm_dense_destroy(mhm_dense *h);
+
+/* One-shot factor and solve of the same batch (the per-cell LocalLinearSolver
+ * seam, src/pymhm/core/assembly.py:32-47,150-156, and condense_batched's
+ * factor-then-solve, cuda.py:304-309).  Allocates its workspace, synchronises
+ * the stream and frees it before returning. */
+int mhm_dense_factor_solve(int32_t device, int32_t batch, int32_t n, int32_t nrhs, const double *d_matrices,
+                           const double *d_rhs, double *d_x, double rtol, double atol, int32_t *d_status,
+                           void *stream);
 
 /* Kernel timing with HIP events recorded on the launch stream (no
  * counterpart in the reference, whose benchmarks time phases with

@@ -16,7 +16,9 @@ LIBRARY = _HERE / "libmhm_hip.so"
 
 MHM_OK, MHM_EINVAL, MHM_ESINGULAR, MHM_ERESIDUAL, MHM_ENOMEM, MHM_EDEVICE = range(6)
 MHM_FP64, MHM_MIXED = 0, 1
-STATUS_PIVOT, STATUS_RESIDUAL, STATUS_NONFINITE = 1, 2, 4
+STATUS_PIVOT, STATUS_RESIDUAL, STATUS_NONFINITE, STATUS_COEFFICIENT = 1, 2, 4, 8
+(DENSE_NONFINITE_MATRIX, DENSE_ZERO_ROW, DENSE_ZERO_COLUMN, DENSE_SINGULAR, DENSE_RANK, DENSE_NONFINITE_RHS,
+ DENSE_NONFINITE_RESULT, DENSE_RESIDUAL) = (1, 2, 4, 8, 16, 32, 64, 128)
 
 EXPORTS = (
     "mhm_last_error", "mhm_device_count", "mhm_version",
@@ -26,6 +28,7 @@ EXPORTS = (
     "mhm_batch_solve", "mhm_batch_sources", "mhm_batch_reconstruct", "mhm_batch_status",
     "mhm_batch_iterations", "mhm_scatter_coordinates", "mhm_ordered_gather",
     "mhm_timing_enable", "mhm_timing_reset", "mhm_timing_read",
+    "mhm_dense_create", "mhm_dense_solve", "mhm_dense_destroy", "mhm_dense_factor_solve",
 )
 TIMERS = {"assemble": 0, "factor": 1, "solve": 2, "residual": 3, "condense": 4}
 
@@ -96,6 +99,10 @@ def load(path: str | os.PathLike | None = None):
         "mhm_timing_enable": (I32,),
         "mhm_timing_reset": (),
         "mhm_timing_read": (I32, P, P),
+        "mhm_dense_create": (I32, I32, I32, P, P, P, P),
+        "mhm_dense_solve": (P, P, I32, D, D, P, P, P),
+        "mhm_dense_destroy": (P,),
+        "mhm_dense_factor_solve": (I32, I32, I32, I32, P, P, P, D, D, P, P),
     }
     for name in EXPORTS:
         fn = getattr(lib, name)

@@ -6,9 +6,13 @@
   ``coarse_basis``, ``left_kernel``, ``test_constraints``), groups them by
   local sparsity pattern (the reference groups by ``(N, R)``), condenses every
   group on the GPU and returns responses in input order.
-* ``BatchedFactorization`` mirrors ``cuda.py:165-288`` for the augmented
-  operators of such problems: factor once, ``solve`` new right-hand sides
-  against the resident factors, ``close`` idempotently.
+* ``BatchedFactorization`` is the reference's dense resident factor object
+  (``cuda.py:165-288``: ``(batch, n, n)`` matrices, ``rtol``/``atol``),
+  implemented by the dense batched LU of ``pymhm_amd.dense``.
+* ``LocalFactorization`` keeps the band factors of the augmented operators of
+  ``LocalProblem``-shaped inputs resident: factor once, ``solve`` new
+  right-hand sides, ``close`` idempotently.  It is the fast path behind
+  ``condense_batched`` for the primal scalar-diffusion family.
 
 Scope of the HIP factorisation: symmetric (to rounding) local operators with
 at most one retained mode, positive definite on the complement of the mode -
@@ -24,6 +28,7 @@ import numpy as np
 import torch
 from scipy import sparse
 
+from .dense import BatchedFactorization, HipLocalSolver, factor_solve  # noqa: F401  (re-exported)
 from .layout import LocalLayout, choose_permutation
 from .stage import LocalBatch
 
@@ -142,18 +147,19 @@ def condense_batched(problems, *, precision: str = "fp64", device: int = 0):
     return tuple(responses[i] for i in range(len(problems)))
 
 
-class BatchedFactorization:
-    """Resident factors of the augmented operators of equal-pattern problems.
+class LocalFactorization:
+    """Resident band factors of the augmented operators of equal-pattern problems.
 
-    ``solve(rhs)`` takes ``(batch, N, nrhs)`` like ``cuda.py:223-255`` and
-    returns device tensors unless ``host=True``.
+    ``solve(rhs)`` takes ``(batch, N, nrhs)`` like ``cuda.py:223-255`` (any
+    ``nrhs``: columns go to the device ``max_columns`` at a time, each column
+    computed independently) and returns device tensors unless ``host=True``.
     """
 
     def __init__(self, problems, *, precision: str = "fp64", max_columns: int = 8, device: int = 0):
         problems = list(problems)
         groups = _prepare(problems, precision, max_columns, device)
         if len(groups) != 1:
-            raise ValueError("BatchedFactorization needs problems of one local pattern")
+            raise ValueError("LocalFactorization needs problems of one local pattern")
         self._group = groups[0]
         self._group.batch.factor()
         self._closed = False
@@ -161,10 +167,13 @@ class BatchedFactorization:
     def solve(self, rhs, *, host: bool = False):
         if self._closed:
             raise RuntimeError("batched factorization is closed")
-        rhs = torch.as_tensor(rhs, dtype=torch.float64, device=self._group.batch.device)
-        if rhs.dim() != 3:
-            raise ValueError("rhs must have shape (batch, N, nrhs)")
-        x = self._group.batch.solve(rhs.transpose(1, 2).contiguous()).transpose(1, 2)
+        b = self._group.batch
+        rhs = torch.as_tensor(rhs, dtype=torch.float64, device=b.device)
+        if rhs.dim() != 3 or rhs.shape[0] != b.macros or rhs.shape[1] != b.N or rhs.shape[2] == 0:
+            raise ValueError(f"rhs must have shape ({b.macros}, {b.N}, nrhs)")
+        cols = rhs.transpose(1, 2).contiguous()
+        parts = [b.solve(cols[:, c:c + b.max_columns]) for c in range(0, cols.shape[1], b.max_columns)]
+        x = torch.cat(parts, dim=1).transpose(1, 2)
         return x.cpu().numpy() if host else x
 
     def close(self):

@@ -28,6 +28,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cfloat>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -284,7 +285,9 @@ enum { S_AMAX = 0, S_SIGMA, S_CS, S_A11, S_A12, S_A21, S_A22, S_DET, NSCAL };
 struct BatchK {
   int M, R, N, ldy;
   double *aval, *load, *az, *dscale, *scal, *uw;
-  int *status, *iters;
+  int *status, *iters;  // status: factor-time bits (pivot, non-finite operator), kept until refactored
+  int *cstatus;         // per-call bits of the last solve (residual, non-finite data), cleared per call
+  int *astatus;         // bits of the last assembly (coefficient outside (0, inf) or non-finite source)
   const int *colmap;                          // per-macro signed column map or null
   const double *colscale, *bcoup, *cons, *kern;  // per-macro overrides (band rows) or null
   const double *glam, *meas;                     // per-macro element geometry or null
@@ -346,7 +349,7 @@ template <int NB, int D>
 __global__ void __launch_bounds__(64 * ASM_WAVES)
     k_assemble(LayoutK L, int M, const double *__restrict__ kq, const double *__restrict__ fq,
                double fconst, double *__restrict__ ework, double *__restrict__ fwork,
-               const double *__restrict__ bglam, const double *__restrict__ bmeas) {
+               const double *__restrict__ bglam, const double *__restrict__ bmeas, int *__restrict__ astatus) {
 #pragma clang fp contract(off)
   constexpr int EPW = 64 / NB;
   constexpr int DP = D == 3 ? 4 : D;  // padded row of k grad(phi) in LDS
@@ -390,8 +393,10 @@ __global__ void __launch_bounds__(64 * ASM_WAVES)
     const double *fp = fq ? fq + ((size_t)mac * L.ne + e) * nq : nullptr;
     double knext = kp[0];
     double fnext = fp ? fp[0] : fconst;
+    bool valid = true;  // diffusion finite and positive, source finite (materials/evaluation.py:16-29)
     for (int q = 0; q < nq; ++q) {
       const double k = knext, f = fnext;
+      valid &= (k > 0.0) & (fabs(k) <= DBL_MAX) & (fabs(f) <= DBL_MAX);
       if (q + 1 < nq) {
         knext = kp[q + 1];
         if (fp) fnext = fp[q + 1];
@@ -425,6 +430,7 @@ __global__ void __launch_bounds__(64 * ASM_WAVES)
 #pragma unroll
       for (int j = 0; j < NB; ++j) ework[base + (size_t)(i * NB + j) * L.ne + e] = tot[j] + cor[j];
       fwork[(size_t)mac * NB * L.ne + (size_t)i * L.ne + e] = F;
+      if (!valid && i == 0) atomicOr(&astatus[mac], MHM_STATUS_COEFFICIENT);
     }
   }
 }
@@ -568,6 +574,18 @@ __global__ void __launch_bounds__(128)
     dd_fma(hi, lo, B.aval[(size_t)mac * L.nnz + s], kernel_at(L, B, mac, L.cols[s]));
   B.az[(size_t)mac * L.n + i] = hi + lo;
   B.load[(size_t)mac * L.n + i] = loads_orig[(size_t)mac * L.n + L.perm[i]];
+}
+
+// A Z again after new retained modes (same double-double slot-order sums as
+// k_gather and k_rowsum, so the bits do not depend on the call order).
+__global__ void __launch_bounds__(128) k_retained_action(LayoutK L, BatchK B) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int mac = blockIdx.y;
+  if (i >= L.n) return;
+  double hi = 0.0, lo = 0.0;
+  for (int s = L.rowptr[i]; s < L.rowptr[i + 1]; ++s)
+    dd_fma(hi, lo, B.aval[(size_t)mac * L.nnz + s], kernel_at(L, B, mac, L.cols[s]));
+  B.az[(size_t)mac * L.n + i] = hi + lo;
 }
 
 // ----------------------------------------------------------------- factor preparation
@@ -1881,6 +1899,7 @@ __device__ __forceinline__ double rhs_at(const LayoutK &L, const BatchK &B, cons
 struct ResShared {
   const double *colptr[64];
   int colstride[64];
+  int store[64];  // the column's residual becomes its next correction's right-hand side
   double colscale[64], mu[64];
 };
 
@@ -1940,6 +1959,10 @@ __global__ void __launch_bounds__(64 * RES_WAVES)
     }
     sh.colptr[c] = ptr;
     sh.colstride[c] = stride;
+    // the double-double stage rewrites only the columns it takes over: a
+    // column whose binary64 residual was kept must not see a more accurate one
+    // because another column of its macro needed it (columns stay independent)
+    sh.store[c] = c < ncols && (FAST || (state[c] == 0 && done[c] == 0));
     sh.colscale[c] = scale;
     sh.mu[c] = (c < ncols && L.k) ? x[(size_t)n * ncols + c] : 0.0;
   }
@@ -1960,7 +1983,7 @@ __global__ void __launch_bounds__(64 * RES_WAVES)
     for (int e = tid; e < RES_ROWS * ncols; e += NT) {
       const int q = divc(e), c = e - q * ncols;
       const int i = i0 + q;
-      if (i < n) Y[(size_t)i * B.ldy + c] = T(stg[q * ldr + c]);
+      if (i < n && sh.store[c]) Y[(size_t)i * B.ldy + c] = T(stg[q * ldr + c]);
     }
   };
   // ring rows entering before chunk i0: [i0 + b, i0 + RES_ROWS + b) (all of
@@ -2182,9 +2205,9 @@ __global__ void __launch_bounds__(64 * RES_WAVES)
     }
     return;
   }
-  if (BAD != 0.0) atomicOr(&B.status[mac], MHM_STATUS_NONFINITE);
+  if (BAD != 0.0) atomicOr(&B.cstatus[mac], MHM_STATUS_NONFINITE);
   if (check_only) {
-    if (!resid_ok) atomicOr(&B.status[mac], MHM_STATUS_RESIDUAL);
+    if (!resid_ok) atomicOr(&B.cstatus[mac], MHM_STATUS_RESIDUAL);
   } else {
     state[c] = ok ? 1 : 0;
     beta[(size_t)mac * ncols + c] = rn;
@@ -2326,7 +2349,7 @@ __global__ void __launch_bounds__(SADDLE_THREADS) __attribute__((amdgpu_waves_pe
     colstate[(size_t)mac * ncols + tid] = 0;
     beta[(size_t)mac * ncols + tid] = rhs ? rhs[((size_t)mac * N + n) * ncols + tid] : 0.0;
   }
-  if (tid == 0 && bad) atomicOr(&B.status[mac], MHM_STATUS_NONFINITE);
+  if (tid == 0 && bad) atomicOr(&B.cstatus[mac], MHM_STATUS_NONFINITE);
 }
 
 // Chunks per k_rhs_init row group: the most (up to 8) whose values fit
@@ -2745,6 +2768,7 @@ struct mhm_batch {
   int *colmap = nullptr;
   double *colscale = nullptr, *bcoup = nullptr, *cons = nullptr, *kern = nullptr;
   bool factored = false, condensed = false;
+  bool operator_set = false;  // aval/load/az hold an assembled or supplied operator
   bool uw_pending = false;  // u, w and the bordered 2x2 not yet computed for the current factors
   std::vector<void *> owned;
 };
@@ -3182,6 +3206,10 @@ static int launch_solve(const mhm_batch *b, int ncols, const int *colstate, int 
 
 // Exported entry points: C linkage comes from their declarations in mhm_hip.h.
 
+// Sets the calling thread's last error for the other translation units of the
+// library (the dense batched LU); returns code.
+int mhm_internal_fail(int code, const char *msg) { return fail(code, msg); }
+
 int mhm_last_error(char *buf, size_t len) {
   if (buf && len) {
     std::strncpy(buf, g_error.c_str(), len - 1);
@@ -3395,7 +3423,9 @@ int mhm_batch_create(const mhm_layout *L, int32_t macros, int32_t precision, int
   if ((rc = dev_alloc(b->owned, &k.aval, M * lk.nnz)) || (rc = dev_alloc(b->owned, &k.load, M * n)) ||
       (rc = dev_alloc(b->owned, &k.az, M * n)) || (rc = dev_alloc(b->owned, &k.dscale, M * n)) ||
       (rc = dev_alloc(b->owned, &k.scal, M * NSCAL)) || (rc = dev_alloc(b->owned, &k.uw, M * 2 * n)) ||
-      (rc = dev_alloc(b->owned, &k.status, M)) || (rc = dev_alloc(b->owned, &k.iters, M)) ||
+      (rc = dev_alloc(b->owned, &k.status, M)) || (rc = dev_alloc(b->owned, &k.cstatus, M)) ||
+      (rc = dev_alloc(b->owned, &k.astatus, M)) ||
+      (rc = dev_alloc(b->owned, &k.iters, M)) ||
       (rc = dev_alloc(b->owned, &b->xstd, M * R * N)) ||
       (rc = dev_alloc(b->owned, &b->beta, M * ncm)) ||
       (rc = dev_alloc(b->owned, &b->colnorm, M * ncm * 3)) || (rc = dev_alloc(b->owned, &b->bnorm, M * ncm)) ||
@@ -3426,6 +3456,8 @@ int mhm_batch_create(const mhm_layout *L, int32_t macros, int32_t precision, int
     }
   }
   HIP_TRY(hipMemset(k.status, 0, M * sizeof(int)));
+  HIP_TRY(hipMemset(k.cstatus, 0, M * sizeof(int)));
+  HIP_TRY(hipMemset(k.astatus, 0, M * sizeof(int)));
   HIP_TRY(hipMemset(b->rdone, 0, M * ncm * sizeof(int)));
   if (hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreateWithFlags(&b->ev_mark, hipEventDisableTiming) != hipSuccess ||
@@ -3470,8 +3502,9 @@ static int launch_assemble(const mhm_batch *b, const double *kq, const double *f
   }
   const int tasks = (L.ne + EPW - 1) / EPW;
   const dim3 grid((unsigned)((tasks + ASM_WAVES - 1) / ASM_WAVES), (unsigned)std::min(b->M, 2048));
+  hipLaunchKernelGGL(k_fill_int, dim3((b->M + 255) / 256), dim3(256), 0, st, b->bk.astatus, 0, (size_t)b->M);
   hipLaunchKernelGGL((k_assemble<NB, D>), grid, dim3(64 * ASM_WAVES), shm, st, L, b->M, kq, fq, fc, b->ework,
-                     b->fwork, b->bk.glam, b->bk.meas);
+                     b->fwork, b->bk.glam, b->bk.meas, b->bk.astatus);
   LAUNCH_CHECK("k_assemble");
   return MHM_OK;
 }
@@ -3507,6 +3540,7 @@ int mhm_batch_assemble(mhm_batch *b, const double *kq, const double *fq, double 
   hipLaunchKernelGGL(k_gather, grid2(L.n, GATHER_ROWS, b->M), dim3(256), gshm, st, L, b->bk, b->ework, b->fwork);
   LAUNCH_CHECK("k_gather");
   b->factored = b->condensed = false;
+  b->operator_set = true;
   return MHM_OK;
 }
 
@@ -3517,9 +3551,11 @@ int mhm_batch_set_operator(mhm_batch *b, const double *values, const double *loa
   hipStream_t st = (hipStream_t)stream;
   HIP_TRY(hipMemcpyAsync(b->bk.aval, values, (size_t)b->M * L.nnz * sizeof(double),
                          hipMemcpyDeviceToDevice, st));
+  hipLaunchKernelGGL(k_fill_int, dim3((b->M + 255) / 256), dim3(256), 0, st, b->bk.astatus, 0, (size_t)b->M);
   hipLaunchKernelGGL(k_rowsum, grid2(L.n, 128, b->M), dim3(128), 0, st, L, b->bk, loads);
   LAUNCH_CHECK("k_rowsum");
   b->factored = b->condensed = false;
+  b->operator_set = true;
   return MHM_OK;
 }
 
@@ -3596,7 +3632,14 @@ int mhm_batch_set_coupling(mhm_batch *b, const int32_t *colmap, const double *co
   if (kernels) {
     if ((rc = permuted_copy(b, kernels, &b->kern, 1, st))) return rc;
     b->bk.kern = b->kern;
+    if (b->operator_set) {  // A Z of the operator already in place follows the new modes
+      hipLaunchKernelGGL(k_retained_action, grid2(L.n, 128, b->M), dim3(128), 0, st, L, b->bk);
+      LAUNCH_CHECK("k_retained_action");
+    }
   }
+  // the constraint scaling, u, w and the responses depend on these data:
+  // factor and condense again before using the batch
+  if (colmap || colscale || couplings || constraints || kernels) b->factored = b->condensed = false;
   if (!L.coupling && !b->bk.bcoup && L.m > 0)
     return fail(MHM_EINVAL, "no coupling: the layout has none and no per-macro couplings were set");
   if (L.k && !L.constraint && !b->bk.cons) return fail(MHM_EINVAL, "no constraint vectors");
@@ -3626,6 +3669,7 @@ static int run_factor(mhm_batch *b, hipStream_t st) {
   const mhm_layout *lay = b->layout;
   const LayoutK &L = lay->k;
   hipLaunchKernelGGL(k_fill_int, dim3((b->M + 255) / 256), dim3(256), 0, st, b->bk.status, 0, (size_t)b->M);
+  hipLaunchKernelGGL(k_fill_int, dim3((b->M + 255) / 256), dim3(256), 0, st, b->bk.cstatus, 0, (size_t)b->M);
   const size_t pshm = factor_panel_lds(L.b, sizeof(T));
   const bool panel = factor_panel_ok(L.b, sizeof(T), lay->device);
   hipLaunchKernelGGL((k_prepare<T>), dim3(b->M), dim3(256), 0, st, L, b->bk, (T *)b->band,
@@ -3678,6 +3722,7 @@ int mhm_batch_factor(mhm_batch *b, void *stream) {
   if (L.m > 0 && !L.coupling && !b->bk.bcoup)
     return fail(MHM_EINVAL, "no coupling: set per-macro couplings before factoring");
   if (L.k && !L.constraint && !b->bk.cons) return fail(MHM_EINVAL, "no constraint vectors");
+  if (!b->operator_set) return fail(MHM_EINVAL, "no operator: assemble or set the operator before factoring");
   DeviceGuard guard(b->layout->device);
   hipStream_t st = (hipStream_t)stream;
   int rc = b->precision == MHM_FP64 ? run_factor<double>(b, st) : run_factor<float>(b, st);
@@ -3730,6 +3775,8 @@ static int run_solve(mhm_batch *b, const double *rhs, double *X, int ncols, int 
                      double rtol, hipStream_t st, const std::function<int()> *after_first_pass = nullptr) {
   const LayoutK &L = b->layout->k;
   const double fwd = b->precision == MHM_MIXED ? 1.0e-12 : 0.0;
+  // the status of a solve is its own: bits of an earlier call do not carry over
+  hipLaunchKernelGGL(k_fill_int, dim3((b->M + 255) / 256), dim3(256), 0, st, b->bk.cstatus, 0, (size_t)b->M);
   hipLaunchKernelGGL(k_fill_int, dim3((unsigned)(((size_t)b->M * ncols + 255) / 256)), dim3(256), 0, st,
                      b->colstate, 0, (size_t)b->M * ncols);
   hipLaunchKernelGGL(k_fill_int, dim3((b->M + 255) / 256), dim3(256), 0, st, b->bk.iters, 0, (size_t)b->M);
@@ -3934,14 +3981,17 @@ int mhm_batch_reconstruct(mhm_batch *b, const double *tr, const double *co, doub
 int mhm_batch_status(mhm_batch *b, int32_t *host_status, int32_t *any, void *stream) {
   if (!b) return fail(MHM_EINVAL, "batch is NULL");
   DeviceGuard guard(b->layout->device);
-  std::vector<int> tmp(b->M);
-  HIP_TRY(hipMemcpyAsync(tmp.data(), b->bk.status, b->M * sizeof(int), hipMemcpyDeviceToHost,
-                         (hipStream_t)stream));
+  std::vector<int> tmp(3 * (size_t)b->M);
+  const int *arrays[3] = {b->bk.status, b->bk.cstatus, b->bk.astatus};
+  for (int a = 0; a < 3; ++a)
+    HIP_TRY(hipMemcpyAsync(tmp.data() + (size_t)a * b->M, arrays[a], b->M * sizeof(int), hipMemcpyDeviceToHost,
+                           (hipStream_t)stream));
   HIP_TRY(hipStreamSynchronize((hipStream_t)stream));
   int acc = 0;
   for (int i = 0; i < b->M; ++i) {
-    acc |= tmp[i];
-    if (host_status) host_status[i] = tmp[i];
+    const int v = tmp[i] | tmp[(size_t)b->M + i] | tmp[2 * (size_t)b->M + i];
+    acc |= v;
+    if (host_status) host_status[i] = v;
   }
   if (any) *any = acc;
   return MHM_OK;

@@ -63,8 +63,13 @@ class LocalBatch:
         t = torch.as_tensor(a, dtype=dtype, device=self.device).contiguous()
         return t
 
-    def assemble(self, kq, fq=None, f_const: float = 0.0):
-        """Assemble from diffusion (and source) values at quadrature points ``(M, ne, nq)``."""
+    def assemble(self, kq, fq=None, f_const: float = 0.0, check: bool = False):
+        """Assemble from diffusion (and source) values at quadrature points ``(M, ne, nq)``.
+
+        The device checks that the diffusion is finite and positive and the
+        source finite at every point (``materials/evaluation.py:16-29``); with
+        ``check`` a violation raises ``ValueError`` here (one synchronisation),
+        otherwise it is reported by ``status``/``raise_on_failure`` later."""
         kq = self._dev(kq, torch.float64)
         shape = (self.macros, self.layout.ne, self.layout.nq)
         if tuple(kq.shape) != shape:
@@ -73,8 +78,12 @@ class LocalBatch:
             fq = self._dev(fq, torch.float64)
             if tuple(fq.shape) != shape:
                 raise ValueError(f"fq must have shape {shape}")
+        if not np.isfinite(f_const):
+            raise ValueError("f_const must be finite")
         self._keep = [kq, fq]
         _lib.call("mhm_batch_assemble", self.handle, _ptr(kq), _ptr(fq), float(f_const), _stream())
+        if check:
+            self._raise_coefficients(self.status())
 
     def set_operator(self, values, loads):
         """Assembled operators: CSR values in layout slot order ``(M, nnz)``, loads ``(M, n)``."""
@@ -93,15 +102,29 @@ class LocalBatch:
         _lib.call("mhm_batch_operator", self.handle, _ptr(vals), _ptr(loads), _ptr(az), _stream())
         return vals, loads, az
 
+    def _shaped(self, name, a, dtype, shape):
+        """Device copy of ``a``, which must have exactly ``shape`` (ValueError otherwise)."""
+        if a is None:
+            return None
+        t = self._dev(a, dtype)
+        if tuple(t.shape) != tuple(shape):
+            raise ValueError(f"{name} must have shape {tuple(shape)}, got {tuple(t.shape)}")
+        return t
+
     def set_coupling(self, colmap=None, colscale=None, couplings=None, constraints=None, kernels=None):
-        """Per-macro coupling column maps/scales or full couplings, constraints, kernels."""
+        """Per-macro coupling column maps/scales or full couplings, constraints, kernels.
+
+        Changing any of them invalidates the factors and responses: factor and
+        condense again before solving."""
         M = self.macros
-        cm = None if colmap is None else self._dev(np.asarray(colmap).reshape(M, self.m), torch.int32)
-        cs = None if colscale is None else self._dev(np.asarray(colscale).reshape(M, self.m), torch.float64)
-        cp = None if couplings is None else self._dev(couplings, torch.float64).reshape(M, self.m, self.n)
-        cn = None if constraints is None else self._dev(constraints, torch.float64).reshape(M, self.n)
-        kn = None if kernels is None else self._dev(kernels, torch.float64).reshape(M, self.n)
-        if cm is not None and (int(cm.min()) < 0 or int(cm.max()) >= self.m):
+        cm = self._shaped("colmap", colmap, torch.int32, (M, self.m))
+        cs = self._shaped("colscale", colscale, torch.float64, (M, self.m))
+        cp = self._shaped("couplings", couplings, torch.float64, (M, self.m, self.n))
+        cn = self._shaped("constraints", constraints, torch.float64, (M, self.n))
+        kn = self._shaped("kernels", kernels, torch.float64, (M, self.n))
+        if (constraints is not None or kernels is not None) and not self.k:
+            raise ValueError("constraints and kernels need a layout with a retained mode")
+        if cm is not None and self.m and (int(cm.min()) < 0 or int(cm.max()) >= self.m):
             raise ValueError("colmap entries outside the coupling columns")
         _lib.call("mhm_batch_set_coupling", self.handle, _ptr(cm), _ptr(cs), _ptr(cp), _ptr(cn),
                   _ptr(kn), _stream())
@@ -111,10 +134,11 @@ class LocalBatch:
         """Per-macro element geometry ``(M, elements, dim+1, dim)`` barycentric
         gradients and ``(M, elements)`` measures, for macros that share this
         layout's topology but not its shape."""
-        g = self._dev(grad_lambda, torch.float64).contiguous()
-        ms = self._dev(measure, torch.float64).contiguous()
-        if g.shape[0] != self.macros or ms.shape != (self.macros, g.shape[1]):
-            raise ValueError("grad_lambda (M, elements, dim+1, dim) and measure (M, elements) required")
+        lay = self.layout
+        if getattr(lay, "ne", 0) <= 0:
+            raise ValueError("the layout has no assembly description")
+        g = self._shaped("grad_lambda", grad_lambda, torch.float64, (self.macros, lay.ne, lay.dim + 1, lay.dim))
+        ms = self._shaped("measure", measure, torch.float64, (self.macros, lay.ne))
         _lib.call("mhm_batch_set_geometry", self.handle, _ptr(g), _ptr(ms), _stream())
         torch.cuda.current_stream().synchronize()
 
@@ -153,6 +177,8 @@ class LocalBatch:
         rhs = self._dev(rhs, torch.float64)
         if rhs.dim() != 3 or rhs.shape[0] != self.macros or rhs.shape[2] != self.N:
             raise ValueError(f"rhs must have shape (macros, ncols, {self.N})")
+        if not 1 <= rhs.shape[1] <= self.max_columns:
+            raise ValueError(f"ncols must be in [1, {self.max_columns}] (max_columns)")
         x = torch.empty_like(rhs)
         _lib.call("mhm_batch_solve", self.handle, _ptr(rhs), _ptr(x), int(rhs.shape[1]), int(max_refine),
                   float(rtol), _stream())
@@ -173,13 +199,19 @@ class LocalBatch:
         """
         if max_refine is None:
             max_refine = DEFAULT_REFINE[self.precision]
+        lay = self.layout
+        if getattr(lay, "ne", 0) <= 0:
+            raise ValueError("the layout has no assembly description")
         if fq is not None:
             fq = self._dev(fq, torch.float64)
-            if fq.dim() != 4 or fq.shape[1] != self.macros:
-                raise ValueError("fq must have shape (fields, macros, elements, q)")
+            if fq.dim() != 4 or tuple(fq.shape[1:]) != (self.macros, lay.ne, lay.nq):
+                raise ValueError(f"fq must have shape (fields, {self.macros}, {lay.ne}, {lay.nq}), "
+                                 f"got {tuple(fq.shape)}")
             ncols = int(fq.shape[0])
         elif ncols is None:
             raise ValueError("ncols is required without fq")
+        if not 1 <= int(ncols) <= self.max_columns:
+            raise ValueError(f"ncols must be in [1, {self.max_columns}] (max_columns)")
         loads = torch.empty((ncols, self.macros, self.width), dtype=torch.float64, device=self.device)
         src = (torch.empty((ncols, self.macros, self.n), dtype=torch.float64, device=self.device)
                if source else None)
@@ -194,10 +226,18 @@ class LocalBatch:
         trace = self._dev(trace, torch.float64)
         if trace.dim() == 2:
             trace = trace[:, None, :]
+        if trace.dim() != 3 or trace.shape[0] != self.macros or trace.shape[2] != self.m or trace.shape[1] < 1:
+            raise ValueError(f"trace must have shape ({self.macros}, cases, {self.m}), got {tuple(trace.shape)}")
         cases = trace.shape[1]
         co = None
         if self.k:
-            co = self._dev(coarse, torch.float64).reshape(self.macros, cases, self.k)
+            if coarse is None:
+                raise ValueError("coarse amplitudes are required with a retained mode")
+            co = self._dev(coarse, torch.float64)
+            if co.dim() == 2 and self.k == 1:  # (M, cases) amplitudes of the one retained mode
+                co = co[:, :, None]
+            if tuple(co.shape) != (self.macros, cases, self.k):
+                raise ValueError(f"coarse must have shape ({self.macros}, {cases}, {self.k}), got {tuple(co.shape)}")
         fields = torch.empty((self.macros, cases, self.n), dtype=torch.float64, device=self.device)
         _lib.call("mhm_batch_reconstruct", self.handle, _ptr(trace.contiguous()), _ptr(co), _ptr(fields),
                   int(cases), _stream())
@@ -216,8 +256,15 @@ class LocalBatch:
         _lib.call("mhm_batch_iterations", self.handle, out.ctypes.data_as(ctypes.c_void_p), _stream())
         return out
 
+    def _raise_coefficients(self, st):
+        bad = np.flatnonzero(st & _lib.STATUS_COEFFICIENT)
+        if len(bad):
+            raise ValueError("diffusion must be finite and positive and the source finite at every quadrature "
+                             f"point: macros {bad[:8].tolist()}" + ("..." if len(bad) > 8 else ""))
+
     def raise_on_failure(self, what: str):
         st = self.status()
+        self._raise_coefficients(st)
         if st.any():
             bad = np.flatnonzero(st)
             kinds = []

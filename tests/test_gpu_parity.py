@@ -238,13 +238,14 @@ def test_golden_npz_through_condense_batched():
         assert np.linalg.norm(pres - ref) / np.linalg.norm(ref) < TOL
 
 
-def test_batched_factorization_resident_solves(config1):
+def test_local_factorization_resident_solves(config1):
+    """Resident band factors: more columns than max_columns (solved in chunks)
+    and a nonzero constraint-row right-hand side (the bordered row's beta)."""
     case, ref = config1
     problems = [p for _, p, _ in ref["problems"]][:6]
-    with batched.BatchedFactorization(problems, max_columns=4) as fac:
-        rhs = np.zeros((6, problems[0].n + 1, 3))
+    with batched.LocalFactorization(problems, max_columns=4) as fac:
         rng = np.random.default_rng(3)
-        rhs[:, :-1, :] = rng.normal(size=(6, problems[0].n, 3))
+        rhs = rng.normal(size=(6, problems[0].n + 1, 7))
         x = fac.solve(rhs, host=True)
     for c, p in enumerate(problems):
         s = p.condensation_matrix().toarray()
