@@ -288,6 +288,21 @@ int mhm_dense_factor_solve(int32_t device, int32_t batch, int32_t n, int32_t nrh
                            const double *d_rhs, double *d_x, double rtol, double atol, int32_t *d_status,
                            void *stream);
 
+/* mhm_dense_factor_solve without any device allocation: no counterpart in the
+ * reference.  For callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * workspace is a fixed per-device array of 64 MiB held until the process ends;
+ * a call whose workspace (mhm_dense_scratch_bytes) exceeds it is rejected with
+ * MHM_EINVAL.  Asynchronous.  Every call on one device shares that array:
+ * calls that overlap on one GPU (different streams or threads) race. */
+int mhm_dense_factor_solve_noalloc(int32_t device, int32_t batch, int32_t n, int32_t nrhs, const double *d_matrices,
+                                   const double *d_rhs, double *d_x, double rtol, double atol, int32_t *d_status,
+                                   void *stream);
+
+/* Workspace bytes of one mhm_dense_factor_solve(_noalloc) call and the
+ * no-allocation limit. */
+int mhm_dense_scratch_bytes(int32_t batch, int32_t n, int32_t nrhs, int64_t *bytes, int64_t *limit);
+
 /* Kernel timing with HIP events recorded on the launch stream (no
  * counterpart in the reference, whose benchmarks time phases with
  * perf_counter wall clocks, benchmarks/execution_modes.py:227-258).

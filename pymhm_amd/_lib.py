@@ -29,6 +29,7 @@ EXPORTS = (
     "mhm_batch_iterations", "mhm_scatter_coordinates", "mhm_ordered_gather",
     "mhm_timing_enable", "mhm_timing_reset", "mhm_timing_read",
     "mhm_dense_create", "mhm_dense_solve", "mhm_dense_destroy", "mhm_dense_factor_solve",
+    "mhm_dense_factor_solve_noalloc", "mhm_dense_scratch_bytes",
 )
 TIMERS = {"assemble": 0, "factor": 1, "solve": 2, "residual": 3, "condense": 4}
 
@@ -103,6 +104,8 @@ def load(path: str | os.PathLike | None = None):
         "mhm_dense_solve": (P, P, I32, D, D, P, P, P),
         "mhm_dense_destroy": (P,),
         "mhm_dense_factor_solve": (I32, I32, I32, I32, P, P, P, D, D, P, P),
+        "mhm_dense_factor_solve_noalloc": (I32, I32, I32, I32, P, P, P, D, D, P, P),
+        "mhm_dense_scratch_bytes": (I32, I32, I32, P, P),
     }
     for name in EXPORTS:
         fn = getattr(lib, name)
