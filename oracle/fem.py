@@ -10,6 +10,8 @@ Restates, with closed-form Lagrange polynomials instead of Basix:
 * ``scalar_operators``        ``src/pymhm/fem/scalar/triangle.py:208-257``
 * ``trace_coupling``          ``src/pymhm/fem/scalar/triangle.py:160-205``
 * ``FaceSpace``/``SkeletonSpace`` ``src/pymhm/fem/traces/interval.py:13-161``
+* ``boundary_data``           ``src/pymhm/fem/scalar/operators.py:247-321`` (scalar
+  Dirichlet data), with ``boundary_moments`` ``fem/scalar/_integration.py:92-108``
 
 Basix is not installed; Lagrange values are the textbook product formula on
 the equispaced barycentric lattice, evaluated in the reference node order.
@@ -260,6 +262,20 @@ class FaceSpace:
         return out
 
 
+    def quadrature(self, order: int = 4):
+        """Gauss points and weights per segment on [0, 1] (``interval.py:92-118``):
+        the reference rule is renormalised to mass two in extended precision,
+        then mapped onto every segment."""
+        x, w = leggauss(int(order))
+        w = np.asarray(w, dtype=np.longdouble)
+        w *= np.longdouble(2) / np.sum(w)
+        pts, wts = [], []
+        for lo, hi in zip(self.breaks[:-1], self.breaks[1:]):
+            pts.extend(lo + (x + 1) * (hi - lo) / 2)
+            wts.extend(w * (hi - lo) / 2)
+        return np.asarray(pts), np.asarray(wts, dtype=float)
+
+
 def legendre(x, degree: int):
     """Conventional Legendre ``L_0..L_degree`` by Bonnet's recurrence."""
     x = np.asarray(x, dtype=float)
@@ -321,3 +337,35 @@ def trace_coupling(coarse, cell: int, fine, skeleton, degree: int):
                 mat[ids, off:off + space.size] += basis.T @ (wts[:, None] * space.evaluate(par))
         off += space.size
     return mat
+
+
+def boundary_moments(basis, weights, values):
+    """Neumaier-compensated ``sum_q basis[q, i] weights[q] values[q]``
+    (``fem/scalar/_integration.py:92-108``, one scalar component)."""
+    out = np.zeros(basis.shape[1])
+    for i in range(basis.shape[1]):
+        total, corr = 0.0, 0.0
+        for q in range(len(weights)):
+            total, corr = _neumaier(total, corr, basis[q, i] * weights[q] * values[q])
+        out[i] = total + corr
+    return out
+
+
+def boundary_data(skeleton, dirichlet, order: int = 5):
+    """Weak Dirichlet trace load of a scalar skeleton (``operators.py:247-321``
+    without Neumann faces): on every boundary face, ``load[dofs] = |face| *
+    sum_q mu(t_q) w_q g(x_q)`` with the face's own segment quadrature of order
+    ``max(order, max degree + 2)``, ``x_q`` on the face from its stored start
+    to its stored end.  Returns the load over the whole skeleton."""
+    mesh = skeleton.mesh
+    load = np.zeros(skeleton.size)
+    for face in mesh.boundary_faces:
+        space = skeleton.faces[face]
+        par, wts = space.quadrature(max(order, max(space.degrees) + 2))
+        start, end = mesh.points[mesh.faces[face]]
+        pts = start + par[:, None] * (end - start)
+        vals = np.broadcast_to(np.asarray(dirichlet(pts) if callable(dirichlet) else dirichlet, dtype=float),
+                               (len(pts),))
+        moments = boundary_moments(space.evaluate(par), wts, vals)
+        load[skeleton.dofs(int(face))] = moments * mesh.lengths[face]
+    return load
