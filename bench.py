@@ -111,8 +111,30 @@ def _cpu_worker_init():
     import oracle.workloads  # noqa: F401
 
 
+CPU_REPEATS = 3
+
+
+def cpu_cores():
+    """Worker processes of the CPU baseline: every core this process may run
+    on (the reference's policy, ``execution/cpu.py:25-82``), at most 16 - the
+    CPU share one GPU's process gets on the GPU host."""
+    return max(1, min(16, len(os.sched_getaffinity(0))))
+
+
+def timed_pool(pool, fn, specs, repeats=CPU_REPEATS):
+    """``repeats`` timed passes of ``pool.map(fn, specs)``: median wall, (min,
+    max), and the last pass's results."""
+    walls, results = [], None
+    for _ in range(repeats):
+        t0 = time.perf_counter()
+        results = list(pool.map(fn, specs))
+        walls.append(time.perf_counter() - t0)
+    return float(np.median(walls)), (min(walls), max(walls)), results
+
+
 def cpu_baseline(case, sample_macros, cores):
-    """Oracle (FP64 NumPy/SciPy restatement) on a bounded sample, process pool."""
+    """Oracle (FP64 NumPy/SciPy restatement) on a bounded sample, process pool,
+    one warm-up and CPU_REPEATS timed passes (median and range)."""
     from oracle import workloads as ow
 
     spec_of = ow.cube_spec if hasattr(case.problem, "split") else ow.grid_spec
@@ -121,16 +143,14 @@ def cpu_baseline(case, sample_macros, cores):
     with ProcessPoolExecutor(max_workers=cores, initializer=_cpu_worker_init) as pool:
         # warm-up: each worker builds its macro mesh once (mesh-only, untimed)
         list(pool.map(ow.condense_spec, [spec_of(case, [int(c[0])]) for c in chunks]))
-        t0 = time.perf_counter()
-        results = list(pool.map(ow.condense_spec, specs))
-        wall = time.perf_counter() - t0
+        wall, span, results = timed_pool(pool, ow.condense_spec, specs)
     blocks = np.zeros((len(sample_macros),) + results[0][1].shape[1:])
     loads = np.zeros((len(sample_macros),) + results[0][2].shape[1:])
     pos = {m: i for i, m in enumerate(sample_macros)}
     for macros, blk, ld in results:
         for r, m in enumerate(macros):
             blocks[pos[m]], loads[pos[m]] = blk[r], ld[r]
-    return wall, blocks, loads
+    return wall, span, blocks, loads
 
 
 def rel_err(a, b):
@@ -245,7 +265,7 @@ def run_config5(args):
     frac, count, cols, pool = 0.3, args.requests, 8, 16
     cpu = None
     if not args.no_cpu:
-        cores = max(1, min(16, len(os.sched_getaffinity(0))))
+        cores = cpu_cores()
         value, note = serving_cpu_baseline(cases, cores, 32, frac)
         cpu = dict(value=value, unit="requests/s", cores=cores, kind="port",
                    sample=f"per-macro oracle costs on 32 macros of each problem ({note}); a request "
@@ -323,21 +343,20 @@ def run_config4(args):
     sample = np.unique(np.linspace(0, M - 1, min(args.cpu_sample or 256, M)).astype(np.int64)).tolist()
     cpu, oref = None, None
     if not args.no_cpu:
-        cores = max(1, min(16, len(os.sched_getaffinity(0))))
+        cores = cpu_cores()
         chunks = [c for c in np.array_split(np.asarray(sample), cores) if len(c)]
         specs = [ow.polygon_spec(case, c.tolist()) for c in chunks]
         with ProcessPoolExecutor(max_workers=cores, initializer=_cpu_worker_init) as pool:
             list(pool.map(ow.condense_spec, [ow.polygon_spec(case, [int(c[0])]) for c in chunks]))
-            t1 = time.perf_counter()
-            results = list(pool.map(ow.condense_spec, specs))
-            wall = time.perf_counter() - t1
+            wall, span, results = timed_pool(pool, ow.condense_spec, specs)
         oref = {}
         for macros, blk, ld in results:
             for r, c in enumerate(macros):
                 oref[c] = (blk[r], ld[r])
         cpu = dict(value=len(sample) / wall, unit="macro solves/s", cores=cores, kind="port",
                    sample=f"{len(sample)} of {M} macros of the same workload, assembly+SuperLU condensation, "
-                          f"{cores}-process pool, wall {wall:.2f}s")
+                          f"{cores}-process pool, median wall of {CPU_REPEATS} passes {wall:.2f}s "
+                          f"(range {span[0]:.2f}-{span[1]:.2f}s)")
         print(f"[bench] cpu baseline {cpu['value']:.1f} macros/s on {cores} cores", file=sys.stderr, flush=True)
     res = {}
     paths = [p for p in args.paths.split(",") if p]
@@ -457,11 +476,12 @@ def main(argv=None):
     cpu = None
     sample = np.unique(np.linspace(0, M - 1, min(args.cpu_sample or wl["sample"], M)).astype(np.int64)).tolist()
     if not args.no_cpu:
-        cores = max(1, min(16, len(os.sched_getaffinity(0))))
-        wall, oblocks, oloads = cpu_baseline(case, sample, cores)
+        cores = cpu_cores()
+        wall, span, oblocks, oloads = cpu_baseline(case, sample, cores)
         cpu = dict(value=len(sample) / wall, unit="macro solves/s", cores=cores, kind="port",
                    sample=f"{len(sample)} of {M} macros of the same workload, assembly+SuperLU "
-                          f"condensation, {cores}-process pool, wall {wall:.2f}s")
+                          f"condensation, {cores}-process pool, median wall of {CPU_REPEATS} passes "
+                          f"{wall:.2f}s (range {span[0]:.2f}-{span[1]:.2f}s)")
         print(f"[bench] cpu baseline {cpu['value']:.1f} macros/s on {cores} cores", file=sys.stderr, flush=True)
 
     if args.copy_bytes > 0:
@@ -504,13 +524,20 @@ def main(argv=None):
     dominant = max(("factor", "solve"), key=lambda k: times[k][0])
     tot_ms, launches = times[dominant]
     avg_s = tot_ms / launches / 1e3
+    tdt = "f64" if path == "fp64" else "f32"
     if dominant == "factor":
+        # the band Cholesky is priced in flops against the matrix-core peak of
+        # its precision (n b (b + 1) per macro, the algorithm executed); its
+        # HBM view (the minimal bytes) is reported beside it
         units = M  # every launch factors every macro
         per_launch_bytes = units * traffic["factor_bytes"]
+        achieved = units * traffic["factor_flops"] / avg_s / 1e12
+        bound, peak, unit = "mfma", PEAK_TFLOPS[tdt], "TFLOP/s"
     else:
         units = head["macro_solves"] / launches  # macros actually solved per launch
         per_launch_bytes = units * traffic["solve_bytes"]
-    achieved = per_launch_bytes / avg_s / 1e9
+        achieved = per_launch_bytes / avg_s / 1e9
+        bound, peak, unit = "hbm", HBM_PEAK_GBS, "GB/s"
     tname = "double" if path == "fp64" else "float"
     lay = prob.layout
     kname = (factor_kernel(lay.bandwidth, tname) if dominant == "factor"
@@ -547,10 +574,12 @@ def main(argv=None):
         "mixed_vs_fp64": (res["fp64"]["elapsed"] / res["mixed"]["elapsed"]) if len(res) == 2 else None,
         "rhs_solves_per_s": value * (1 + prob.m + 1),
         "roofline": {
-            "kernel": kname, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
+            "kernel": kname, "bound": bound, "achieved": achieved, "peak": peak, "unit": unit,
+            "frac": achieved / peak,
             "traffic": (MEASURED_TRAFFIC_PER_MACRO[kname] * units
                         if args.workload == "config2" and kname in MEASURED_TRAFFIC_PER_MACRO else None),
+            "algorithmic_bytes": per_launch_bytes,
+            "hbm_frac_of_algorithmic_bytes": per_launch_bytes / avg_s / 1e9 / HBM_PEAK_GBS,
             "units_per_launch": units, "avg_launch_ms": avg_s * 1e3,
         },
         "cpu_baseline": cpu,

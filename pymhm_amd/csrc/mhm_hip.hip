@@ -789,30 +789,42 @@ __global__ void __launch_bounds__(NT)
 
 // ----------------------------------------------------------------- blocked band Cholesky
 // Block per macro, panels of FPANEL columns.  The LDS window holds the
-// NB + b active band columns (slot c mod S at win[slot * ldw], entries
-// r = 0..b of column c at rows c + r).  Per panel:
-//  (a) wavefront 0 factors the panel in registers: lane q holds panel rows q
-//      and q + 64 (PR = NB + b <= 127 rows), column t is pivoted, scaled and
-//      applied to the panel columns to its right; the pivot and the
-//      multipliers L(j0 + u, j0 + t) are read from their lanes with
-//      v_readlane (no LDS, no barrier), 1/sqrt by the hardware estimate and
-//      Newton steps.  The step is branch-free: loads at clamped addresses with
-//      selects, a failed pivot flagged and replaced by 1, the columns of a
-//      short last panel padded with identity columns (which leave the others
-//      unchanged).  The panel goes to LDS transposed, LpT[t * ldq + q] =
-//      L(j0 + q, j0 + t), rows PR .. ldq - 1 zero;
-//  (b) the panel's columns go to HBM as the column band (contiguous) and the
-//      row band (rband[R*W + r] = L(R, R-r), runs of NB per row), and with
-//      dblk set its 16 x 16 diagonal block, packed (tri16), to
-//      dblk[(mac * nblocks + j0 / 16) * DBLK];
-//  (c) the rank-NB update of the b trailing columns on the matrix cores:
-//      16 x 16 tiles of the lower triangle, round-robin over the wavefronts,
-//      C = C - L_I L_J^T in four 16x16x4 MFMA steps with the C fragment read
-//      from and written back to the window (ldw makes those accesses
-//      conflict-free in binary64, ldq the operand reads);
-//  (d) the NB columns that enter next, gathered by wavefronts 1-3 during (a),
-//      take the panel's slots.
-// Three block barriers per panel instead of two per column.
+// S = max(NB + b, 2 NB) active band columns (slot c mod S at win[slot * ldw],
+// entries r = 0..b of column c at rows c + r; at least two panels, so the next
+// panel is always in the window); the factored panels live in two
+// transposed LDS buffers, LpT[t * ldq + q] = L(j0 + q, j0 + t) (rows PR ..
+// ldq - 1 zero), used alternately.
+// Panel factorisation (one wavefront, in registers): lane q holds panel rows q
+// and q + 64 (PR = NB + b <= 127 rows); left-looking, column t takes the
+// updates of the finished columns k < t with their multipliers L(j0 + t,
+// j0 + k) read back from the panel buffer (LDS broadcast reads, ordered by a
+// wavefront fence), then is pivoted and scaled; the pivot comes from its lane
+// by v_readlane, 1/sqrt by the hardware estimate and Newton steps.  The step
+// is branch-free:
+// loads at clamped addresses with selects, a failed pivot flagged and replaced
+// by 1, the columns of a short last panel padded with identity columns (which
+// leave the others unchanged).
+// One-panel look-ahead, one block barrier per panel.  While panel k is the
+// source of the rank-NB update (C = C - L_I L_J^T over 16 x 16 tiles of the
+// lower triangle, four 16x16x4 MFMA steps per tile, the C fragment read from
+// and written back to the window; ldw makes those accesses conflict-free in
+// binary64, ldq the operand reads):
+//  - one wavefront (k mod 4, so the serial steps of the blocks sharing a CU
+//    spread over its SIMDs) applies the tiles of the first tile column - the
+//    columns of panel k + 1 - and then factors panel k + 1 into the other
+//    buffer, so the serial panel step of k + 1 overlaps the rest of k's update;
+//  - the other three gather the NB columns entering after panel k (issued
+//    first, consumed last), apply the other tile columns, write panel k to
+//    HBM (the column band, contiguous; the row band, rband[R*W + r] =
+//    L(R, R-r), runs of NB per row; with dblk set its 16 x 16 diagonal block,
+//    packed (tri16), to dblk[(mac * nblocks + j0 / 16) * DBLK]) and move the
+//    entering columns into panel k's window slots, which nobody reads after
+//    panel k was factored.
+// The tiles of panel k + 1's columns are updated by every earlier panel before
+// it is factored: panel k's by the serial wavefront in this step, earlier
+// panels' by the other wavefronts in earlier steps.  Barriers order LDS only: no thread reads
+// back another's HBM stores, and each wavefront waits for its own gather
+// loads where it uses them.
 #define FPANEL 16
 // A 16 x 16 lower-triangular block, packed by rows: entry (rho, sigma), sigma
 // <= rho, at rho (rho + 1) / 2 + sigma.
@@ -849,18 +861,19 @@ static int panel_ldw(int b, size_t tsize) {
   return w;
 }
 
-// LpT row stride: >= 128 rows (the two rows per lane of wavefront 0) and
-// >= NB + 16 * ceil(b / 16) (the tiles), = 16 mod 32 so the four 16-lane
-// groups of an operand read use disjoint banks.
+// LpT row stride: >= NB + 16 * ceil(b / 16) rows (the tiles; the panel rows
+// PR = NB + b are fewer), = 16 mod 32 so the four 16-lane groups of an operand
+// read use disjoint banks.
 static int panel_ldq(int b) {
-  int q = std::max(128, FPANEL + 16 * ((b + 15) / 16));
+  int q = FPANEL + 16 * ((b + 15) / 16);
   while ((q & 31) != 16) ++q;
   return q;
 }
 
+// The window and the two panel buffers.
 static size_t factor_panel_lds(int b, size_t tsize) {
-  const size_t S = (size_t)FPANEL + b;
-  return (S * panel_ldw(b, tsize) + (size_t)FPANEL * panel_ldq(b)) * tsize;
+  const size_t S = std::max((size_t)FPANEL + b, (size_t)2 * FPANEL);
+  return (S * panel_ldw(b, tsize) + 2 * (size_t)FPANEL * panel_ldq(b)) * tsize;
 }
 
 static int lds_limit(int device);
@@ -878,10 +891,11 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
   typedef typename Mfma16<T>::v4 V4;
   extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
   const int W = b + 1;
-  const int S = NB + b;   // window columns
-  const int PR = NB + b;  // panel rows
+  const int S = max(NB + b, 2 * NB);  // window columns: the next panel is always inside
+  const int PR = NB + b;              // panel rows
   T *win = reinterpret_cast<T *>(smem_raw);  // S x ldw
-  T *LpT = win + (size_t)S * ldw;            // NB x ldq
+  T *LpT0 = win + (size_t)S * ldw;           // NB x ldq, panels 0, 2, 4, ...
+  T *LpT1 = LpT0 + (size_t)NB * ldq;         // panels 1, 3, 5, ...
   __shared__ int bad;
   const int mac = blockIdx.x, tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   T *Lg = band + (size_t)mac * n * W;
@@ -898,7 +912,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
     const int q = slot_of(c, r);
     win[c * ldw + r] = q >= 0 ? sv[q] : T(0);
   }
-  for (int e = tid; e < NB * ldq; e += 256) LpT[e] = T(0);
+  for (int e = tid; e < 2 * NB * ldq; e += 256) LpT0[e] = T(0);
   if (Rg)  // entries L(j, j-r) with j - r < 0
     for (int e = tid; e < min(b, n) * W; e += 256) {
       const int j = divW(e), r = e - j * W;
@@ -907,60 +921,153 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
   const int nt = (b + 15) / 16;  // tile rows / columns of the trailing block
   const int ntile = nt * (nt + 1) / 2;
   const int j16 = lane & 15, g16 = lane >> 4;
+
+  // Factor the panel of columns jp .. jp + NB - 1 (one wavefront) into Lp,
+  // left-looking: column t takes the updates of the finished columns k < t -
+  // the multipliers L(jp + t, jp + k), k < t - 1, read back from Lp one step
+  // ahead (LDS broadcast reads off the critical path), L(jp + t, jp + t - 1)
+  // from its lane by v_readlane - and is then pivoted and scaled; the pivot
+  // also comes by v_readlane.  A wavefront's LDS accesses execute in order, so
+  // a row stored in one step is read back two steps later without a fence.
+  auto factor_panel = [&](int jp, T *Lp) {
+    const int sp = jp % S;
+    const int np = min(NB, n - jp);
+    T a0[NB], a1[NB];
+#pragma unroll
+    for (int t = 0; t < NB; ++t) {
+      int slot = sp + t;
+      slot = slot >= S ? slot - S : slot;
+      const T *col = win + slot * ldw;
+      // a laundered lane index: the masks are formed here, not hoisted out of
+      // the panel loop as SGPR pairs
+      int qa = lane;
+      asm volatile("" : "+v"(qa));
+      const int qb = qa + 64;
+      const T v0 = col[min(max(qa - t, 0), b)];
+      const T v1 = col[min(qb - t, b)];
+      const bool pad = t >= np;  // identity column of a short last panel
+      a0[t] = pad ? (qa == t ? T(1) : T(0)) : (qa >= t && qa - t <= b ? v0 : T(0));
+      a1[t] = !pad && qb < PR && qb - t <= b ? v1 : T(0);
+    }
+    bool failed = false;
+    T m[NB];   // m[k] = L(jp + t, jp + k), k < t - 1, for the current column t
+    T mn[NB];  // the same for column t + 1, read during step t
+#pragma unroll
+    for (int t = 0; t < NB; ++t) {
+      // the next column's older multipliers (rows stored two or more steps
+      // back), issued first so the reads overlap this step's pivot chain
+#pragma unroll
+      for (int k = 0; k < t && t + 1 < NB; ++k) mn[k] = Lp[k * ldq + t + 1];
+      // two partial sums halve the dependent chain of the updates
+      T s0a = T(0), s0b = T(0), s1a = T(0), s1b = T(0);
+#pragma unroll
+      for (int k = 0; k + 1 < t; ++k) {
+        if (k & 1) {
+          s0b += a0[k] * m[k];
+          s1b += a1[k] * m[k];
+        } else {
+          s0a += a0[k] * m[k];
+          s1a += a1[k] * m[k];
+        }
+      }
+      a0[t] -= s0a + s0b;
+      a1[t] -= s1a + s1b;
+      if (t > 0) {
+        const T l = lane_bcast(a0[t - 1], t);  // L(jp + t, jp + t - 1), held by lane t
+        a0[t] -= a0[t - 1] * l;
+        a1[t] -= a1[t - 1] * l;
+      }
+      T d = lane_bcast(a0[t], t);  // pivot, held by lane t
+      const bool ok = d > T(tiny) && isfinite((double)d);
+      failed |= !ok;
+      d = ok ? d : T(1);
+      const T inv = rsqrt_nr(d);
+      const T root = d * inv;
+      int qt = lane;  // this step's masks are formed here, not hoisted
+      asm volatile("" : "+v"(qt));
+      a0[t] = qt > t ? a0[t] * inv : (qt == t ? root : T(0));
+      a1[t] = a1[t] * inv;  // q1 >= 64 > t
+      // rows PR .. ldq - 1 stay zero from the start (ldq may be below 128)
+      if (qt < PR) Lp[t * ldq + qt] = a0[t];
+      if (qt + 64 < PR) Lp[t * ldq + qt + 64] = a1[t];
+#pragma unroll
+      for (int k = 0; k < t && t + 1 < NB; ++k) m[k] = mn[k];
+    }
+    if (failed && lane == 0) bad = MHM_STATUS_PIVOT;
+  };
+
+  // Up to 4 tiles (I[c], J[c]) of the rank-NB update by the panel at j0 (slot
+  // s0) in Lp, as independent MFMA chains: all C fragments and operands are
+  // read first, then the steps of the tiles interleave.  Tile (I, J) covers
+  // columns qc0 = NB + 16 J, rows qr0 = NB + 16 I of the window.
+  auto update_tiles = [&](const int (&I)[4], const int (&J)[4], int count, int s0, const T *Lp) {
+    V4 acc[4];
+    T *colp[4];
+    T av[4][4], bv[4][4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      if (c >= count) break;
+      const int qr0 = NB + 16 * I[c], qc0 = NB + 16 * J[c];
+      const int qc = qc0 + j16;  // this lane's column
+      int slot = s0 + qc;
+      slot = slot >= S ? slot - S : slot;
+      slot = slot >= S ? slot - S : slot;
+      colp[c] = win + slot * ldw;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int qr = qr0 + Mfma16<T>::row(r, g16);
+        const bool in = qr >= qc && qr < PR && qc < PR;
+        acc[c][r] = colp[c][in ? qr - qc : 0];
+      }
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        av[c][k] = -Lp[(4 * k + g16) * ldq + qr0 + j16];  // A[i][kk] = -L(j0 + qr0 + i, j0 + 4k + kk)
+        bv[c][k] = Lp[(4 * k + g16) * ldq + qc0 + j16];   // B[kk][j] = L(j0 + qc0 + j, j0 + 4k + kk)
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+#pragma unroll
+      for (int c = 0; c < 4; ++c)
+        if (c < count) acc[c] = Mfma16<T>::step(av[c][k], bv[c][k], acc[c]);
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      if (c >= count) break;
+      const int qr0 = NB + 16 * I[c], qc = NB + 16 * J[c] + j16;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int qr = qr0 + Mfma16<T>::row(r, g16);
+        if (qr >= qc && qr < PR && qc < PR) colp[c][qr - qc] = acc[c][r];
+      }
+    }
+  };
+
+  lds_barrier();  // the window holds columns 0 .. S-1
+  if (wave == 0) factor_panel(0, LpT0);
+  lds_barrier();
   T pf[NPF];
-  __syncthreads();
   for (int j0 = 0; j0 < n; j0 += NB) {
     const int s0 = j0 % S;
     const int np = min(NB, n - j0);
-    // (a) panel factorisation, wavefront 0; meanwhile wavefronts 1-3 gather
-    // the columns j0 + S .. j0 + S + NB - 1, which enter after this panel
-    if (wave == 0) {
-      const int q0 = lane, q1 = lane + 64;
-      T a0[NB], a1[NB];
-#pragma unroll
-      for (int t = 0; t < NB; ++t) {
-        int slot = s0 + t;
-        slot = slot >= S ? slot - S : slot;
-        const T *col = win + slot * ldw;
-        const T v0 = col[min(max(q0 - t, 0), b)];
-        const T v1 = col[min(q1 - t, b)];
-        const bool pad = t >= np;  // identity column of a short last panel
-        a0[t] = pad ? (q0 == t ? T(1) : T(0)) : (q0 >= t && q0 - t <= b ? v0 : T(0));
-        a1[t] = !pad && q1 < PR && q1 - t <= b ? v1 : T(0);
-      }
-      bool failed = false;
-#pragma unroll
-      for (int t = 0; t < NB; ++t) {
-        T d = lane_bcast(a0[t], t);  // pivot, held by lane t
-        const bool ok = d > T(tiny) && isfinite((double)d);
-        failed |= !ok;
-        d = ok ? d : T(1);
-        const T inv = rsqrt_nr(d);
-        const T root = d * inv;
-        a0[t] = q0 > t ? a0[t] * inv : (q0 == t ? root : T(0));
-        a1[t] = a1[t] * inv;  // q1 >= 64 > t
-        LpT[t * ldq + q0] = a0[t];
-        LpT[t * ldq + q1] = a1[t];
-#pragma unroll
-        for (int u = t + 1; u < NB; ++u) {
-          const T l = lane_bcast(a0[t], u);  // L(j0 + u, j0 + t), held by lane u
-          a0[u] -= a0[t] * l;
-          a1[u] -= a1[t] * l;
-        }
-      }
-      if (failed && lane == 0) bad = MHM_STATUS_PIVOT;
-    } else {
-      // all slot indices, then all values: unconditional loads at clamped
-      // positions, masked afterwards (a branch per load waits for each load,
-      // the previous value's included)
+    T *Lp = ((j0 / NB) & 1) ? LpT1 : LpT0;  // panel k (this step's update source)
+    T *Ln = ((j0 / NB) & 1) ? LpT0 : LpT1;  // panel k + 1 (factored during this step)
+    // the serial role moves to the next wavefront every panel, so the serial
+    // steps of the blocks sharing a CU spread over its four SIMDs
+    const int pw = (j0 / NB) & 3;
+    if (wave != pw) {
+      // gather the columns j0 + S .. j0 + S + NB - 1, which enter after this
+      // step: all slot indices, then all values, unconditional loads at
+      // clamped positions masked afterwards (a branch per load waits for each)
+      // entering element e = t W + r is band position (j0 + S + t, r), at
+      // bslot[(j0 + S) W + e]; past row n - 1 the map holds -1
+      const int u = ((wave - pw - 1) & 3) * 64 + lane;
       int qv[NPF];
+      const int gbase = (j0 + S) * W;
 #pragma unroll
       for (int v = 0; v < NPF; ++v) {
-        const int e = tid - 64 + 192 * v;
-        const int t = divW(e);
-        const int j = j0 + S + t, r = e - t * W;
-        const bool in = e < NB * W && j < n && j + r < n;
-        const int raw = global_ptr(bslot)[in ? (size_t)j * W + r : 0];
+        const int e = u + 192 * v;
+        const bool in = e < NB * W && gbase + e < n * W;
+        const int raw = global_ptr(bslot)[in ? (size_t)gbase + e : 0];
         qv[v] = in ? raw : -1;
       }
 #pragma unroll
@@ -969,67 +1076,65 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
         pf[v] = qv[v] >= 0 ? val : T(0);
       }
     }
-    __syncthreads();
-    // (b) the panel to HBM: column band (contiguous) and row band
-    for (int e = tid; e < np * W; e += 256) {
-      const int t = divW(e), r = e - t * W;
-      Lg[(size_t)j0 * W + e] = LpT[t * ldq + t + r];
-    }
-    if (Rg)
-      for (int e = tid; e < PR * NB; e += 256) {
-        const int q = e / NB, t = e - q * NB;
-        const int R = j0 + q, r = q - t;
-        if (t < np && r >= 0 && r <= b && R < n) Rg[(size_t)R * W + r] = LpT[t * ldq + q];
+    if (wave == pw) {
+      // the next panel's columns first, then the next panel itself
+      for (int i0 = 0; i0 < nt; i0 += 4) {
+        const int I[4] = {i0, i0 + 1, i0 + 2, i0 + 3}, J[4] = {0, 0, 0, 0};
+        update_tiles(I, J, min(4, nt - i0), s0, Lp);
       }
-    if (dblk && (tid & 15) <= (tid >> 4))  // the diagonal block, packed (identity-padded past row n - 1)
-      dblk[((size_t)mac * ((n + NB - 1) / NB) + j0 / NB) * DBLK + tri16(tid >> 4, tid & 15)] =
-          LpT[(tid & 15) * ldq + (tid >> 4)];
-    // (c) trailing update on the matrix cores: tile k = (I, J), J <= I < nt,
-    // columns qc0 = NB + 16 J, rows qr0 = NB + 16 I, for wavefront k mod 4
-    for (int k = wave; k < ntile; k += 4) {
-      int J = 0, rem = k;
-      while (rem >= nt - J) {
-        rem -= nt - J;
-        ++J;
-      }
-      const int qr0 = NB + 16 * (J + rem), qc0 = NB + 16 * J;
-      const int qc = qc0 + j16;  // this lane's column
-      int slot = s0 + qc;
-      slot = slot >= S ? slot - S : slot;
-      slot = slot >= S ? slot - S : slot;
-      T *colp = win + slot * ldw;
-      V4 acc;
+      if (j0 + NB < n) factor_panel(j0 + NB, Ln);
+    } else {
+      const int ow = (wave - pw - 1) & 3;  // 0 .. 2 among the other wavefronts
+      const int u = ow * 64 + lane;        // 0 .. 191
+      // tile columns J >= 1, round-robin over the other wavefronts, four at a time
+      for (int k0 = ow; k0 < ntile - nt; k0 += 12) {
+        int I[4] = {0, 0, 0, 0}, J[4] = {1, 1, 1, 1}, count = 0;
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int qr = qr0 + Mfma16<T>::row(r, g16);
-        const bool in = qr >= qc && qr < PR && qc < PR;
-        acc[r] = colp[in ? qr - qc : 0];
+        for (int c = 0; c < 4; ++c) {
+          const int k = k0 + 3 * c;
+          if (k < ntile - nt) {
+            int jj = 1, rem = k;
+            while (rem >= nt - jj) {
+              rem -= nt - jj;
+              ++jj;
+            }
+            I[c] = jj + rem;
+            J[c] = jj;
+            count = c + 1;
+          }
+        }
+        update_tiles(I, J, count, s0, Lp);
       }
-#pragma unroll
-      for (int t0 = 0; t0 < NB; t0 += 4) {
-        const T a = -LpT[(t0 + g16) * ldq + qr0 + j16];  // A[i][k] = -L(j0 + qr0 + i, j0 + t0 + k)
-        const T bb = LpT[(t0 + g16) * ldq + qc0 + j16];  // B[k][j] = L(j0 + qc0 + j, j0 + t0 + k)
-        acc = Mfma16<T>::step(a, bb, acc);
-      }
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int qr = qr0 + Mfma16<T>::row(r, g16);
-        if (qr >= qc && qr < PR && qc < PR) colp[qr - qc] = acc[r];
-      }
-    }
-    __syncthreads();
-    // (d) entering columns take the panel's slots
-#pragma unroll
-    for (int v = 0; v < NPF; ++v) {
-      const int e = tid - 64 + 192 * v;
-      if (wave > 0 && e < NB * W) {
+      // panel k to HBM: column band (contiguous) and row band
+      for (int e = u; e < np * W; e += 192) {
         const int t = divW(e), r = e - t * W;
-        int slot = s0 + t;
-        slot = slot >= S ? slot - S : slot;
-        win[slot * ldw + r] = pf[v];
+        Lg[(size_t)j0 * W + e] = Lp[t * ldq + t + r];
+      }
+      if (Rg)
+        for (int e = u; e < PR * NB; e += 192) {
+          const int q = e / NB, t = e - q * NB;
+          const int R = j0 + q, r = q - t;
+          if (t < np && r >= 0 && r <= b && R < n) Rg[(size_t)R * W + r] = Lp[t * ldq + q];
+        }
+      if (dblk)  // the diagonal block, packed (identity-padded past row n - 1)
+        for (int e = u; e < NB * NB; e += 192)
+          if ((e & 15) <= (e >> 4))
+            dblk[((size_t)mac * ((n + NB - 1) / NB) + j0 / NB) * DBLK + tri16(e >> 4, e & 15)] =
+                Lp[(e & 15) * ldq + (e >> 4)];
+      // the entering columns take panel k's slots (read by nobody after panel
+      // k was factored)
+#pragma unroll
+      for (int v = 0; v < NPF; ++v) {
+        const int e = u + 192 * v;
+        if (e < NB * W) {
+          const int t = divW(e), r = e - t * W;
+          int slot = s0 + t;
+          slot = slot >= S ? slot - S : slot;
+          win[slot * ldw + r] = pf[v];
+        }
       }
     }
-    __syncthreads();
+    lds_barrier();
   }
   if (tid == 0 && bad) atomicOr(&status[mac], bad);
 }
