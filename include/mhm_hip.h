@@ -94,6 +94,25 @@ typedef struct mhm_layout_desc {
   const double *vtable;      /* [nq][nb] phi_b                                    */
 } mhm_layout_desc;
 
+/* Macroface skeleton of a 2D layout (HOST arrays, copied): the boundary edges
+ * of the local mesh cut into pieces that each lie inside one face segment, in
+ * the side's counterclockwise fraction f in [0, 1], and the nonzero coupling
+ * entries (band row, side, segment along f) with the pieces and local edge
+ * basis functions (0 start, 1 end, 2 middle node of a P2 edge) that sum into
+ * each.  The face space has nseg segments of one Legendre degree with breaks
+ * symmetric about 1/2 (the second owner of a face sees them mirrored). */
+typedef struct mhm_skeleton_desc {
+  int32_t nsides, nseg, degree, edeg, ng;
+  const double *breaks;      /* [nseg+1]                                          */
+  const double *gx, *gw;     /* [ng] Gauss-Legendre nodes and weights on [-1, 1]  */
+  int32_t npieces;
+  const double *piece_f;     /* [npieces][4]: f0, f1 of the piece, e0, e1 of its edge */
+  int32_t nentries;
+  const int32_t *entry_row, *entry_side, *entry_seg; /* [nentries]                 */
+  const int32_t *entry_ptr;  /* [nentries+1]                                      */
+  const int32_t *contrib;    /* [entry_ptr[nentries]] piece * 3 + local basis      */
+} mhm_skeleton_desc;
+
 /* Diagnostics. */
 int mhm_last_error(char *buf, size_t len);
 int mhm_device_count(int32_t *count);
@@ -302,6 +321,44 @@ int mhm_dense_factor_solve_noalloc(int32_t device, int32_t batch, int32_t n, int
 /* Workspace bytes of one mhm_dense_factor_solve(_noalloc) call and the
  * no-allocation limit. */
 int mhm_dense_scratch_bytes(int32_t batch, int32_t n, int32_t nrhs, int64_t *bytes, int64_t *limit);
+
+/* ---------------------------------------------------------------- skeleton and boundary forms */
+
+/* Attach the macroface skeleton to a layout (enables mhm_batch_skeleton_coupling).
+ * Replaces the mesh-side part of trace_coupling / face_integration
+ * (src/pymhm/fem/scalar/triangle.py:160-205, fem/scalar/operators.py:200-244). */
+int mhm_layout_set_skeleton(mhm_layout *layout, const mhm_skeleton_desc *desc);
+
+/* Per-macro couplings computed on the device from the layout's skeleton, the
+ * side lengths d_side_len[macro][nsides] and orientation signs
+ * d_side_sign[macro][nsides] (+1 first owner of the face, -1 second: the face
+ * parameter runs against the side and odd modes flip).  Invalidates factors.
+ * Replaces trace_coupling (fem/scalar/triangle.py:160-205) per macro. */
+int mhm_batch_skeleton_coupling(mhm_batch *batch, const double *d_side_len, const double *d_side_sign, void *stream);
+
+/* Per-macro constraints C = M 1 / (1^T M 1) from the batch's element measures
+ * (per-macro geometry when set).  Invalidates factors.  Replaces
+ * constraints = physical_mean / (kernel^T physical_mean) of the primal Darcy
+ * provider (src/pymhm/_legacy/models/darcy/primal.py:70-72). */
+int mhm_batch_mass_constraints(mhm_batch *batch, void *stream);
+
+/* Copy the effective per-macro coupling d_coupling[macro][m][n] and constraint
+ * d_constraints[macro][n] (ORIGINAL rows; either may be NULL).  Mirror of the
+ * coupling / constraints fields of LocalProblem (core/contracts.py:92-211). */
+int mhm_batch_coupling(mhm_batch *batch, double *d_coupling, double *d_constraints, void *stream);
+
+/* Weak Dirichlet trace load over nfaces boundary faces: for face f and mode a,
+ * load = d_length[f] * sum_q (d_basis[q][a] d_weights[q]) d_g[f][q]
+ * (compensated), written to d_load[d_dof0[f] + a] when d_load is not NULL, and
+ * applied to the global system as d_rhs[row] -= load, d_scale[row] += |load|
+ * (either may be NULL).  d_basis[nq][fs] holds the face modes at the face
+ * quadrature points, d_weights[nq] their weights (sum 1 over the face).
+ * Replaces boundary_data (src/pymhm/fem/scalar/operators.py:247-321, scalar
+ * Dirichlet data) and the boundary-load subtraction of the global assembly
+ * (core/contributions.py:185-195). */
+int mhm_dirichlet_load(int32_t nfaces, int32_t nq, int32_t fs, const double *d_basis, const double *d_weights,
+                       const double *d_g, const double *d_length, const int64_t *d_dof0, double *d_load,
+                       double *d_rhs, double *d_scale, void *stream);
 
 /* Kernel timing with HIP events recorded on the launch stream (no
  * counterpart in the reference, whose benchmarks time phases with

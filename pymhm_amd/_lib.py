@@ -28,6 +28,8 @@ EXPORTS = (
     "mhm_batch_solve", "mhm_batch_sources", "mhm_batch_reconstruct", "mhm_batch_status",
     "mhm_batch_iterations", "mhm_scatter_coordinates", "mhm_ordered_gather",
     "mhm_timing_enable", "mhm_timing_reset", "mhm_timing_read",
+    "mhm_layout_set_skeleton", "mhm_batch_skeleton_coupling", "mhm_batch_mass_constraints",
+    "mhm_batch_coupling", "mhm_dirichlet_load",
     "mhm_dense_create", "mhm_dense_solve", "mhm_dense_destroy", "mhm_dense_factor_solve",
     "mhm_dense_factor_solve_noalloc", "mhm_dense_scratch_bytes",
 )
@@ -40,6 +42,17 @@ class LinearSolveError(RuntimeError):
 
 class SolverUnavailableError(ImportError):
     """The HIP library or a GPU is unavailable (``linalg/linear.py:36-37``)."""
+
+
+class SkeletonDesc(ctypes.Structure):
+    _fields_ = [
+        ("nsides", ctypes.c_int32), ("nseg", ctypes.c_int32), ("degree", ctypes.c_int32),
+        ("edeg", ctypes.c_int32), ("ng", ctypes.c_int32),
+        ("breaks", ctypes.c_void_p), ("gx", ctypes.c_void_p), ("gw", ctypes.c_void_p),
+        ("npieces", ctypes.c_int32), ("piece_f", ctypes.c_void_p),
+        ("nentries", ctypes.c_int32), ("entry_row", ctypes.c_void_p), ("entry_side", ctypes.c_void_p),
+        ("entry_seg", ctypes.c_void_p), ("entry_ptr", ctypes.c_void_p), ("contrib", ctypes.c_void_p),
+    ]
 
 
 class LayoutDesc(ctypes.Structure):
@@ -100,6 +113,11 @@ def load(path: str | os.PathLike | None = None):
         "mhm_timing_enable": (I32,),
         "mhm_timing_reset": (),
         "mhm_timing_read": (I32, P, P),
+        "mhm_layout_set_skeleton": (P, P),
+        "mhm_batch_skeleton_coupling": (P, P, P, P),
+        "mhm_batch_mass_constraints": (P, P),
+        "mhm_batch_coupling": (P, P, P, P),
+        "mhm_dirichlet_load": (I32, I32, I32, P, P, P, P, P, P, P, P, P),
         "mhm_dense_create": (I32, I32, I32, P, P, P, P),
         "mhm_dense_solve": (P, P, I32, D, D, P, P, P),
         "mhm_dense_destroy": (P,),

@@ -279,6 +279,18 @@ struct LayoutK {
   const int *srow;   // row of every CSR slot
 };
 
+// Macroface skeleton of a layout (mhm_layout_set_skeleton): the boundary
+// edges of the local mesh split into pieces that each lie inside one face
+// segment, and the nonzero (band row, side, segment) entries of the coupling
+// with the (piece, local basis) pairs that sum into each.
+struct SkelK {
+  int nsides, nseg, degree, edeg, ng, npieces, nentries;
+  const double *breaks;  // [nseg + 1] in the face parameter (symmetric)
+  const double *pf;      // [npieces][4]: piece f0, f1 and its edge's e0, e1 (side fractions)
+  const double *gx, *gw; // [ng] Gauss-Legendre nodes and weights on [-1, 1]
+  const int *eside, *eseg, *erow, *eptr, *contrib;  // entries and their (piece * 3 + basis) lists
+};
+
 // Per-macro scalars kept in scal[mac * NSCAL + ...].
 enum { S_AMAX = 0, S_SIGMA, S_CS, S_A11, S_A12, S_A21, S_A22, S_DET, NSCAL };
 
@@ -574,6 +586,131 @@ __global__ void __launch_bounds__(128)
     dd_fma(hi, lo, B.aval[(size_t)mac * L.nnz + s], kernel_at(L, B, mac, L.cols[s]));
   B.az[(size_t)mac * L.n + i] = hi + lo;
   B.load[(size_t)mac * L.n + i] = loads_orig[(size_t)mac * L.n + L.perm[i]];
+}
+
+// ----------------------------------------------------------------- skeleton and boundary forms
+// Per-macro coupling B[i, a] = sign_s |side s| int_0^1 phi_i mu_a over the fine
+// boundary edges of macro side s (trace_coupling, fem/scalar/triangle.py:160-205,
+// and face_integration, fem/scalar/operators.py:200-244): thread per (coupling
+// entry, macro).  Each piece of a boundary edge lies inside one face segment; a
+// Gauss-Legendre rule of ng points integrates the nodal function phi (P1 or P2
+// on the edge, in the side's counterclockwise fraction f) against the Legendre
+// modes of the segment in the face parameter t - t = f for the face's first
+// owner, t = 1 - f for the second (sign -1), which mirrors the segment and flips
+// the odd modes.  Column a = s fs + segment(t) (p + 1) + j.
+#define SKEL_MAXMODES 8
+__global__ void __launch_bounds__(256)
+    k_trace_coupling(LayoutK L, SkelK K, const double *__restrict__ side_len, const double *__restrict__ side_sign,
+                     double *__restrict__ bcoup) {
+  const int en = blockIdx.x * blockDim.x + threadIdx.x;
+  const int mac = blockIdx.y;
+  if (en >= K.nentries) return;
+  const int s = K.eside[en], row = K.erow[en];
+  const double len = side_len[(size_t)mac * K.nsides + s];
+  const double sg = side_sign[(size_t)mac * K.nsides + s];
+  const bool fwd = sg > 0.0;
+  const int seg = fwd ? K.eseg[en] : K.nseg - 1 - K.eseg[en];
+  const double lo = K.breaks[seg], hi = K.breaks[seg + 1];
+  double acc[SKEL_MAXMODES];
+#pragma unroll
+  for (int j = 0; j < SKEL_MAXMODES; ++j) acc[j] = 0.0;
+  for (int t = K.eptr[en]; t < K.eptr[en + 1]; ++t) {
+    const int code = K.contrib[t];
+    const int piece = code / 3, bas = code - 3 * piece;
+    const double *pp = K.pf + 4 * (size_t)piece;
+    const double f0 = pp[0], f1 = pp[1], e0 = pp[2], e1 = pp[3];
+    const double half = 0.5 * (f1 - f0);
+    for (int q = 0; q < K.ng; ++q) {
+      const double f = f0 + (K.gx[q] + 1.0) * half;
+      const double u = (f - e0) / (e1 - e0);  // coordinate along the edge, start -> end
+      double phi;
+      if (K.edeg == 1)
+        phi = bas == 0 ? 1.0 - u : u;
+      else
+        phi = bas == 0 ? (1.0 - u) * (1.0 - 2.0 * u) : (bas == 1 ? u * (2.0 * u - 1.0) : 4.0 * u * (1.0 - u));
+      const double tq = fwd ? f : 1.0 - f;
+      const double x = 2.0 * (tq - lo) / (hi - lo) - 1.0;
+      const double wq = K.gw[q] * half * phi;
+      double lm1 = 1.0, lj = x;  // Bonnet's recurrence for the conventional Legendre modes
+      acc[0] += wq;
+#pragma unroll
+      for (int j = 1; j < SKEL_MAXMODES; ++j) {
+        if (j > K.degree) break;
+        acc[j] += wq * lj;
+        const double nxt = ((2 * j + 1) * x * lj - j * lm1) / (j + 1);
+        lm1 = lj;
+        lj = nxt;
+      }
+    }
+  }
+  const int fs = K.nseg * (K.degree + 1);
+#pragma unroll
+  for (int j = 0; j < SKEL_MAXMODES; ++j)
+    if (j <= K.degree)
+      bcoup[((size_t)mac * L.m + s * fs + seg * (K.degree + 1) + j) * L.n + row] = acc[j] * len * sg;
+}
+
+// C = M 1 / (1^T M 1) per macro from its own element measures (the constraint
+// of the primal Darcy provider, _legacy/models/darcy/primal.py:70-72, with M 1
+// the integral of each basis function): row i sums meas[e] vint[b] over its
+// element contributions in the fixed load order, then every row is divided by
+// the block's fixed-order total.
+__global__ void __launch_bounds__(256)
+    k_mass_constraints(LayoutK L, BatchK B, const double *__restrict__ vint, double *__restrict__ cons) {
+  __shared__ double red[4];
+  const int mac = blockIdx.x;
+  const double *ms = B.meas ? B.meas + (size_t)mac * L.ne : L.measure;
+  double *c = cons + (size_t)mac * L.n;
+  double sum = 0.0;
+  for (int i = threadIdx.x; i < L.n; i += 256) {
+    double v = 0.0;
+    for (int t = L.load_ptr[i]; t < L.load_ptr[i + 1]; ++t) {
+      const int code = L.load_code[t];
+      const int bas = code / L.ne, e = code - bas * L.ne;
+      v += ms[e] * vint[bas];
+    }
+    c[i] = v;
+    sum += v;
+  }
+  sum = block_sum<256>(sum, red);
+  for (int i = threadIdx.x; i < L.n; i += 256) c[i] = c[i] / sum;
+}
+
+// Weak Dirichlet trace load of a scalar skeleton (boundary_data,
+// fem/scalar/operators.py:247-321, without Neumann faces): thread per (boundary
+// face f, mode a), load = |f| sum_q (mu_a(t_q) w_q) g_q with the compensated
+// accumulation of boundary_moments (fem/scalar/_integration.py:92-108).  The
+// global rows take rhs -= load and scale += |load|, as the boundary load enters
+// the hybrid system (core/contributions.py:185-195).
+__global__ void __launch_bounds__(256)
+    k_dirichlet_load(int nfaces, int nq, int fs, const double *__restrict__ basis, const double *__restrict__ w,
+                     const double *__restrict__ g, const double *__restrict__ len, const int64_t *__restrict__ dof0,
+                     double *__restrict__ load, double *__restrict__ rhs, double *__restrict__ scale) {
+#pragma clang fp contract(off)
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)nfaces * fs) return;
+  const int f = (int)(t / fs), a = (int)(t - (long)f * fs);
+  double total = 0.0, corr = 0.0;
+  for (int q = 0; q < nq; ++q) comp_add(total, corr, (basis[q * fs + a] * w[q]) * g[(size_t)f * nq + q]);
+  const double v = (total + corr) * len[f];
+  const int64_t row = dof0[f] + a;
+  if (load) load[row] = v;
+  if (rhs) rhs[row] -= v;
+  if (scale) scale[row] += fabs(v);
+}
+
+// The effective per-macro coupling and constraint in ORIGINAL rows (inspection).
+__global__ void __launch_bounds__(256)
+    k_coupling_out(LayoutK L, BatchK B, double *__restrict__ coup, double *__restrict__ cons) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int mac = blockIdx.y;
+  if (t >= (long)(L.m + 1) * L.n) return;
+  const int a = (int)(t / L.n), i = (int)(t - (long)a * L.n);
+  if (a < L.m) {
+    if (coup) coup[((size_t)mac * L.m + a) * L.n + L.perm[i]] = coupling_at(L, B, mac, a, i);
+  } else if (cons && L.k) {
+    cons[(size_t)mac * L.n + L.perm[i]] = constraint_at(L, B, mac, i);
+  }
 }
 
 // A Z again after new retained modes (same double-double slot-order sums as
@@ -2603,6 +2740,11 @@ __global__ void k_fill_int(int *p, int v, size_t count) {
   if (t < count) p[t] = v;
 }
 
+__global__ void k_fill_f64(double *p, double v, size_t count) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < count) p[t] = v;
+}
+
 // done[mac] = 1 when every column of the macro has converged (its X is final).
 __global__ void k_mark_done(const int *__restrict__ colstate, int ncols, int macros, int *__restrict__ done) {
   const int mac = blockIdx.x * blockDim.x + threadIdx.x;
@@ -2851,6 +2993,9 @@ struct mhm_layout {
   const double *coupling_rm = nullptr;  // template coupling, row-major n x m (band rows)
   std::vector<void *> owned;
   double cmax_host = 0.0;
+  bool has_skel = false;
+  SkelK skel{};
+  const double *vint = nullptr;  // [nb] integral of each reference basis function / |T|
 };
 
 struct mhm_batch {
@@ -3454,8 +3599,64 @@ int mhm_layout_create(int32_t device, const mhm_layout_desc *d, mhm_layout **out
         (rc = upload(L->owned, d->dtable, nq * nb * (dim + 1), &k.dtab)) ||
         (rc = upload(L->owned, d->vtable, nq * nb, &k.vtab)))
       return cleanup(rc);
+    // the integral of each reference basis function over a unit-measure
+    // element, in quadrature order (M 1 = measure * vint per element)
+    std::vector<double> vint(nb, 0.0);
+    for (size_t b = 0; b < nb; ++b)
+      for (size_t q = 0; q < nq; ++q) vint[b] += d->qweights[q] * d->vtable[q * nb + b];
+    if ((rc = upload(L->owned, vint.data(), nb, &L->vint))) return cleanup(rc);
   }
   *out = L;
+  return MHM_OK;
+}
+
+int mhm_layout_set_skeleton(mhm_layout *L, const mhm_skeleton_desc *d) {
+  if (!L || !d) return fail(MHM_EINVAL, "layout/desc is NULL");
+  const LayoutK &k = L->k;
+  if (d->nsides <= 0 || d->nseg <= 0 || d->degree < 0 || d->degree >= SKEL_MAXMODES)
+    return fail(MHM_EINVAL, "skeleton: nsides, nseg >= 1 and 0 <= degree < " + std::to_string(SKEL_MAXMODES));
+  if (d->nsides * d->nseg * (d->degree + 1) != k.m)
+    return fail(MHM_EINVAL, "skeleton: nsides * nseg * (degree + 1) must equal the layout's m");
+  if (d->edeg != 1 && d->edeg != 2) return fail(MHM_EINVAL, "skeleton: edge degree must be 1 or 2");
+  if (d->ng <= 0 || d->ng > 16 || !d->gx || !d->gw) return fail(MHM_EINVAL, "skeleton: 1..16 Gauss points");
+  if (!d->breaks || d->breaks[0] != 0.0 || d->breaks[d->nseg] != 1.0)
+    return fail(MHM_EINVAL, "skeleton: breaks must run from 0 to 1");
+  for (int i = 0; i < d->nseg; ++i) {
+    if (!(d->breaks[i + 1] > d->breaks[i])) return fail(MHM_EINVAL, "skeleton: breaks must increase");
+    if (std::fabs(d->breaks[i] - (1.0 - d->breaks[d->nseg - i])) > 1e-15)
+      return fail(MHM_EINVAL, "skeleton: breaks must be symmetric (the second owner mirrors them)");
+  }
+  if (d->npieces < 0 || d->nentries < 0 || (d->nentries && (!d->entry_ptr || !d->contrib)))
+    return fail(MHM_EINVAL, "skeleton: piece or entry tables missing");
+  for (int e = 0; e < d->nentries; ++e) {
+    if (d->entry_row[e] < 0 || d->entry_row[e] >= k.n || d->entry_side[e] < 0 || d->entry_side[e] >= d->nsides ||
+        d->entry_seg[e] < 0 || d->entry_seg[e] >= d->nseg || d->entry_ptr[e + 1] < d->entry_ptr[e])
+      return fail(MHM_EINVAL, "skeleton: entry out of range");
+    for (int t = d->entry_ptr[e]; t < d->entry_ptr[e + 1]; ++t)
+      if (d->contrib[t] < 0 || d->contrib[t] / 3 >= d->npieces || d->contrib[t] % 3 > d->edeg)
+        return fail(MHM_EINVAL, "skeleton: contribution out of range");
+  }
+  DeviceGuard guard(L->device);
+  SkelK &K = L->skel;
+  K.nsides = d->nsides;
+  K.nseg = d->nseg;
+  K.degree = d->degree;
+  K.edeg = d->edeg;
+  K.ng = d->ng;
+  K.npieces = d->npieces;
+  K.nentries = d->nentries;
+  const size_t ncontrib = d->nentries ? (size_t)d->entry_ptr[d->nentries] : 0;
+  int rc;
+  if ((rc = upload(L->owned, d->breaks, (size_t)d->nseg + 1, &K.breaks)) ||
+      (rc = upload(L->owned, d->piece_f, (size_t)d->npieces * 4, &K.pf)) ||
+      (rc = upload(L->owned, d->gx, (size_t)d->ng, &K.gx)) || (rc = upload(L->owned, d->gw, (size_t)d->ng, &K.gw)) ||
+      (rc = upload(L->owned, d->entry_side, (size_t)d->nentries, &K.eside)) ||
+      (rc = upload(L->owned, d->entry_seg, (size_t)d->nentries, &K.eseg)) ||
+      (rc = upload(L->owned, d->entry_row, (size_t)d->nentries, &K.erow)) ||
+      (rc = upload(L->owned, d->entry_ptr, (size_t)d->nentries + 1, &K.eptr)) ||
+      (rc = upload(L->owned, d->contrib, ncontrib, &K.contrib)))
+    return rc;
+  L->has_skel = true;
   return MHM_OK;
 }
 
@@ -3766,6 +3967,74 @@ int mhm_batch_set_geometry(mhm_batch *b, const double *grad_lambda, const double
   b->bk.glam = b->glam;
   b->bk.meas = b->meas;
   b->factored = b->condensed = false;
+  return MHM_OK;
+}
+
+int mhm_batch_skeleton_coupling(mhm_batch *b, const double *side_len, const double *side_sign, void *stream) {
+  if (!b || !side_len || !side_sign) return fail(MHM_EINVAL, "batch/side_len/side_sign is NULL");
+  const mhm_layout *lay = b->layout;
+  if (!lay->has_skel) return fail(MHM_EINVAL, "the layout has no skeleton description");
+  const LayoutK &L = lay->k;
+  DeviceGuard guard(lay->device);
+  hipStream_t st = (hipStream_t)stream;
+  if (!b->bcoup) {  // entries outside the skeleton pattern stay zero
+    int rc = dev_alloc(b->owned, &b->bcoup, (size_t)b->M * L.m * L.n);
+    if (rc) return rc;
+    const size_t cnt = (size_t)b->M * L.m * L.n;
+    hipLaunchKernelGGL(k_fill_f64, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, st, b->bcoup, 0.0, cnt);
+    LAUNCH_CHECK("k_fill_f64");
+  }
+  if (lay->skel.nentries > 0) {
+    hipLaunchKernelGGL(k_trace_coupling, grid2(lay->skel.nentries, 256, b->M), dim3(256), 0, st, L, lay->skel,
+                       side_len, side_sign, b->bcoup);
+    LAUNCH_CHECK("k_trace_coupling");
+  }
+  b->bk.bcoup = b->bcoup;
+  b->factored = b->condensed = false;
+  return MHM_OK;
+}
+
+int mhm_batch_mass_constraints(mhm_batch *b, void *stream) {
+  if (!b) return fail(MHM_EINVAL, "batch is NULL");
+  const mhm_layout *lay = b->layout;
+  const LayoutK &L = lay->k;
+  if (L.ne <= 0 || !lay->vint) return fail(MHM_EINVAL, "layout has no assembly description");
+  if (!L.k) return fail(MHM_EINVAL, "constraints need a layout with a retained mode");
+  DeviceGuard guard(lay->device);
+  hipStream_t st = (hipStream_t)stream;
+  if (!b->cons) {
+    int rc = dev_alloc(b->owned, &b->cons, (size_t)b->M * L.n);
+    if (rc) return rc;
+  }
+  hipLaunchKernelGGL(k_mass_constraints, dim3(b->M), dim3(256), 0, st, L, b->bk, lay->vint, b->cons);
+  LAUNCH_CHECK("k_mass_constraints");
+  b->bk.cons = b->cons;
+  b->factored = b->condensed = false;
+  return MHM_OK;
+}
+
+int mhm_batch_coupling(mhm_batch *b, double *coupling, double *constraints, void *stream) {
+  if (!b) return fail(MHM_EINVAL, "batch is NULL");
+  const LayoutK &L = b->layout->k;
+  if (L.m > 0 && !L.coupling && !b->bk.bcoup) return fail(MHM_EINVAL, "no coupling set");
+  if (constraints && L.k && !L.constraint && !b->bk.cons) return fail(MHM_EINVAL, "no constraint set");
+  DeviceGuard guard(b->layout->device);
+  hipLaunchKernelGGL(k_coupling_out, grid2((size_t)(L.m + 1) * L.n, 256, b->M), dim3(256), 0, (hipStream_t)stream,
+                     L, b->bk, coupling, constraints);
+  LAUNCH_CHECK("k_coupling_out");
+  return MHM_OK;
+}
+
+int mhm_dirichlet_load(int32_t nfaces, int32_t nq, int32_t fs, const double *basis, const double *weights,
+                       const double *g, const double *length, const int64_t *dof0, double *load, double *rhs,
+                       double *scale, void *stream) {
+  if (nfaces < 0 || nq <= 0 || fs <= 0) return fail(MHM_EINVAL, "nfaces >= 0, nq >= 1 and fs >= 1 required");
+  if (nfaces == 0) return MHM_OK;
+  if (!basis || !weights || !g || !length || !dof0) return fail(MHM_EINVAL, "dirichlet load: NULL input");
+  const size_t work = (size_t)nfaces * fs;
+  hipLaunchKernelGGL(k_dirichlet_load, dim3((unsigned)((work + 255) / 256)), dim3(256), 0, (hipStream_t)stream,
+                     nfaces, nq, fs, basis, weights, g, length, dof0, load, rhs, scale);
+  LAUNCH_CHECK("k_dirichlet_load");
   return MHM_OK;
 }
 

@@ -115,8 +115,11 @@ class HybridScatter:
                   _stream())
         return rows, cols
 
-    def gather(self, blocks, loads):
-        """Global CSC values (pattern before zero elimination), rhs and load scale on the GPU."""
+    def gather(self, blocks, loads, boundary=None):
+        """Global CSC values (pattern before zero elimination), rhs and load scale on the GPU.
+
+        ``boundary`` = ``(DirichletBoundary, g)`` applies the weak Dirichlet
+        load of the data ``g`` on the device (``rhs -= load``, ``scale += |load|``)."""
         blocks = blocks.reshape(-1).contiguous()
         loads = loads.reshape(-1).contiguous()
         vals = torch.empty(len(self.csc_indices), dtype=torch.float64, device=self.device)
@@ -126,11 +129,17 @@ class HybridScatter:
                   int(len(self.csc_indices)), _ptr(vals), None, _stream())
         _lib.call("mhm_ordered_gather", _ptr(self.lptr), _ptr(self.lcode), _ptr(loads), int(self.size),
                   _ptr(rhs), _ptr(scale), _stream())
+        if boundary is not None:
+            bnd, g = boundary
+            if bnd.trace_size != self.trace_size:
+                raise ValueError("the boundary belongs to another skeleton")
+            bnd.load(g, rhs=rhs, scale=scale)
         return vals, rhs, scale
 
-    def assemble(self, blocks, loads, boundary_load=None):
-        """``(matrix, rhs, load_scale)`` as ``assemble_hybrid_contributions`` returns them."""
-        vals, rhs, scale = self.gather(blocks, loads)
+    def assemble(self, blocks, loads, boundary_load=None, boundary=None):
+        """``(matrix, rhs, load_scale)`` as ``assemble_hybrid_contributions`` returns them
+        (``boundary_load`` a host vector, or ``boundary`` applied on the device)."""
+        vals, rhs, scale = self.gather(blocks, loads, boundary)
         mat = sparse.csc_matrix((vals.cpu().numpy(), self.csc_indices.copy(), self.csc_indptr.copy()),
                                 shape=(self.size, self.size))
         mat.eliminate_zeros()
@@ -143,3 +152,47 @@ class HybridScatter:
             rhs[:self.trace_size] -= b
             scale[:self.trace_size] += np.abs(b)
         return mat, rhs, scale
+
+
+class DirichletBoundary:
+    """Weak Dirichlet data on the boundary macrofaces of a 2D macro mesh, on the
+    device (``boundary_data``, ``src/pymhm/fem/scalar/operators.py:247-321``,
+    scalar Dirichlet data, no Neumann faces).
+
+    Every boundary face uses the face space's segment quadrature of order
+    ``max(order, degree + 2)`` from the face's stored start to its end;
+    ``points`` ``(faces, q, 2)`` are where the caller evaluates the data ``g``.
+    ``load(g)`` returns the trace load ``|f| sum_q mu_a(t_q) w_q g_q`` per
+    face DOF (zero on interior faces) and can apply it to a global rhs.
+    """
+
+    def __init__(self, macro, face_space, order: int = 5, device: int = 0):
+        self.space = face_space
+        fs = face_space.size
+        faces = np.asarray(macro.boundary_faces, dtype=np.int64)
+        par, w = face_space.quadrature(max(int(order), max(face_space.degrees) + 2))
+        basis = face_space.evaluate(par)
+        start = macro.points[macro.faces[faces, 0]]
+        end = macro.points[macro.faces[faces, 1]]
+        self.faces = faces
+        self.points = start[:, None, :] + par[None, :, None] * (end - start)[:, None, :]
+        self.length = np.linalg.norm(end - start, axis=1)
+        self.trace_size = len(macro.faces) * fs
+        self.nq, self.fs = len(par), fs
+        self.device = torch.device("cuda", device)
+        self.d_basis = torch.as_tensor(np.ascontiguousarray(basis), device=self.device)
+        self.d_weights = torch.as_tensor(w, device=self.device)
+        self.d_length = torch.as_tensor(self.length, device=self.device)
+        self.d_dof0 = torch.as_tensor(faces * fs, device=self.device)
+
+    def load(self, g, rhs=None, scale=None):
+        """The trace load of data ``g`` ``(faces, q)`` at ``points``; with
+        ``rhs``/``scale`` (device vectors of the global system) also
+        ``rhs -= load`` and ``scale += |load|`` on the boundary rows."""
+        g = torch.as_tensor(g, dtype=torch.float64, device=self.device).contiguous()
+        if tuple(g.shape) != (len(self.faces), self.nq):
+            raise ValueError(f"g must have shape ({len(self.faces)}, {self.nq})")
+        out = torch.zeros(self.trace_size, dtype=torch.float64, device=self.device)
+        _lib.call("mhm_dirichlet_load", len(self.faces), self.nq, self.fs, _ptr(self.d_basis), _ptr(self.d_weights),
+                  _ptr(g), _ptr(self.d_length), _ptr(self.d_dof0), _ptr(out), _ptr(rhs), _ptr(scale), _stream())
+        return out

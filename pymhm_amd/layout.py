@@ -180,7 +180,43 @@ class LocalLayout:
             handle = ctypes.c_void_p()
             _lib.call("mhm_layout_create", self.device, ctypes.byref(self._desc), ctypes.byref(handle))
             self._handle = handle
+            if getattr(self, "_skel_desc", None) is not None:
+                _lib.call("mhm_layout_set_skeleton", handle, ctypes.byref(self._skel_desc))
         return self._handle
+
+    def set_skeleton(self, tables):
+        """Attach the macroface skeleton (``meshing.skeleton_tables`` of the
+        template macro, local DOF ids) so batches can compute their couplings on
+        the device (``LocalBatch.skeleton_coupling``)."""
+        t = dict(tables)
+        if t["nsides"] * t["nseg"] * (t["degree"] + 1) != self.m:
+            raise ValueError("skeleton size does not match the layout's multiplier count")
+        rows = self.inv[np.asarray(t["entry_node"], dtype=np.int64)]
+        keep = {}
+
+        def arr(name, a, dtype):
+            a = np.ascontiguousarray(a, dtype=dtype)
+            keep[name] = a
+            return a.ctypes.data
+
+        d = _lib.SkeletonDesc()
+        d.nsides, d.nseg, d.degree, d.edeg = t["nsides"], t["nseg"], t["degree"], t["edeg"]
+        d.ng = len(t["gx"])
+        d.breaks = arr("breaks", t["breaks"], np.float64)
+        d.gx = arr("gx", t["gx"], np.float64)
+        d.gw = arr("gw", t["gw"], np.float64)
+        d.npieces = len(t["piece_f"])
+        d.piece_f = arr("piece_f", t["piece_f"], np.float64)
+        d.nentries = len(rows)
+        d.entry_row = arr("entry_row", rows, np.int32)
+        d.entry_side = arr("entry_side", t["entry_side"], np.int32)
+        d.entry_seg = arr("entry_seg", t["entry_seg"], np.int32)
+        d.entry_ptr = arr("entry_ptr", t["entry_ptr"], np.int32)
+        d.contrib = arr("contrib", t["contrib"], np.int32)
+        self._skel_keep = keep
+        self._skel_desc = d
+        if self._handle is not None:
+            _lib.call("mhm_layout_set_skeleton", self._handle, ctypes.byref(d))
 
     def _assembly_maps(self, asm, d, arr):
         edofs = np.asarray(asm["dofs"], dtype=np.int64)

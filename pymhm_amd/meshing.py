@@ -297,6 +297,19 @@ class FaceSpace:
             col += p + 1
         return out
 
+    def quadrature(self, order: int = 4):
+        """Gauss points and weights per segment on [0, 1] (``fem/traces/interval.py:92-118``):
+        the reference rule renormalised to mass two in extended precision, then
+        mapped onto every segment; the weights sum to 1 over the face."""
+        x, w = leggauss(int(order))
+        w = np.asarray(w, dtype=np.longdouble)
+        w *= np.longdouble(2) / np.sum(w)
+        pts, wts = [], []
+        for lo, hi in zip(self.breaks[:-1], self.breaks[1:]):
+            pts.extend(lo + (x + 1) * (hi - lo) / 2)
+            wts.extend(w * (hi - lo) / 2)
+        return np.asarray(pts), np.asarray(wts, dtype=float)
+
     def reversal(self):
         """Signed column map of the same space under t -> 1 - t (uniform breaks only)."""
         b = np.asarray(self.breaks)
@@ -337,6 +350,63 @@ def side_coupling(mesh: LocalMesh, start, end, space: FaceSpace, sign: float = 1
             local = elements.interval_tables(k, (par - t[0]) / (t[1] - t[0]))
             out[rows, :] += local.T @ (w[:, None] * space.evaluate(par))
     return out
+
+
+def skeleton_tables(mesh: LocalMesh, verts, space: FaceSpace):
+    """Mesh-only description of a macro's skeleton coupling for the device
+    kernel (``mhm_layout_set_skeleton``).
+
+    Side ``s`` runs counterclockwise from ``verts[s]`` to ``verts[s+1]`` with
+    fraction ``f`` in [0, 1].  Every fine boundary edge on it (endpoints at
+    fractions ``e0 < e1``) is cut at the face segment breaks into pieces
+    ``[f0, f1]``; a piece contributes its edge's nodal functions (0: the node at
+    ``e0``, 1: the node at ``e1``, 2: the P2 midpoint) to the coupling entry
+    (node, side, segment containing the piece).  Returns the piece table, the
+    entries in sorted order with their contribution lists (local DOF ids), the
+    face space's breaks and a Gauss-Legendre rule exact for the products.
+    """
+    verts = np.asarray(verts, dtype=float)
+    k = len(verts)
+    b = np.asarray(space.breaks, dtype=float)
+    if len(set(space.degrees)) != 1:
+        raise ValueError("the device skeleton needs one Legendre degree for every segment")
+    pieces, contribs = [], {}
+    for s in range(k):
+        start, end = verts[s], verts[(s + 1) % k]
+        tangent = end - start
+        length = float(np.sqrt(tangent @ tangent))
+        for f in mesh.boundary:
+            a, c = mesh.faces[f]
+            xy = mesh.points[[a, c]]
+            t = (xy - start) @ tangent / (length * length)
+            if (not np.allclose(xy, start + t[:, None] * tangent, atol=1e-12, rtol=1e-12)
+                    or t.min() < -1e-12 or t.max() > 1 + 1e-12):
+                continue
+            nodes = [a, c] if t[0] < t[1] else [c, a]
+            if mesh.degree == 2:
+                nodes.append(len(mesh.points) + f)
+            e0, e1 = float(np.clip(t.min(), 0.0, 1.0)), float(np.clip(t.max(), 0.0, 1.0))
+            cuts = sorted({e0, e1, *(x for x in b if e0 < x < e1)})
+            for f0, f1 in zip(cuts[:-1], cuts[1:]):
+                seg = int(np.searchsorted(b, 0.5 * (f0 + f1), side="right") - 1)
+                seg = min(max(seg, 0), len(b) - 2)
+                pid = len(pieces)
+                pieces.append((f0, f1, e0, e1))
+                for basis, node in enumerate(nodes):
+                    contribs.setdefault((int(node), s, seg), []).append(pid * 3 + basis)
+    keys = sorted(contribs)
+    ptr = np.zeros(len(keys) + 1, dtype=np.int64)
+    ptr[1:] = np.cumsum([len(contribs[key]) for key in keys])
+    ng = space.degrees[0] + mesh.degree + 1
+    gx, gw = leggauss(ng)
+    return dict(
+        nsides=k, nseg=len(b) - 1, degree=int(space.degrees[0]), edeg=int(mesh.degree), breaks=b, gx=gx, gw=gw,
+        piece_f=np.asarray(pieces, dtype=float).reshape(-1, 4),
+        entry_node=np.array([key[0] for key in keys], dtype=np.int64),
+        entry_side=np.array([key[1] for key in keys], dtype=np.int64),
+        entry_seg=np.array([key[2] for key in keys], dtype=np.int64),
+        entry_ptr=ptr, contrib=np.concatenate([contribs[key] for key in keys]).astype(np.int64) if keys
+        else np.zeros(0, dtype=np.int64))
 
 
 def local_mass_ones(mesh: LocalMesh):

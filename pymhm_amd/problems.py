@@ -193,10 +193,11 @@ class PolygonProblem:
     form a **bucket** that shares one layout (pattern, band ordering, assembly
     plan) and one batch; the geometry (barycentric gradients, element
     measures), the constraints ``M 1 / 1^T M 1`` and the couplings are per
-    macro.  Inside a bucket every macro's boundary nodes sit at the same
-    fractions of its sides, so a side's coupling is the bucket's unit-length
-    template times the side length, with the macroface orientation as a signed
-    column map (``fem/scalar/triangle.py:160-205``).
+    macro and computed on the device: the constraints from the element
+    measures, the couplings from the bucket's skeleton tables (its boundary
+    edges at fixed fractions of the sides - the same in every macro of the
+    bucket) with each macro's side lengths and macroface orientations
+    (``fem/scalar/triangle.py:160-205``).
     """
 
     def __init__(self, macro: meshing.MacroMesh, refinements, degree: int = 1,
@@ -265,14 +266,12 @@ class PolygonBucket:
         template = np.concatenate(
             [meshing.side_coupling(mesh, verts0[s], verts0[(s + 1) % k], prob.space) / lengths0[s]
              for s in range(k)], axis=1)
-        rev_map, rev_scale = prob.space.reversal()
         nq = len(prob.qw)
         self.glam = np.empty((Mb, ne, 3, 2))
         self.meas = np.empty((Mb, ne))
         self.qpts = np.empty((Mb, ne, nq, 2))
-        self.constraints = np.empty((Mb, n))
-        self.colmap = np.empty((Mb, m), dtype=np.int32)
-        self.colscale = np.empty((Mb, m))
+        self.side_len = np.empty((Mb, k))
+        self.side_sign = np.empty((Mb, k))
         self.trace_dofs = np.empty((Mb, m), dtype=np.int64)
         for i, c in enumerate(cells):
             verts = macro.points[macro.cells[c]]
@@ -282,37 +281,30 @@ class PolygonBucket:
                 raise ValueError(f"macro {c}: inverted element in its ear refinement")
             self.glam[i], self.meas[i] = g, ms
             self.qpts[i] = np.einsum("qi,tij->tqj", prob.qbary, pts[mesh.cells])
-            mass1 = np.zeros(n)
-            if prob.degree == 1:
-                np.add.at(mass1, mesh.dofs.ravel(), np.repeat(ms / 3.0, 3))
-            else:
-                np.add.at(mass1, mesh.dofs[:, 3:].ravel(), np.repeat(ms / 3.0, 3))
-            self.constraints[i] = mass1 / mass1.sum()
-            lengths = np.linalg.norm(np.roll(verts, -1, axis=0) - verts, axis=1)
+            self.side_len[i] = np.linalg.norm(np.roll(verts, -1, axis=0) - verts, axis=1)
+            self.side_sign[i] = macro.signs[c]
             for s in range(k):
-                cols = slice(s * fs, (s + 1) * fs)
-                if macro.signs[c][s] > 0:
-                    self.colmap[i, cols] = s * fs + np.arange(fs)
-                    self.colscale[i, cols] = lengths[s]
-                else:  # second owner: the face's stored direction is reversed, sign -1
-                    self.colmap[i, cols] = s * fs + rev_map
-                    self.colscale[i, cols] = -rev_scale * lengths[s]
-                self.trace_dofs[i, cols] = macro.cell_faces[c][s] * fs + np.arange(fs)
+                self.trace_dofs[i, s * fs:(s + 1) * fs] = macro.cell_faces[c][s] * fs + np.arange(fs)
         indptr, indices = element_pattern(n, mesh.dofs)
         tangent = verts0[1] - verts0[0]
         dist = np.abs(meshing._cross(tangent, mesh.nodes - verts0[0]))
         start = np.flatnonzero(dist <= 1e-12 * float(np.sqrt(tangent @ tangent)))
         perm = choose_permutation(indptr, indices, start, mesh.nodes - verts0[0], tangent)
+        mass1 = meshing.local_mass_ones(mesh)
         self.layout = LocalLayout(
-            n, perm, indptr, indices, m=m, k=1, coupling=template, constraint=self.constraints[0],
+            n, perm, indptr, indices, m=m, k=1, coupling=template, constraint=mass1 / mass1.sum(),
             assembly=dict(dofs=mesh.dofs, dim=2, qweights=prob.qw, dtable=prob.der, vtable=prob.vals,
                           grad_lambda=self.glam[0], measure=self.meas[0]),
             device=device)
+        self.layout.set_skeleton(meshing.skeleton_tables(mesh, verts0, prob.space))
 
     def batch(self, precision: str = "fp64", max_columns: int = 0):
+        """A batch of the bucket's macros with their geometry, couplings and
+        constraints set on the device."""
         b = LocalBatch(self.layout, len(self.cells), precision=precision, max_columns=max_columns)
-        b.set_coupling(colmap=self.colmap, colscale=self.colscale, constraints=self.constraints)
         b.set_geometry(self.glam, self.meas)
+        b.skeleton_coupling(self.side_len, self.side_sign)
+        b.mass_constraints()
         return b
 
 

@@ -142,6 +142,30 @@ class LocalBatch:
         _lib.call("mhm_batch_set_geometry", self.handle, _ptr(g), _ptr(ms), _stream())
         torch.cuda.current_stream().synchronize()
 
+    def skeleton_coupling(self, side_len, side_sign):
+        """Couplings computed on the device from the layout's skeleton:
+        ``side_len`` and ``side_sign`` are ``(M, sides)`` (sign +1 for the first
+        owner of the macroface, -1 for the second)."""
+        ns = int(self.layout._skel_desc.nsides) if getattr(self.layout, "_skel_desc", None) else 0
+        if not ns:
+            raise ValueError("the layout has no skeleton (LocalLayout.set_skeleton)")
+        ln = self._shaped("side_len", side_len, torch.float64, (self.macros, ns))
+        sg = self._shaped("side_sign", side_sign, torch.float64, (self.macros, ns))
+        _lib.call("mhm_batch_skeleton_coupling", self.handle, _ptr(ln), _ptr(sg), _stream())
+        torch.cuda.current_stream().synchronize()
+
+    def mass_constraints(self):
+        """Constraints ``M 1 / (1^T M 1)`` computed on the device from the
+        batch's element geometry (set it first for uneven macros)."""
+        _lib.call("mhm_batch_mass_constraints", self.handle, _stream())
+
+    def coupling(self):
+        """The effective ``(coupling (M, m, n), constraints (M, n))`` in original rows."""
+        cp = torch.empty((self.macros, self.m, self.n), dtype=torch.float64, device=self.device)
+        cn = torch.empty((self.macros, self.n), dtype=torch.float64, device=self.device) if self.k else None
+        _lib.call("mhm_batch_coupling", self.handle, _ptr(cp), _ptr(cn), _stream())
+        return cp, cn
+
     # -------------------------------------------------------------- numerics
     def factor(self, check: bool = True):
         _lib.call("mhm_batch_factor", self.handle, _stream())
