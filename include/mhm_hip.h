@@ -360,6 +360,44 @@ int mhm_dirichlet_load(int32_t nfaces, int32_t nq, int32_t fs, const double *d_b
                        const double *d_g, const double *d_length, const int64_t *d_dof0, double *d_load,
                        double *d_rhs, double *d_scale, void *stream);
 
+/* ---------------------------------------------------------------- global hybrid solve
+ * The assembled global system (CSC pattern of mhm_scatter / HybridScatter,
+ * traces then coarse amplitudes) solved on the device by restarted GMRES,
+ * right-preconditioned by weighted additive Schwarz over the contribution
+ * cells (each cell's assembled w x w block inverted, w <= 64).  Replaces the
+ * global solve of solve_hybrid_system (src/pymhm/core/system.py:355-528) and
+ * SaddleBlockSolver (src/pymhm/linalg/block.py:99-237): prepare once per
+ * matrix, then solve any number of right-hand sides against the resident
+ * preconditioner, the reuse contract of a prepared factorization
+ * (system.py:442-451, core/offline.py:234-307). */
+typedef struct mhm_global mhm_global;
+
+/* Pattern (host arrays): csc_indptr[size + 1], csc_indices[nnz] (rows
+ * ascending within a column), and the cells' global indices
+ * cell_idx[cell_ptr[c] .. cell_ptr[c + 1]) in cell order.  Every row must
+ * belong to a cell.  Replaces SaddleBlockSolver.__init__'s structural part
+ * (block.py:115-160). */
+int mhm_global_create(int32_t device, int32_t size, int64_t nnz, const int32_t *csc_indptr,
+                      const int32_t *csc_indices, int32_t ncells, const int64_t *cell_ptr, const int64_t *cell_idx,
+                      mhm_global **out);
+
+/* Device values in CSC order; builds the preconditioner and synchronises the
+ * stream.  MHM_ESINGULAR (with *bad_cell) when a cell's assembled block is
+ * singular or not finite.  Replaces the factorisation of the preconditioner
+ * blocks (block.py:160-175) and factorize(matrix) (linalg/linear.py:467-537). */
+int mhm_global_set_values(mhm_global *g, const double *d_csc_values, int32_t *bad_cell, void *stream);
+
+/* x = S^-1 b for one device right-hand side: ||b - S x|| <= rtol ||b|| with the
+ * residual in double-double (linalg/linear.py:132-152), at most maxiter
+ * Arnoldi steps, restarted every `restart`.  Synchronous (one host round trip
+ * per step).  MHM_ERESIDUAL when the criterion is not met.  Replaces
+ * SaddleBlockSolver.solve (block.py:176-200). */
+int mhm_global_solve(mhm_global *g, const double *d_rhs, double *d_x, double rtol, int32_t maxiter,
+                     int32_t restart, int32_t *iterations, double *residual, void *stream);
+
+/* Release.  Replaces SaddleBlockSolver.close (block.py:218-221). */
+int mhm_global_destroy(mhm_global *g);
+
 /* Kernel timing with HIP events recorded on the launch stream (no
  * counterpart in the reference, whose benchmarks time phases with
  * perf_counter wall clocks, benchmarks/execution_modes.py:227-258).

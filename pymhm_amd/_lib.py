@@ -30,6 +30,7 @@ EXPORTS = (
     "mhm_timing_enable", "mhm_timing_reset", "mhm_timing_read",
     "mhm_layout_set_skeleton", "mhm_batch_skeleton_coupling", "mhm_batch_mass_constraints",
     "mhm_batch_coupling", "mhm_dirichlet_load",
+    "mhm_global_create", "mhm_global_set_values", "mhm_global_solve", "mhm_global_destroy",
     "mhm_dense_create", "mhm_dense_solve", "mhm_dense_destroy", "mhm_dense_factor_solve",
     "mhm_dense_factor_solve_noalloc", "mhm_dense_scratch_bytes",
 )
@@ -118,6 +119,10 @@ def load(path: str | os.PathLike | None = None):
         "mhm_batch_mass_constraints": (P, P),
         "mhm_batch_coupling": (P, P, P, P),
         "mhm_dirichlet_load": (I32, I32, I32, P, P, P, P, P, P, P, P, P),
+        "mhm_global_create": (I32, I32, I64, P, P, I32, P, P, P),
+        "mhm_global_set_values": (P, P, P, P),
+        "mhm_global_solve": (P, P, P, D, I32, I32, P, P, P),
+        "mhm_global_destroy": (P,),
         "mhm_dense_create": (I32, I32, I32, P, P, P, P),
         "mhm_dense_solve": (P, P, I32, D, D, P, P, P),
         "mhm_dense_destroy": (P,),

@@ -196,3 +196,103 @@ class DirichletBoundary:
         _lib.call("mhm_dirichlet_load", len(self.faces), self.nq, self.fs, _ptr(self.d_basis), _ptr(self.d_weights),
                   _ptr(g), _ptr(self.d_length), _ptr(self.d_dof0), _ptr(out), _ptr(rhs), _ptr(scale), _stream())
         return out
+
+
+class HybridSolver:
+    """The assembled global hybrid system solved on the device.
+
+    Mirrors ``SaddleBlockSolver`` (``src/pymhm/linalg/block.py:99-237``):
+    prepared once per matrix (``set_values``), then ``solve`` for any number of
+    right-hand sides against the resident preconditioner, and ``close``.  The
+    Krylov method is restarted GMRES, right-preconditioned by weighted additive
+    Schwarz over the plan's contribution cells (each cell's assembled block
+    inverted on the device); a solve is accepted when the double-double
+    residual of the original equations meets ``||b - S x|| <= rtol ||b||``
+    (``linalg/linear.py:132-152``), else it raises ``LinearSolveError``.
+    ``restart`` defaults to ``maxiter``: the whole Krylov basis stays in HBM.
+    """
+
+    def __init__(self, plan: HybridScatter, rtol: float = 1e-10, maxiter: int = 3000, restart: int | None = None,
+                 device: int = 0):
+        if not (0.0 < rtol < 1.0):
+            raise ValueError("rtol must lie in (0, 1)")
+        self.plan = plan
+        self.rtol, self.maxiter = float(rtol), int(maxiter)
+        self.restart = self.maxiter if restart is None else int(restart)
+        if self.maxiter < 1 or self.restart < 1:
+            raise ValueError("maxiter and restart must be positive")
+        self.device = torch.device("cuda", device)
+        cells = [np.asarray(r, dtype=np.int64) for r in (plan.indices if plan.w is None else list(plan.indices))]
+        cell_ptr = np.zeros(len(cells) + 1, dtype=np.int64)
+        cell_ptr[1:] = np.cumsum([len(r) for r in cells])
+        cell_idx = np.ascontiguousarray(np.concatenate(cells))
+        indptr = np.ascontiguousarray(plan.csc_indptr, dtype=np.int32)
+        indices = np.ascontiguousarray(plan.csc_indices, dtype=np.int32)
+        h = ctypes.c_void_p()
+        _lib.call("mhm_global_create", device, int(plan.size), int(len(indices)), indptr.ctypes.data,
+                  indices.ctypes.data, len(cells), cell_ptr.ctypes.data, cell_idx.ctypes.data, ctypes.byref(h))
+        self._handle = h.value
+        self.iterations: list[int] = []
+        self.residuals: list[float] = []
+        self.bad_cell = -1
+
+    @property
+    def handle(self):
+        if self._handle is None:
+            raise RuntimeError("hybrid solver is closed")
+        return ctypes.c_void_p(self._handle)
+
+    def set_values(self, vals):
+        """Device CSC values of the plan's pattern (``HybridScatter.gather``)."""
+        vals = torch.as_tensor(vals, dtype=torch.float64, device=self.device).contiguous()
+        if vals.numel() != len(self.plan.csc_indices):
+            raise ValueError("values must match the plan's CSC pattern")
+        bad = ctypes.c_int32(-1)
+        try:
+            _lib.call("mhm_global_set_values", self.handle, _ptr(vals), ctypes.byref(bad), _stream())
+        finally:
+            self.bad_cell = bad.value
+
+    def solve(self, rhs):
+        """``x`` (device) with ``S x = rhs`` for one right-hand side."""
+        rhs = torch.as_tensor(rhs, dtype=torch.float64, device=self.device).contiguous()
+        if tuple(rhs.shape) != (self.plan.size,):
+            raise ValueError(f"rhs must have shape ({self.plan.size},)")
+        x = torch.empty_like(rhs)
+        it, res = ctypes.c_int32(0), ctypes.c_double(0.0)
+        try:
+            _lib.call("mhm_global_solve", self.handle, _ptr(rhs), _ptr(x), self.rtol, self.maxiter, self.restart,
+                      ctypes.byref(it), ctypes.byref(res), _stream())
+        finally:
+            self.iterations.append(it.value)
+            self.residuals.append(res.value)
+        return x
+
+    def close(self):
+        if self._handle is not None:
+            _lib.call("mhm_global_destroy", ctypes.c_void_p(self._handle))
+            self._handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def solve_hybrid(plan: HybridScatter, blocks, loads, boundary=None, rtol: float = 1e-10, maxiter: int = 3000):
+    """Gather, prepare and solve on the device: ``(trace, coarse, solver)``
+    (device tensors; the solver keeps the preconditioner for more right-hand
+    sides).  The device counterpart of ``solve_hybrid_system`` without gauges
+    (``src/pymhm/core/system.py:355-528``)."""
+    vals, rhs, _ = plan.gather(blocks, loads, boundary)
+    solver = HybridSolver(plan, rtol=rtol, maxiter=maxiter)
+    solver.set_values(vals)
+    x = solver.solve(rhs)
+    return x[:plan.trace_size], x[plan.trace_size:], solver
