@@ -284,6 +284,8 @@ def run_config5(args):
                 rate = 0.5 * capacity
             live = srv.run(serving.poisson_requests(count, rate, 3, frac, seed=3), realtime=True)
             st = serving.latency_stats(live)
+            failed = sum(r.error is not None for r in sat + live)
+            memory = dict(resident_bytes=srv.resident_bytes, hbm_budget=srv.hbm_budget, peak_inflight_bytes=srv.peak_bytes)
             # spot check: a source request on config 1 against the oracle
             from oracle import workloads as ow
             fq = srv.source_fields(0, [0])
@@ -296,7 +298,7 @@ def run_config5(args):
             torch.cuda.synchronize()
         res[precision] = dict(capacity=capacity, offered_rate=rate, served_rate=st["requests_per_s"],
                               p50_ms=st["p50_ms"], p99_ms=st["p99_ms"], mean_ms=st["mean_ms"],
-                              max_rel_err_vs_oracle=err)
+                              max_rel_err_vs_oracle=err, failed_requests=failed, memory=memory)
         print(f"[bench] {precision}: capacity {capacity:.1f} req/s, at {rate:.1f} req/s p50 {st['p50_ms']:.1f} ms "
               f"p99 {st['p99_ms']:.1f} ms, err {err:.2e}", file=sys.stderr, flush=True)
     ok = [p for p in res if res[p]["max_rel_err_vs_oracle"] <= 1e-10]
@@ -311,7 +313,8 @@ def run_config5(args):
         "config": {"workload": "config5: Poisson requests over config1 (8x8 squares, P1), config2 (64x64 squares, "
                                "P2) and config3 (16^3 cubes, 4^3x6 P2 tets); coefficient requests rerun the stage "
                                "with exp(+-log K + b), source requests re-solve 8-bump Gaussian sources against "
-                               "resident factors, up to 8 per batch",
+                               "resident factors, up to 8 per batch; one stream per problem, batches of different "
+                               "problems overlap, completion polled by events, admission by HBM footprint",
                    "requests": count, "precision": path},
         "p50_ms": head["p50_ms"], "p99_ms": head["p99_ms"],
         "paths": res,
@@ -437,7 +440,7 @@ def main(argv=None):
     ap.add_argument("--workload", choices=sorted(WORKLOADS) + ["config4", "config5"], default="config2",
                     help="config2: 2D square macros (headline); config3: 3D cube macros; "
                          "config5: serving requests against resident factors of configs 1-3")
-    ap.add_argument("--requests", type=int, default=400, help="config5: requests per run")
+    ap.add_argument("--requests", type=int, default=10000, help="config5: requests per run")
     ap.add_argument("--rate", type=float, default=None,
                     help="config5: offered request rate (default half the measured fp64 capacity)")
     ap.add_argument("--nx", type=int, default=None, help="macros per axis (default 64 in 2D, 16 in 3D)")

@@ -235,6 +235,14 @@ int mhm_batch_sources(mhm_batch *batch, const double *d_fq, double f_const, int3
 int mhm_batch_reconstruct(mhm_batch *batch, const double *d_trace, const double *d_coarse,
                           double *d_fields, int32_t ncases, void *stream);
 
+/* mhm_batch_reconstruct with the source responses of new loads:
+ * d_source[case][macro][n] (original rows, as mhm_batch_sources returns them)
+ * in place of the standard source response.  The online reconstruction of
+ * OfflineHybridSystem.solve (core/offline.py:234-307, per cell
+ * reconstruct_response with the new LocalResponse, core/offline.py:121-127). */
+int mhm_batch_reconstruct_sources(mhm_batch *batch, const double *d_source, const double *d_trace,
+                                  const double *d_coarse, double *d_fields, int32_t ncases, void *stream);
+
 /* Synchronise the stream and copy the per-macro status bits to host memory
  * (host_status[macros]); also returns the bitwise OR in *any.  The bits are
  * those of the current factors (PIVOT, NONFINITE operator; kept until the next

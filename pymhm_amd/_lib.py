@@ -25,7 +25,8 @@ EXPORTS = (
     "mhm_layout_create", "mhm_layout_destroy", "mhm_layout_info",
     "mhm_batch_create", "mhm_batch_destroy", "mhm_batch_assemble", "mhm_batch_set_operator",
     "mhm_batch_operator", "mhm_batch_set_coupling", "mhm_batch_set_geometry", "mhm_batch_factor", "mhm_batch_condense", "mhm_batch_responses",
-    "mhm_batch_solve", "mhm_batch_sources", "mhm_batch_reconstruct", "mhm_batch_status",
+    "mhm_batch_solve", "mhm_batch_sources", "mhm_batch_reconstruct", "mhm_batch_reconstruct_sources",
+    "mhm_batch_status",
     "mhm_batch_iterations", "mhm_scatter_coordinates", "mhm_ordered_gather",
     "mhm_timing_enable", "mhm_timing_reset", "mhm_timing_read",
     "mhm_layout_set_skeleton", "mhm_batch_skeleton_coupling", "mhm_batch_mass_constraints",
@@ -107,6 +108,7 @@ def load(path: str | os.PathLike | None = None):
         "mhm_batch_solve": (P, P, P, I32, I32, D, P),
         "mhm_batch_sources": (P, P, D, I32, I32, D, P, P, P),
         "mhm_batch_reconstruct": (P, P, P, P, I32, P),
+        "mhm_batch_reconstruct_sources": (P, P, P, P, P, I32, P),
         "mhm_batch_status": (P, P, P, P),
         "mhm_batch_iterations": (P, P, P),
         "mhm_scatter_coordinates": (P, I32, I32, P, P, P),

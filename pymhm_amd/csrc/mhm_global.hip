@@ -26,6 +26,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <cstdio>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -508,7 +509,7 @@ extern "C" int mhm_global_solve(mhm_global *g, const double *d_rhs, double *d_x,
   const double target = rtol * bnorm;
   std::vector<double> H((size_t)(R + 1) * R), cs(R), sn(R), gv(R + 1), y(R);
   int total = 0;
-  double rel = INFINITY;
+  double rel = INFINITY, last = INFINITY;
   for (;;) {
     // r = b - S x (double-double rows), beta = ||r||
     hipLaunchKernelGGL(k_glb_residual, dim3(nb), dim3(256), 0, st, n, (const int *)g->rowptr, (const int *)g->col,
@@ -519,17 +520,25 @@ extern "C" int mhm_global_solve(mhm_global *g, const double *d_rhs, double *d_x,
     rel = beta / bnorm;
     if (!(beta < INFINITY)) return dfail(MHM_ERESIDUAL, "global solver: residual not finite");
     if (beta <= target) break;
-    if (total >= maxiter) {
+    if (total >= maxiter || !(beta < 0.5 * last)) {
+      // out of steps, or a whole cycle no longer halves the residual (the
+      // attainable accuracy of the preconditioned iteration)
       if (iterations) *iterations = total;
       if (residual) *residual = rel;
-      return dfail(MHM_ERESIDUAL, "global solver: GMRES did not reach rtol in " + std::to_string(maxiter) +
-                                      " iterations (relative residual " + std::to_string(rel) + ")");
+      char buf[160];
+      snprintf(buf, sizeof buf, "global solver: GMRES stopped at relative residual %.3e > rtol %.3e after %d steps",
+               rel, rtol, total);
+      return dfail(MHM_ERESIDUAL, buf);
     }
+    last = beta;
     hipLaunchKernelGGL(k_glb_scale, dim3(nb), dim3(256), 0, st, n, 1.0 / beta, (const double *)g->r, g->V);
     std::fill(gv.begin(), gv.end(), 0.0);
     gv[0] = beta;
     int k = 0;
-    const double inner = 0.1 * target;  // the Arnoldi estimate aims below the checked criterion
+    // a cycle aims one decade below the checked criterion, or nine decades
+    // below its own starting residual, whichever is larger: later cycles
+    // restart from the double-double residual, refining the solution
+    const double inner = std::max(0.1 * target, 1e-9 * beta);
     for (; k < R && total < maxiter; ++k, ++total) {
       double *zk = g->Z + (size_t)k * n;
       glb_precond(g, g->V + (size_t)k * n, zk, st);

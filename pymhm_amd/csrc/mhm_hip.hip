@@ -2904,10 +2904,12 @@ __global__ void __launch_bounds__(256)
   if (ret && L.k) ret[(size_t)mac * n + o] = kernel_at(L, B, mac, i) - xr[1 + m];
 }
 
-// u = s - L lambda + E c, original rows.
+// u = s - L lambda + E c, original rows; s from src[case][macro][n] (original
+// rows) when src is not NULL, else the standard source response.
 __global__ void __launch_bounds__(256)
     k_reconstruct(LayoutK L, BatchK B, const double *__restrict__ X, const double *__restrict__ tr,
-                  const double *__restrict__ co, double *__restrict__ fields, int ncases) {
+                  const double *__restrict__ co, double *__restrict__ fields, int ncases,
+                  const double *__restrict__ src) {
   const int mac = blockIdx.y;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= L.n) return;
@@ -2919,7 +2921,7 @@ __global__ void __launch_bounds__(256)
     const double *lam = tr + ((size_t)mac * ncases + c) * m;
     double t = 0.0;
     for (int a = 0; a < m; ++a) t += xr[1 + a] * lam[a];
-    double u = xr[0] - t;
+    double u = (src ? src[((size_t)c * B.M + mac) * n + o] : xr[0]) - t;
     if (k) u += e * co[((size_t)mac * ncases + c) * k];
     fields[((size_t)mac * ncases + c) * n + o] = u;
   }
@@ -4347,7 +4349,20 @@ int mhm_batch_reconstruct(mhm_batch *b, const double *tr, const double *co, doub
   if (L.k && !co) return fail(MHM_EINVAL, "coarse amplitudes required");
   DeviceGuard guard(b->layout->device);
   hipLaunchKernelGGL(k_reconstruct, grid2(L.n, 256, b->M), dim3(256), 0, (hipStream_t)stream, L, b->bk,
-                     (const double *)b->xstd, tr, co, fields, ncases);
+                     (const double *)b->xstd, tr, co, fields, ncases, (const double *)nullptr);
+  LAUNCH_CHECK("k_reconstruct");
+  return MHM_OK;
+}
+
+int mhm_batch_reconstruct_sources(mhm_batch *b, const double *src, const double *tr, const double *co,
+                                  double *fields, int32_t ncases, void *stream) {
+  if (!b || !src || !tr || !fields || ncases <= 0) return fail(MHM_EINVAL, "bad reconstruct arguments");
+  if (!b->condensed) return fail(MHM_EINVAL, "no condensed responses");
+  const LayoutK &L = b->layout->k;
+  if (L.k && !co) return fail(MHM_EINVAL, "coarse amplitudes required");
+  DeviceGuard guard(b->layout->device);
+  hipLaunchKernelGGL(k_reconstruct, grid2(L.n, 256, b->M), dim3(256), 0, (hipStream_t)stream, L, b->bk,
+                     (const double *)b->xstd, tr, co, fields, ncases, src);
   LAUNCH_CHECK("k_reconstruct");
   return MHM_OK;
 }

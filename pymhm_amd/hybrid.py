@@ -136,6 +136,22 @@ class HybridScatter:
             bnd.load(g, rhs=rhs, scale=scale)
         return vals, rhs, scale
 
+    def gather_loads(self, loads, boundary=None):
+        """Global rhs and load scale of new loads on the GPU, the matrix unchanged
+        (the online assembly of ``core/offline.py:272-286``); ``boundary`` as in
+        ``gather``."""
+        loads = loads.reshape(-1).contiguous()
+        rhs = torch.empty(self.size, dtype=torch.float64, device=self.device)
+        scale = torch.empty_like(rhs)
+        _lib.call("mhm_ordered_gather", _ptr(self.lptr), _ptr(self.lcode), _ptr(loads), int(self.size),
+                  _ptr(rhs), _ptr(scale), _stream())
+        if boundary is not None:
+            bnd, g = boundary
+            if bnd.trace_size != self.trace_size:
+                raise ValueError("the boundary belongs to another skeleton")
+            bnd.load(g, rhs=rhs, scale=scale)
+        return rhs, scale
+
     def assemble(self, blocks, loads, boundary_load=None, boundary=None):
         """``(matrix, rhs, load_scale)`` as ``assemble_hybrid_contributions`` returns them
         (``boundary_load`` a host vector, or ``boundary`` applied on the device)."""

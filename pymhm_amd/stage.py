@@ -245,8 +245,10 @@ class LocalBatch:
             self.raise_on_failure("sources")
         return (loads, src) if source else loads
 
-    def reconstruct(self, trace, coarse=None):
-        """Fields ``(M, cases, n)`` from local traces ``(M, cases, m)`` and amplitudes ``(M, cases, k)``."""
+    def reconstruct(self, trace, coarse=None, source=None):
+        """Fields ``(M, cases, n)`` from local traces ``(M, cases, m)`` and amplitudes ``(M, cases, k)``;
+        ``source`` ``(cases, M, n)`` (from ``sources(..., source=True)``) replaces the standard
+        source response (the online reconstruction of ``core/offline.py:234-307``)."""
         trace = self._dev(trace, torch.float64)
         if trace.dim() == 2:
             trace = trace[:, None, :]
@@ -263,8 +265,15 @@ class LocalBatch:
             if tuple(co.shape) != (self.macros, cases, self.k):
                 raise ValueError(f"coarse must have shape ({self.macros}, {cases}, {self.k}), got {tuple(co.shape)}")
         fields = torch.empty((self.macros, cases, self.n), dtype=torch.float64, device=self.device)
-        _lib.call("mhm_batch_reconstruct", self.handle, _ptr(trace.contiguous()), _ptr(co), _ptr(fields),
-                  int(cases), _stream())
+        if source is None:
+            _lib.call("mhm_batch_reconstruct", self.handle, _ptr(trace.contiguous()), _ptr(co), _ptr(fields),
+                      int(cases), _stream())
+            return fields
+        src = self._dev(source, torch.float64).contiguous()
+        if tuple(src.shape) != (cases, self.macros, self.n):
+            raise ValueError(f"source must have shape ({cases}, {self.macros}, {self.n}), got {tuple(src.shape)}")
+        _lib.call("mhm_batch_reconstruct_sources", self.handle, _ptr(src), _ptr(trace.contiguous()), _ptr(co),
+                  _ptr(fields), int(cases), _stream())
         return fields
 
     # -------------------------------------------------------------- status

@@ -163,14 +163,14 @@ def test_error_paths():
     with b:
         plan = hybrid.HybridScatter(prob.global_indices(), prob.trace_size, prob.kernel_offsets())
         vals, rhs, _ = plan.gather(blocks, loads)
-        with hybrid.HybridSolver(plan, rtol=1e-12, maxiter=2) as solver:
+        with hybrid.HybridSolver(plan, rtol=1e-13, maxiter=2) as solver:
             with pytest.raises(ValueError):
                 solver.solve(rhs)  # not prepared: refused before any work
-        with hybrid.HybridSolver(plan, rtol=1e-12) as solver:
+        with hybrid.HybridSolver(plan, rtol=1e-13) as solver:
             with pytest.raises(LinearSolveError):
                 solver.set_values(torch.zeros_like(vals))
             assert solver.bad_cell == 0
-        with hybrid.HybridSolver(plan, rtol=1e-12, maxiter=2) as solver:
+        with hybrid.HybridSolver(plan, rtol=1e-13, maxiter=2) as solver:
             solver.set_values(vals)
             with pytest.raises(LinearSolveError):
                 solver.solve(rhs)

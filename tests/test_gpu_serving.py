@@ -56,34 +56,70 @@ def test_sources_against_oracle_and_batch_independent(which, precision):
         assert rel(stacked, ref_stacked) < TOL, c
 
 
-def test_served_results_equal_alone():
-    cases = [pw.config1(nx=4), pw.config3(nx=2, refinement=2)]
-    print("serving: 14 requests over two problems", flush=True)
-    reqs = serving.poisson_requests(14, rate=1000.0, nproblems=2, coefficient_fraction=0.3, seed=5)
-    srv = serving.Server(cases, "fp64", max_columns=4, source_pool=5)
+def replay_alone(srv, cases, done, precision):
+    """Each problem's requests one at a time on fresh batches, bit for bit."""
+    for p, case in enumerate(cases):
+        mine = [r for r in done if r.problem == p]
+        kq = torch.as_tensor(case.kq, dtype=torch.float64, device="cuda")
+        with case.problem.batch(precision, max_columns=srv.max_columns) as b:
+            b.assemble(kq, case.fq, case.f_const)
+            b.factor()
+            b.condense()
+            for r in mine:
+                assert r.error is None, (r.rid, r.error)
+                if r.kind == serving.COEFFICIENT:
+                    kq = serving.coefficient_field(torch.as_tensor(case.kq, dtype=torch.float64, device="cuda"),
+                                                   r.seed)
+                    b.assemble(kq, case.fq, case.f_const)
+                    b.factor()
+                    blocks, loads = b.condense()
+                    assert torch.equal(blocks, r.result[0]) and torch.equal(loads, r.result[1]), r.rid
+                else:
+                    fq = srv.source_fields(p, [r.seed])
+                    assert torch.equal(b.sources(fq)[0], r.result), r.rid
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_served_results_equal_alone(precision):
+    """Three problems (the config-1 and config-2 square meshes, the config-3
+    cubes) served concurrently on their own streams: every result equals the
+    same request computed alone."""
+    cases = [pw.config1(nx=4), pw.config2(nx=6, refinement=16), pw.config3(nx=2, refinement=2)]
+    print(f"serving {precision}: 30 requests over three problems", flush=True)
+    reqs = serving.poisson_requests(30, rate=1000.0, nproblems=3, coefficient_fraction=0.3, seed=5)
+    srv = serving.Server(cases, precision, max_columns=4, source_pool=5)
     try:
         done = srv.run(reqs, realtime=False)
         assert all(np.isfinite(r.done) for r in done)
-        # replay each problem's requests one at a time on fresh batches
-        for p, case in enumerate(cases):
-            mine = [r for r in done if r.problem == p]
-            kq = torch.as_tensor(case.kq, dtype=torch.float64, device="cuda")
-            with case.problem.batch("fp64", max_columns=4) as b:
-                b.assemble(kq, case.fq, case.f_const)
-                b.factor()
-                b.condense()
-                for r in mine:
-                    if r.kind == serving.COEFFICIENT:
-                        kq = serving.coefficient_field(torch.as_tensor(case.kq, dtype=torch.float64, device="cuda"),
-                                                       r.seed)
-                        b.assemble(kq, case.fq, case.f_const)
-                        b.factor()
-                        blocks, loads = b.condense()
-                        assert torch.equal(blocks, r.result[0]) and torch.equal(loads, r.result[1]), r.rid
-                    else:
-                        fq = srv.source_fields(p, [r.seed])
-                        assert torch.equal(b.sources(fq)[0], r.result), r.rid
+        assert srv.peak_bytes <= srv.hbm_budget
+        replay_alone(srv, cases, done, precision)
         stats = serving.latency_stats(done)
-        assert stats["requests"] == 14 and stats["p99_ms"] >= stats["p50_ms"]
+        assert stats["requests"] == 30 and stats["p99_ms"] >= stats["p50_ms"]
     finally:
         srv.close()
+
+
+def test_admission_by_footprint():
+    """A budget of one and a half source columns: every source batch is cut to
+    one column, the in-flight bytes never pass the budget, and the results are
+    unchanged; a budget below one batch is refused."""
+    cases = [pw.config1(nx=4), pw.config2(nx=4, refinement=8)]
+    reqs = [serving.Request(i, 0.0, i % 2, serving.SOURCE, 100 + i) for i in range(8)]
+    probe = serving.Server(cases, "fp64", max_columns=4, source_pool=0)
+    try:
+        per = max(probe.request_bytes(p, serving.SOURCE, 1) for p in range(2))
+    finally:
+        probe.close()
+    srv = serving.Server(cases, "fp64", max_columns=4, source_pool=0, hbm_budget=int(1.5 * per))
+    try:
+        done = srv.run(reqs, realtime=False)
+        assert srv.peak_bytes <= srv.hbm_budget
+        replay_alone(srv, cases, done, "fp64")
+    finally:
+        srv.close()
+    tiny = serving.Server(cases, "fp64", max_columns=4, source_pool=0, hbm_budget=16)
+    try:
+        with pytest.raises(MemoryError):
+            tiny.run(reqs[:1], realtime=False)
+    finally:
+        tiny.close()
