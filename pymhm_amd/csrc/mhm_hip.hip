@@ -2116,8 +2116,7 @@ __device__ __forceinline__ double rhs_at(const LayoutK &L, const BatchK &B, cons
 //  DD:   double-double evaluation for the pending columns left (rdone = 0,
 //        near the rounding floor), and for the final check (check_only: set
 //        MHM_STATUS_RESIDUAL for columns that miss the criterion).
-// fwd_tol > 0 also requires forward convergence of the refinement.  ||b||^2
-// comes from bnorm (kept by k_rhs_init).
+// ||b||^2 comes from bnorm (kept by k_rhs_init).
 // Block per macro, RES_WAVES wavefronts, lane c = column c (ncols <= 64).
 // Rows go in chunks of RES_ROWS; wavefront w takes rows i0 + w, i0 + w +
 // RES_WAVES, ... of the chunk, one at a time.  The chunk's CSR slots (values,
@@ -2155,8 +2154,8 @@ static size_t res_stage(int ncols, int slots) {
 template <typename T, bool RING, bool FAST>
 __global__ void __launch_bounds__(64 * RES_WAVES)
     k_residual(LayoutK L, BatchK B, int ncols, const double *__restrict__ rhs, const double *__restrict__ X,
-               double rtol, double fwd_tol, int it, T *__restrict__ y, double *__restrict__ beta,
-               int *__restrict__ colstate, const double *__restrict__ colnorm, int check_only,
+               double rtol, int it, T *__restrict__ y, double *__restrict__ beta,
+               int *__restrict__ colstate, int check_only,
                const int *__restrict__ rowptr, const int *__restrict__ cols, const double *__restrict__ aval,
                const double *__restrict__ dscale, const double *__restrict__ coupling_rm, int ring,
                int slots, double *__restrict__ bnorm, int *__restrict__ rdone, double gam_row, double gam_con) {
@@ -2428,13 +2427,7 @@ __global__ void __launch_bounds__(64 * RES_WAVES)
     if (FAST) E2 = gam_con * (fabs(bn) + l) * (gam_con * (fabs(bn) + l));
   }
   const bool resid_ok = sqrt(R2) <= rtol * sqrt(B2) && BAD == 0.0;
-  bool ok = resid_ok;
-  if (ok && fwd_tol > 0.0) {
-    // forward convergence: the last correction is below fwd_tol ||x||, or it
-    // no longer halves (refinement has reached the binary64 noise floor)
-    const double *cn = colnorm + ((size_t)mac * ncols + c) * 3;
-    ok = cn[0] <= fwd_tol * fwd_tol * cn[1] || cn[0] >= 0.25 * cn[2];
-  }
+  const bool ok = resid_ok;
   if (FAST) {
     // accepted, or a correction from the binary64 residual when it is at
     // least 16 times its rounding bound; the rest go to the double-double stage
@@ -2604,8 +2597,7 @@ static int rhs_group_chunks(int ncols, int width) {
 }
 
 // x = D y (+ bordered correction when k == 1); the first pass (REFINE false)
-// sets X, refinement passes add the correction and record ||delta||^2,
-// ||x||^2 and the previous ||delta||^2 in colnorm[3 * col].
+// sets X, refinement passes add the correction.
 // Per-column sums over the rows go in fixed chunks of SADDLE_ROWS rows, each
 // summed in row order, then the chunk sums in chunk order: the result does not
 // depend on how many columns share the call (a column's bits are the same
@@ -2616,9 +2608,8 @@ static int rhs_group_chunks(int ncols, int width) {
 template <typename T, bool REFINE>
 __global__ void __launch_bounds__(SADDLE_THREADS) __attribute__((amdgpu_waves_per_eu(8)))
     k_saddle_apply(LayoutK L, BatchK B, int ncols, const T *__restrict__ y, const double *__restrict__ beta,
-                   double *__restrict__ X, const int *__restrict__ colstate,
-                   double *__restrict__ colnorm) {
-  extern __shared__ double part[];  // 2 x nchunk x ncols
+                   double *__restrict__ X, const int *__restrict__ colstate) {
+  extern __shared__ double part[];  // nchunk x ncols
   __shared__ double sxp[64], smu[64];
   const int mac = blockIdx.x;
   const int tid = threadIdx.x;
@@ -2663,7 +2654,6 @@ __global__ void __launch_bounds__(SADDLE_THREADS) __attribute__((amdgpu_waves_pe
   if (!skip) {
     const double xp = L.k ? sxp[c] : 0.0, mu = L.k ? smu[c] : 0.0;
     for (int k = lane; k < nchunk; k += lanes) {
-      double dn = 0.0, xn = 0.0;
       const int i1 = min(n, (k + 1) * SADDLE_ROWS);
       // SAP_ROWS rows in flight: unconditional loads of clamped rows, one
       // wait, then the stores (a per-row loop waits for the previous row's
@@ -2689,20 +2679,10 @@ __global__ void __launch_bounds__(SADDLE_THREADS) __attribute__((amdgpu_waves_pe
           if (i < i1) {
             double v = dv[q] * yv[q];
             if (L.k) v += sigma * xp * uv[q] - mu * wv[q];
-            double *xi = x + (size_t)i * ncols + c;
-            if (REFINE) {
-              const double nv = xv[q] + v;
-              *xi = nv;
-              dn += v * v;
-              xn += nv * nv;
-            } else {
-              *xi = v;
-            }
+            x[(size_t)i * ncols + c] = REFINE ? xv[q] + v : v;
           }
         }
       }
-      part[k * ncols + c] = dn;
-      part[(nchunk + k) * ncols + c] = xn;
     }
     if (L.k && lane == 0) {
       double *xm = x + (size_t)n * ncols + c;
@@ -2712,27 +2692,11 @@ __global__ void __launch_bounds__(SADDLE_THREADS) __attribute__((amdgpu_waves_pe
         *xm = mu;
     }
   }
-  __syncthreads();
-  if (tid < ncols && !(REFINE && colstate[(size_t)mac * ncols + tid])) {
-    double D2 = 0.0, X2 = 0.0;
-    for (int k = 0; k < nchunk; ++k) {
-      D2 += part[k * ncols + tid];
-      X2 += part[(nchunk + k) * ncols + tid];
-    }
-    double *cn = colnorm + ((size_t)mac * ncols + tid) * 3;
-    if (REFINE) {
-      cn[2] = cn[0];
-      cn[0] = D2;
-      cn[1] = X2;
-    } else {
-      cn[0] = cn[2] = INFINITY;
-      cn[1] = 0.0;
-    }
-  }
 }
 
+// LDS of the per-column chunk sums (nchunk x ncols)
 static size_t saddle_lds(int n, int ncols) {
-  return (size_t)2 * ((n + SADDLE_ROWS - 1) / SADDLE_ROWS) * ncols * sizeof(double);
+  return (size_t)((n + SADDLE_ROWS - 1) / SADDLE_ROWS) * ncols * sizeof(double);
 }
 
 __global__ void k_fill_int(int *p, int v, size_t count) {
@@ -3008,7 +2972,7 @@ struct mhm_batch {
   void *band = nullptr, *rband = nullptr, *y = nullptr;  // rband: row-major band (wavefront solve)
   void *sval = nullptr;  // scaled operator in CSR slot order (LDS-window factor only)
   double *dinv = nullptr;  // inverses of the 16 x 16 diagonal blocks (binary64 matrix-core solve)
-  double *xstd = nullptr, *beta = nullptr, *colnorm = nullptr, *bnorm = nullptr;
+  double *xstd = nullptr, *beta = nullptr, *bnorm = nullptr;
   double *rhsu = nullptr, *xu = nullptr, *srcscale = nullptr;
   double *glam = nullptr, *meas = nullptr;  // per-macro element geometry (uneven macros)
   int *colstate = nullptr;
@@ -3696,8 +3660,7 @@ int mhm_batch_create(const mhm_layout *L, int32_t macros, int32_t precision, int
   const int widest = std::max(std::max(1 + lk.m + lk.k, 2), max_columns);
   if (!solve_supported(lk.b, widest, tsize, limit) || !solve_supported(lk.b, 2, tsize, limit))
     return fail(MHM_EINVAL, "solve window exceeds the register or LDS budget");
-  if (std::max(saddle_lds(lk.n, widest), saddle_lds(lk.n, widest) / 2 + (RHS_TILE + lk.n) * sizeof(double)) >
-      (size_t)limit)
+  if (saddle_lds(lk.n, widest) + (RHS_TILE + lk.n) * sizeof(double) > (size_t)limit)
     return fail(MHM_EINVAL, "too many local rows for the refinement's per-column sums");
   if (factor_lds(lk.b, tsize) > (size_t)limit &&
       !(FPANEL + lk.b <= 127 && factor_panel_lds(lk.b, tsize) <= (size_t)limit) &&
@@ -3736,7 +3699,7 @@ int mhm_batch_create(const mhm_layout *L, int32_t macros, int32_t precision, int
       (rc = dev_alloc(b->owned, &k.iters, M)) ||
       (rc = dev_alloc(b->owned, &b->xstd, M * R * N)) ||
       (rc = dev_alloc(b->owned, &b->beta, M * ncm)) ||
-      (rc = dev_alloc(b->owned, &b->colnorm, M * ncm * 3)) || (rc = dev_alloc(b->owned, &b->bnorm, M * ncm)) ||
+      (rc = dev_alloc(b->owned, &b->bnorm, M * ncm)) ||
       (rc = dev_alloc(b->owned, &b->colstate, M * ncm)) || (rc = dev_alloc(b->owned, &b->rdone, M * ncm)) ||
       (rc = dev_alloc(b->owned, &b->done, M)))
     return cleanup(rc);
@@ -4110,7 +4073,7 @@ int mhm_batch_factor(mhm_batch *b, void *stream) {
 
 // Residual launch: column width G per wavefront and the x ring when it fits in LDS.
 template <typename T>
-static int launch_residual(mhm_batch *b, int ncols, const double *rhs, const double *X, double rtol, double fwd,
+static int launch_residual(mhm_batch *b, int ncols, const double *rhs, const double *X, double rtol,
                            int it, int check_only, hipStream_t st) {
   const LayoutK &L = b->layout->k;
   if (it < 1 || ncols < 1 || ncols > 64) return fail(MHM_EINVAL, "residual pass needs it >= 1 and 1..64 columns");
@@ -4135,8 +4098,8 @@ static int launch_residual(mhm_batch *b, int ncols, const double *rhs, const dou
                            : (ring > 0 ? k_residual<T, true, false> : k_residual<T, false, false>);
     if (shm > 65536)
       HIP_TRY(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
-    hipLaunchKernelGGL(kern, dim3(b->M), dim3(64 * RES_WAVES), shm, st, L, b->bk, ncols, rhs, X, rtol, fwd, it,
-                       (T *)b->y, b->beta, b->colstate, (const double *)b->colnorm, check_only, L.rowptr, L.cols,
+    hipLaunchKernelGGL(kern, dim3(b->M), dim3(64 * RES_WAVES), shm, st, L, b->bk, ncols, rhs, X, rtol, it,
+                       (T *)b->y, b->beta, b->colstate, check_only, L.rowptr, L.cols,
                        (const double *)b->bk.aval, (const double *)b->bk.dscale, b->layout->coupling_rm, ring, slots,
                        b->bnorm, b->rdone, gam_row, gam_con);
     LAUNCH_CHECK("k_residual");
@@ -4148,9 +4111,8 @@ static int launch_residual(mhm_batch *b, int ncols, const double *rhs, const dou
 // binary64 refinement; rhs == NULL selects the standard columns [f, B, A Z].
 template <typename T>
 static int run_solve(mhm_batch *b, const double *rhs, double *X, int ncols, int max_refine,
-                     double rtol, hipStream_t st, const std::function<int()> *after_first_pass = nullptr) {
+                     double rtol, hipStream_t st, const std::function<int()> *after_settled = nullptr) {
   const LayoutK &L = b->layout->k;
-  const double fwd = b->precision == MHM_MIXED ? 1.0e-12 : 0.0;
   // the status of a solve is its own: bits of an earlier call do not carry over
   hipLaunchKernelGGL(k_fill_int, dim3((b->M + 255) / 256), dim3(256), 0, st, b->bk.cstatus, 0, (size_t)b->M);
   hipLaunchKernelGGL(k_fill_int, dim3((unsigned)(((size_t)b->M * ncols + 255) / 256)), dim3(256), 0, st,
@@ -4167,24 +4129,29 @@ static int run_solve(mhm_batch *b, const double *rhs, double *X, int ncols, int 
     LAUNCH_CHECK("k_saddle_init");
     b->uw_pending = false;
   }
+  // the pass whose residual finds almost every macro converged: the first
+  // with binary64 factors; with binary32 ones the third (a correction
+  // contracts the error by about 1e-6 on these operators, so the solution
+  // and two corrections meet the 1e-10 criterion)
+  const int settled = sizeof(T) == 8 ? 1 : 3;
   for (int it = 0; it <= max_refine; ++it) {
     {
       TimerSpan span(MHM_TIMER_RESIDUAL, st);
       if (it == 0) {
         const int gch = rhs_group_chunks(ncols, uw_inline ? ncols + 2 : ncols);
-        const size_t shm = saddle_lds(L.n, ncols) / 2 + ((size_t)gch * SADDLE_ROWS * ncols + L.n) * sizeof(double);
+        const size_t shm = saddle_lds(L.n, ncols) + ((size_t)gch * SADDLE_ROWS * ncols + L.n) * sizeof(double);
         if (shm > 65536)
           HIP_TRY(hipFuncSetAttribute((const void *)k_rhs_init<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                       (int)shm));
         hipLaunchKernelGGL((k_rhs_init<T>), dim3(b->M), dim3(SADDLE_THREADS), shm, st, L, b->bk, ncols, rhs, (T *)b->y,
                            b->beta, b->colstate, b->layout->coupling_rm, b->bnorm, uw_inline ? ncols : -1, gch);
         LAUNCH_CHECK("k_rhs_init");
-      } else if (int rc = launch_residual<T>(b, ncols, rhs, X, rtol, fwd, it, 0, st)) {
+      } else if (int rc = launch_residual<T>(b, ncols, rhs, X, rtol, it, 0, st)) {
         return rc;
       }
     }
-    if (it == 1 && after_first_pass)
-      if (int rc = (*after_first_pass)()) return rc;
+    if (it == settled && after_settled)
+      if (int rc = (*after_settled)()) return rc;
     if (it == 0 && uw_inline) {
       {
         // every column is pending at it == 0, so no column state is needed
@@ -4205,13 +4172,13 @@ static int run_solve(mhm_batch *b, const double *rhs, double *X, int ncols, int 
     const int sthreads = (it > 0 && sizeof(T) == 8) ? 256 : SADDLE_THREADS;
     if (it > 0)
       hipLaunchKernelGGL((k_saddle_apply<T, true>), dim3(b->M), dim3(sthreads), saddle_lds(L.n, ncols), st, L, b->bk,
-                         ncols, (const T *)b->y, (const double *)b->beta, X, (const int *)b->colstate, b->colnorm);
+                         ncols, (const T *)b->y, (const double *)b->beta, X, (const int *)b->colstate);
     else
       hipLaunchKernelGGL((k_saddle_apply<T, false>), dim3(b->M), dim3(sthreads), saddle_lds(L.n, ncols), st, L, b->bk,
-                         ncols, (const T *)b->y, (const double *)b->beta, X, (const int *)b->colstate, b->colnorm);
+                         ncols, (const T *)b->y, (const double *)b->beta, X, (const int *)b->colstate);
     LAUNCH_CHECK("k_saddle_apply");
   }
-  if (int rc = launch_residual<T>(b, ncols, rhs, X, rtol, 0.0, 1, 1, st)) return rc;
+  if (int rc = launch_residual<T>(b, ncols, rhs, X, rtol, 1, 1, st)) return rc;
   return MHM_OK;
 }
 
@@ -4226,7 +4193,7 @@ int mhm_batch_condense(mhm_batch *b, int32_t max_refine, double rtol, double *bl
   const size_t cshm = condense_lds(L.m, L.k);
   if (cshm > 65536)
     HIP_TRY(hipFuncSetAttribute((const void *)k_condense, hipFuncAttributeMaxDynamicSharedMemorySize, (int)cshm));
-  // After the first refinement pass almost every macro has converged: those
+  // Once almost every macro has converged (run_solve's settled pass) those
   // are condensed on the side stream while the few others finish refining on
   // the caller's stream, then the caller's stream condenses the rest.
   bool early = false;
