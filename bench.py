@@ -366,7 +366,7 @@ def run_config4(args):
     from pymhm_amd.problems import BucketedStage
 
     for precision in paths:
-        stage = BucketedStage(prob, precision, streams=4)
+        stage = BucketedStage(prob, precision, streams=args.streams)
         bpos, lpos = stage.positions()
         plan = hybrid.HybridScatter(idx, prob.trace_size, prob.kernel_offsets(), positions=bpos,
                                     load_positions=lpos)
@@ -450,6 +450,8 @@ def main(argv=None):
                     help="macros in the oracle CPU-baseline sample (default 768 in 2D, 256 in 3D)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--paths", default="mixed,fp64", help="precision paths to run (comma separated)")
+    ap.add_argument("--streams", type=int, default=16,
+                    help="config4: HIP streams the buckets are issued over, each on a hardware queue of its own")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="run one device-to-device copy of this many bytes first (a known-size kernel "
                          "to calibrate profiler byte counters against)")
@@ -461,6 +463,9 @@ def main(argv=None):
     if args.workload == "config5":
         return run_config5(args)
     if args.workload == "config4":
+        # one hardware queue per bucket stream (HIP's default is 4 per process;
+        # it is read when the runtime starts, so before anything touches the GPU)
+        os.environ.setdefault("GPU_MAX_HW_QUEUES", str(args.streams))
         return run_config4(args)
     from pymhm_amd import hybrid
     from pymhm_amd import workloads as pw

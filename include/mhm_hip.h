@@ -189,6 +189,16 @@ int mhm_batch_factor(mhm_batch *batch, void *stream);
 int mhm_batch_condense(mhm_batch *batch, int32_t max_refine, double rtol, double *d_blocks,
                        double *d_loads, void *stream);
 
+/* Whether mhm_batch_condense condenses the macros that converge early on the
+ * batch's own side stream while the others finish refining (1, the default)
+ * or everything on the caller's stream (0).  No counterpart in the reference
+ * (its condensation is one host loop); for callers that run many batches
+ * concurrently on their own streams, such as topology buckets of uneven
+ * macros: a side stream shares the device's few hardware queues with those
+ * streams, and its wait for the batch's settled pass holds up every kernel
+ * queued behind it. */
+int mhm_batch_set_side_stream(mhm_batch *batch, int32_t enable);
+
 /* Copy the responses of the last mhm_batch_condense in ORIGINAL row order:
  * d_source[macro][n], d_lifts[macro][n][m] (row-major like numpy (n, m)),
  * d_retained[macro][n] (E = Z - R A Z, or Z when A Z == 0).  Any may be NULL.

@@ -24,6 +24,7 @@ EXPORTS = (
     "mhm_last_error", "mhm_device_count", "mhm_version",
     "mhm_layout_create", "mhm_layout_destroy", "mhm_layout_info",
     "mhm_batch_create", "mhm_batch_destroy", "mhm_batch_assemble", "mhm_batch_set_operator",
+    "mhm_batch_set_side_stream",
     "mhm_batch_operator", "mhm_batch_set_coupling", "mhm_batch_set_geometry", "mhm_batch_factor", "mhm_batch_condense", "mhm_batch_responses",
     "mhm_batch_solve", "mhm_batch_sources", "mhm_batch_reconstruct", "mhm_batch_reconstruct_sources",
     "mhm_batch_status",
@@ -104,6 +105,7 @@ def load(path: str | os.PathLike | None = None):
         "mhm_batch_set_geometry": (P, P, P, P),
         "mhm_batch_factor": (P, P),
         "mhm_batch_condense": (P, I32, D, P, P, P),
+        "mhm_batch_set_side_stream": (P, I32),
         "mhm_batch_responses": (P, P, P, P, P),
         "mhm_batch_solve": (P, P, P, I32, I32, D, P),
         "mhm_batch_sources": (P, P, D, I32, I32, D, P, P, P),

@@ -33,7 +33,10 @@
 #include <cstdio>
 #include <cstring>
 #include <functional>
+#include <map>
+#include <mutex>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/mhm_hip.h"
@@ -2980,6 +2983,7 @@ struct mhm_batch {
   int *done = nullptr;   // per macro: converged after the first refinement pass (early condensation)
   hipStream_t side = nullptr;                    // early condensation, beside the caller's stream
   hipEvent_t ev_mark = nullptr, ev_side = nullptr;
+  bool use_side = true;  // early condensation on the side stream
   double *ework = nullptr, *fwork = nullptr;
   int *colmap = nullptr;
   double *colscale = nullptr, *bcoup = nullptr, *cons = nullptr, *kern = nullptr;
@@ -3019,8 +3023,10 @@ static int upload(std::vector<void *> &owned, const P *host, size_t count, const
 struct DeviceGuard {
   int prev = -1;
   explicit DeviceGuard(int dev) {
-    if (hipGetDevice(&prev) != hipSuccess) prev = -1;
-    if (prev != dev) (void)hipSetDevice(dev);
+    int cur = -1;
+    if (hipGetDevice(&cur) == hipSuccess && cur == dev) return;  // already current: nothing to restore
+    prev = cur;
+    (void)hipSetDevice(dev);
   }
   ~DeviceGuard() {
     if (prev >= 0) (void)hipSetDevice(prev);
@@ -3038,6 +3044,23 @@ static int lds_limit(int device) {
       v <= 0)
     v = 65536;
   return v;
+}
+
+// A kernel launched with more than 64 KiB of dynamic LDS needs its limit
+// raised first; that host call costs about as much as a launch, so it is made
+// once per (device, kernel) and again only when a launch asks for more.
+static hipError_t set_max_lds(const void *fn, size_t bytes) {
+  static std::mutex mu;
+  static std::map<std::pair<int, const void *>, size_t> granted;
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  std::lock_guard<std::mutex> lock(mu);
+  size_t &g = granted[{dev, fn}];
+  if (bytes <= g) return hipSuccess;
+  e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+  if (e == hipSuccess) g = bytes;
+  return e;
 }
 
 static int block_threads(int b) { return b >= 12 ? 256 : 64; }
@@ -3301,8 +3324,7 @@ static int launch_solve_blk(const mhm_batch *b, int ncols, const int *colstate, 
 #define MHM_BLK(c)                                                                                         \
   case c:                                                                                                  \
     if (shm > 65536)                                                                                       \
-      HIP_TRY(hipFuncSetAttribute((const void *)k_solve_blk<T, c>, hipFuncAttributeMaxDynamicSharedMemorySize, \
-                                  (int)shm));                                                              \
+      HIP_TRY(set_max_lds((const void *)k_solve_blk<T, c>, shm));                                          \
     hipLaunchKernelGGL((k_solve_blk<T, c>), dim3(b->M), dim3(256), shm, st, L.n, L.b, b->ldy, ncols,         \
                        (const T *)b->band, (T *)b->y, colstate, iters);                                    \
     break;
@@ -3768,8 +3790,7 @@ static int launch_assemble(const mhm_batch *b, const double *kq, const double *f
   const size_t shm = (tab + (size_t)ASM_WAVES * 2 * EPW * NB * DP) * sizeof(double);
   if (shm > 64 * 1024) {
     if (shm > 160 * 1024) return fail(MHM_EINVAL, "assembly tables exceed the LDS of a workgroup");
-    HIP_TRY(hipFuncSetAttribute((const void *)k_assemble<NB, D>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)shm));
+    HIP_TRY(set_max_lds((const void *)k_assemble<NB, D>, shm));
   }
   const int tasks = (L.ne + EPW - 1) / EPW;
   const dim3 grid((unsigned)((tasks + ASM_WAVES - 1) / ASM_WAVES), (unsigned)std::min(b->M, 2048));
@@ -3807,7 +3828,7 @@ int mhm_batch_assemble(mhm_batch *b, const double *kq, const double *fq, double 
   if (gshm > (size_t)lds_limit(b->layout->device))
     return fail(MHM_EINVAL, "pattern too dense for the gather stage (" + std::to_string(GATHER_ROWS) + " rows)");
   if (gshm > 65536)
-    HIP_TRY(hipFuncSetAttribute((const void *)k_gather, hipFuncAttributeMaxDynamicSharedMemorySize, (int)gshm));
+    HIP_TRY(set_max_lds((const void *)k_gather, gshm));
   hipLaunchKernelGGL(k_gather, grid2(L.n, GATHER_ROWS, b->M), dim3(256), gshm, st, L, b->bk, b->ework, b->fwork);
   LAUNCH_CHECK("k_gather");
   b->factored = b->condensed = false;
@@ -4020,22 +4041,19 @@ static int run_factor(mhm_batch *b, hipStream_t st) {
   TimerSpan span(MHM_TIMER_FACTOR, st);
   if (panel) {
     if (pshm > 65536)
-      HIP_TRY(hipFuncSetAttribute((const void *)k_factor_panel<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  (int)pshm));
+      HIP_TRY(set_max_lds((const void *)k_factor_panel<T>, pshm));
     hipLaunchKernelGGL((k_factor_panel<T>), dim3(b->M), dim3(256), pshm, st, L.n, L.b, panel_ldw(L.b, sizeof(T)),
                        panel_ldq(L.b), (T *)b->band, (T *)b->rband, b->bk.status, tiny,
                        (const T *)b->sval, L.nnz, L.bslot, sizeof(T) == 8 ? (T *)b->dinv : (T *)nullptr);
   } else if (factor_wide_ok(L.b, sizeof(T), lds_limit(lay->device))) {
     const size_t wshm = factor_wide_lds(L.b, sizeof(T));
     if (wshm > 65536)
-      HIP_TRY(hipFuncSetAttribute((const void *)k_factor_wide<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  (int)wshm));
+      HIP_TRY(set_max_lds((const void *)k_factor_wide<T>, wshm));
     hipLaunchKernelGGL((k_factor_wide<T>), dim3(b->M), dim3(256), wshm, st, L.n, L.b, wide_ldq(L.b),
                        (T *)b->band, (T *)b->rband, b->bk.status, tiny);
   } else if (L.b >= 12) {
     if (shm > 65536)
-      HIP_TRY(hipFuncSetAttribute((const void *)k_factor<T, 256>,
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+      HIP_TRY(set_max_lds((const void *)k_factor<T, 256>, shm));
     hipLaunchKernelGGL((k_factor<T, 256>), dim3(b->M), dim3(256), shm, st, L.n, L.b, (T *)b->band,
                        (T *)b->rband, b->bk.status, tiny);
   } else {
@@ -4097,7 +4115,7 @@ static int launch_residual(mhm_batch *b, int ncols, const double *rhs, const dou
     auto kern = stage == 0 ? (ring > 0 ? k_residual<T, true, true> : k_residual<T, false, true>)
                            : (ring > 0 ? k_residual<T, true, false> : k_residual<T, false, false>);
     if (shm > 65536)
-      HIP_TRY(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+      HIP_TRY(set_max_lds((const void *)kern, shm));
     hipLaunchKernelGGL(kern, dim3(b->M), dim3(64 * RES_WAVES), shm, st, L, b->bk, ncols, rhs, X, rtol, it,
                        (T *)b->y, b->beta, b->colstate, check_only, L.rowptr, L.cols,
                        (const double *)b->bk.aval, (const double *)b->bk.dscale, b->layout->coupling_rm, ring, slots,
@@ -4141,8 +4159,7 @@ static int run_solve(mhm_batch *b, const double *rhs, double *X, int ncols, int 
         const int gch = rhs_group_chunks(ncols, uw_inline ? ncols + 2 : ncols);
         const size_t shm = saddle_lds(L.n, ncols) + ((size_t)gch * SADDLE_ROWS * ncols + L.n) * sizeof(double);
         if (shm > 65536)
-          HIP_TRY(hipFuncSetAttribute((const void *)k_rhs_init<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                      (int)shm));
+          HIP_TRY(set_max_lds((const void *)k_rhs_init<T>, shm));
         hipLaunchKernelGGL((k_rhs_init<T>), dim3(b->M), dim3(SADDLE_THREADS), shm, st, L, b->bk, ncols, rhs, (T *)b->y,
                            b->beta, b->colstate, b->layout->coupling_rm, b->bnorm, uw_inline ? ncols : -1, gch);
         LAUNCH_CHECK("k_rhs_init");
@@ -4167,7 +4184,7 @@ static int run_solve(mhm_batch *b, const double *rhs, double *X, int ncols, int 
     }
     const void *sap = it > 0 ? (const void *)k_saddle_apply<T, true> : (const void *)k_saddle_apply<T, false>;
     if (saddle_lds(L.n, ncols) > 65536)
-      HIP_TRY(hipFuncSetAttribute(sap, hipFuncAttributeMaxDynamicSharedMemorySize, (int)saddle_lds(L.n, ncols)));
+      HIP_TRY(set_max_lds(sap, saddle_lds(L.n, ncols)));
     // binary64 refinement passes leave few macros: small blocks dispatch faster
     const int sthreads = (it > 0 && sizeof(T) == 8) ? 256 : SADDLE_THREADS;
     if (it > 0)
@@ -4192,7 +4209,7 @@ int mhm_batch_condense(mhm_batch *b, int32_t max_refine, double rtol, double *bl
   hipStream_t st = (hipStream_t)stream;
   const size_t cshm = condense_lds(L.m, L.k);
   if (cshm > 65536)
-    HIP_TRY(hipFuncSetAttribute((const void *)k_condense, hipFuncAttributeMaxDynamicSharedMemorySize, (int)cshm));
+    HIP_TRY(set_max_lds((const void *)k_condense, cshm));
   // Once almost every macro has converged (run_solve's settled pass) those
   // are condensed on the side stream while the few others finish refining on
   // the caller's stream, then the caller's stream condenses the rest.
@@ -4210,9 +4227,9 @@ int mhm_batch_condense(mhm_batch *b, int32_t max_refine, double rtol, double *bl
     early = true;
     return MHM_OK;
   };
-  int rc = b->precision == MHM_FP64
-               ? run_solve<double>(b, nullptr, b->xstd, b->R, max_refine, rtol, st, &early_condense)
-               : run_solve<float>(b, nullptr, b->xstd, b->R, max_refine, rtol, st, &early_condense);
+  const std::function<int()> *hook = b->use_side ? &early_condense : nullptr;
+  int rc = b->precision == MHM_FP64 ? run_solve<double>(b, nullptr, b->xstd, b->R, max_refine, rtol, st, hook)
+                                    : run_solve<float>(b, nullptr, b->xstd, b->R, max_refine, rtol, st, hook);
   if (rc) {
     if (early) (void)hipStreamWaitEvent(st, b->ev_side, 0);
     return rc;
@@ -4223,6 +4240,12 @@ int mhm_batch_condense(mhm_batch *b, int32_t max_refine, double rtol, double *bl
                      loads, (const int *)b->done, early ? 2 : 0);
   LAUNCH_CHECK("k_condense");
   b->condensed = true;
+  return MHM_OK;
+}
+
+int mhm_batch_set_side_stream(mhm_batch *b, int32_t enable) {
+  if (!b) return fail(MHM_EINVAL, "batch is NULL");
+  b->use_side = enable != 0;
   return MHM_OK;
 }
 
@@ -4279,23 +4302,19 @@ int mhm_batch_sources(mhm_batch *b, const double *fq, double f_const, int32_t nc
   hipStream_t st = (hipStream_t)stream;
   if (L.dim == 2 && L.nb == 3) {
     if (shm > 65536)
-      HIP_TRY(hipFuncSetAttribute((const void *)k_source_load<3, 2>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  (int)shm));
+      HIP_TRY(set_max_lds((const void *)k_source_load<3, 2>, shm));
     launch_source_load<3, 2>(b, fq, f_const, ncols, shm, st);
   } else if (L.dim == 2 && L.nb == 6) {
     if (shm > 65536)
-      HIP_TRY(hipFuncSetAttribute((const void *)k_source_load<6, 2>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  (int)shm));
+      HIP_TRY(set_max_lds((const void *)k_source_load<6, 2>, shm));
     launch_source_load<6, 2>(b, fq, f_const, ncols, shm, st);
   } else if (L.dim == 3 && L.nb == 4) {
     if (shm > 65536)
-      HIP_TRY(hipFuncSetAttribute((const void *)k_source_load<4, 3>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  (int)shm));
+      HIP_TRY(set_max_lds((const void *)k_source_load<4, 3>, shm));
     launch_source_load<4, 3>(b, fq, f_const, ncols, shm, st);
   } else {
     if (shm > 65536)
-      HIP_TRY(hipFuncSetAttribute((const void *)k_source_load<10, 3>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  (int)shm));
+      HIP_TRY(set_max_lds((const void *)k_source_load<10, 3>, shm));
     launch_source_load<10, 3>(b, fq, f_const, ncols, shm, st);
   }
   LAUNCH_CHECK("k_source_load");

@@ -314,17 +314,30 @@ class BucketedStage:
     Buckets are independent batches; a bucket of a few dozen macros leaves most
     of the GPU idle while its band solves walk their rows, so the buckets are
     issued largest first, round-robin over ``streams`` HIP streams (default
-    priority), and joined before the global gather.  ``plan`` (optional) is a
-    ``HybridScatter`` whose block/load positions follow the bucket order.
+    priority), and joined before the global gather.  Streams beyond the HIP
+    runtime's hardware queues share queues and run in order, so ``streams``
+    defaults to ``GPU_MAX_HW_QUEUES`` (HIP's default 4); on 2048 uneven macros
+    16 queues and streams ran the step 1.6x faster than 4, more than 20
+    slower again.  The batches condense on their own bucket's stream only:
+    a batch's side stream would share a queue with other buckets and hold up
+    what is queued behind its wait.
     """
 
-    def __init__(self, prob: PolygonProblem, precision: str = "fp64", streams: int = 4, max_columns: int = 0):
+    def __init__(self, prob: PolygonProblem, precision: str = "fp64", streams: int | None = None,
+                 max_columns: int = 0):
+        import os
+
         import torch
+
+        if streams is None:
+            streams = int(os.environ.get("GPU_MAX_HW_QUEUES", "4"))
 
         self.prob = prob
         work = [len(bk.cells) * bk.n * (bk.layout.bandwidth + 1) for bk in prob.buckets]
         self.order = sorted(range(len(prob.buckets)), key=lambda i: -work[i])
         self.batches = {i: prob.buckets[i].batch(precision, max_columns=max_columns) for i in self.order}
+        for b in self.batches.values():  # the buckets' own streams overlap them; no side streams
+            b.set_side_stream(False)
         self.streams = [torch.cuda.Stream() for _ in range(max(1, int(streams)))]
 
     def positions(self):

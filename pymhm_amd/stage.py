@@ -186,6 +186,12 @@ class LocalBatch:
             self.raise_on_failure("condense")
         return self.blocks, self.loads
 
+    def set_side_stream(self, enable: bool):
+        """Condense the macros that converge early on the batch's side stream
+        (default) or everything on the caller's stream; batches that run
+        concurrently on streams of their own turn it off."""
+        _lib.call("mhm_batch_set_side_stream", self.handle, int(bool(enable)))
+
     def responses(self):
         """(source ``(M,n)``, lifts ``(M,n,m)``, retained ``(M,n)``) in original rows."""
         src = torch.empty((self.macros, self.n), dtype=torch.float64, device=self.device)

@@ -50,10 +50,18 @@ def test_polygon_assembly_bit_exact(voronoi):
             assert np.array_equal(loads[r], p.load), c
 
 
-@pytest.mark.parametrize("precision", ["fp64", "mixed"])
-def test_polygon_stage(voronoi, precision):
-    case, ref = voronoi
-    print(f"config4 {precision}", flush=True)
+@pytest.fixture(scope="module")
+def voronoi_wide():
+    """24 Voronoi macros refined to h = 1/1024 within 1000-3000 P1 DOFs: six
+    buckets, n up to 2835, one of half-bandwidth 131 (past the LDS-window
+    factor: the wide-band factor and the blocked solve)."""
+    case = pw.config4(npts=24, h=1 / 1024, dof_range=(1000, 3000))
+    shapes = [(bk.n, bk.layout.bandwidth, len(bk.cells)) for bk in case.problem.buckets]
+    print(f"config4 wide (24 Voronoi macros, buckets (n, b, macros) {shapes}): oracle", flush=True)
+    return case, ow.oracle_config(case)
+
+
+def check_stage(case, ref, precision):
     out = pw.run_polygon_stage(case, precision)
     for c in range(case.problem.M):
         for key in ("blocks", "loads", "retained"):
@@ -63,8 +71,26 @@ def test_polygon_stage(voronoi, precision):
         assert rel1(got, want) < TOL, c
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_polygon_stage(voronoi, precision):
+    case, ref = voronoi
+    print(f"config4 {precision}", flush=True)
+    check_stage(case, ref, precision)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_polygon_stage_wide_buckets(voronoi_wide, precision):
+    """The bench's local sizes: n up to 2835 and a bucket past the LDS window."""
+    case, ref = voronoi_wide
+    prob = case.problem
+    assert max(bk.n for bk in prob.buckets) == 2835
+    assert max(bk.layout.bandwidth for bk in prob.buckets) > 111
+    print(f"config4 wide {precision}", flush=True)
+    check_stage(case, ref, precision)
+
+
 def test_bucketed_stage_concurrent_and_global_scatter(voronoi):
-    """Buckets on four streams give the sequential results bit for bit, and the
+    """Buckets on 16 streams give the sequential results bit for bit, and the
     ragged global scatter matches the reference assembly of the oracle's
     contributions (indices exact, values 1e-10)."""
     from oracle.local import assemble_hybrid_contributions
@@ -74,7 +100,7 @@ def test_bucketed_stage_concurrent_and_global_scatter(voronoi):
     case, ref = voronoi
     prob = case.problem
     seq = pw.run_polygon_stage(case, "fp64")
-    stage = BucketedStage(prob, "fp64", streams=4)
+    stage = BucketedStage(prob, "fp64", streams=16)
     try:
         bpos, lpos = stage.positions()
         plan = hybrid.HybridScatter(prob.global_indices(), prob.trace_size, prob.kernel_offsets(),
