@@ -372,6 +372,18 @@ def run_config4(args):
                                     load_positions=lpos)
         batches = [stage.batches[i] for i in range(len(prob.buckets))]
         kqs = [torch.as_tensor(np.stack([case.kq[c] for c in bk.cells]), device="cuda") for bk in prob.buckets]
+        tuned = None
+        if not args.no_tune:
+            from pymhm_amd import tuning
+
+            t1 = time.perf_counter()
+            choice = stage.tune(kqs, None, case.f_const)
+            names = [tuning.describe(f, r) for f, r, _ in choice.values()]
+            tuned = dict(seconds=round(time.perf_counter() - t1, 2),
+                         buckets={k: names.count(k) for k in sorted(set(names))},
+                         saved_ms_alone=round(1e3 * sum(v[2] for v in choice.values()), 2))
+            print(f"[bench] {precision}: tuned solve kernels in {tuned['seconds']}s: {tuned['buckets']}",
+                  file=sys.stderr, flush=True)
 
         def step():
             plan.gather(*stage.run(kqs, None, case.f_const, check=False))
@@ -409,7 +421,7 @@ def run_config4(args):
                               p99_ms=float(np.percentile(lat, 99) * 1e3),
                               refinement_solves_per_macro=float(iters.mean()),
                               kernel_ms_per_step={k: v[0] / args.steps for k, v in times.items()},
-                              max_rel_err_vs_oracle=err)
+                              max_rel_err_vs_oracle=err, solve_kernels_tuned=tuned)
         print(f"[bench] {precision}: {res[precision]['value']:.0f} macros/s, p50 {res[precision]['p50_ms']:.2f} ms, "
               f"err {err}", file=sys.stderr, flush=True)
     ok = [p for p in res if res[p]["max_rel_err_vs_oracle"] is not None and res[p]["max_rel_err_vs_oracle"] <= 1e-10]
@@ -450,6 +462,7 @@ def main(argv=None):
                     help="macros in the oracle CPU-baseline sample (default 768 in 2D, 256 in 3D)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--paths", default="mixed,fp64", help="precision paths to run (comma separated)")
+    ap.add_argument("--no-tune", action="store_true", help="config4: keep the automatic solve kernel choice")
     ap.add_argument("--streams", type=int, default=16,
                     help="config4: HIP streams the buckets are issued over, each on a hardware queue of its own")
     ap.add_argument("--copy-bytes", type=int, default=0,
@@ -465,7 +478,7 @@ def main(argv=None):
     if args.workload == "config4":
         # one hardware queue per bucket stream (HIP's default is 4 per process;
         # it is read when the runtime starts, so before anything touches the GPU)
-        os.environ.setdefault("GPU_MAX_HW_QUEUES", str(args.streams))
+        os.environ["GPU_MAX_HW_QUEUES"] = str(args.streams)
         return run_config4(args)
     from pymhm_amd import hybrid
     from pymhm_amd import workloads as pw

@@ -199,6 +199,19 @@ int mhm_batch_condense(mhm_batch *batch, int32_t max_refine, double rtol, double
  * queued behind it. */
 int mhm_batch_set_side_stream(mhm_batch *batch, int32_t enable);
 
+/* Band solve kernel family of the first pass of a solve (`first`) and of its
+ * refinement passes (`refine`), each one of MHM_SOLVE_*; MHM_SOLVE_AUTO picks
+ * by band width and column count.  A family that cannot serve the batch's
+ * band and column count makes the solving call fail with MHM_EINVAL.  No
+ * counterpart in the reference (SciPy picks its kernels); set by the tile
+ * autotuner (pymhm_amd/tuning.py), which times the families on the batch. */
+#define MHM_SOLVE_AUTO 0
+#define MHM_SOLVE_MFMA 1       /* binary64 matrix cores, one wavefront per macro */
+#define MHM_SOLVE_MFMA_TILES 2 /* binary64 matrix cores, one wavefront per 16-column tile */
+#define MHM_SOLVE_WAVE 3       /* one wavefront per macro, lane = column (half-bandwidth <= 95) */
+#define MHM_SOLVE_BLOCKED 4    /* four wavefronts per macro over column groups (half-bandwidth <= 255) */
+int mhm_batch_set_solve_variants(mhm_batch *batch, int32_t first, int32_t refine);
+
 /* Copy the responses of the last mhm_batch_condense in ORIGINAL row order:
  * d_source[macro][n], d_lifts[macro][n][m] (row-major like numpy (n, m)),
  * d_retained[macro][n] (E = Z - R A Z, or Z when A Z == 0).  Any may be NULL.

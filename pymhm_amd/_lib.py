@@ -16,6 +16,7 @@ LIBRARY = _HERE / "libmhm_hip.so"
 
 MHM_OK, MHM_EINVAL, MHM_ESINGULAR, MHM_ERESIDUAL, MHM_ENOMEM, MHM_EDEVICE = range(6)
 MHM_FP64, MHM_MIXED = 0, 1
+SOLVE_AUTO, SOLVE_MFMA, SOLVE_MFMA_TILES, SOLVE_WAVE, SOLVE_BLOCKED = range(5)
 STATUS_PIVOT, STATUS_RESIDUAL, STATUS_NONFINITE, STATUS_COEFFICIENT = 1, 2, 4, 8
 (DENSE_NONFINITE_MATRIX, DENSE_ZERO_ROW, DENSE_ZERO_COLUMN, DENSE_SINGULAR, DENSE_RANK, DENSE_NONFINITE_RHS,
  DENSE_NONFINITE_RESULT, DENSE_RESIDUAL) = (1, 2, 4, 8, 16, 32, 64, 128)
@@ -24,7 +25,7 @@ EXPORTS = (
     "mhm_last_error", "mhm_device_count", "mhm_version",
     "mhm_layout_create", "mhm_layout_destroy", "mhm_layout_info",
     "mhm_batch_create", "mhm_batch_destroy", "mhm_batch_assemble", "mhm_batch_set_operator",
-    "mhm_batch_set_side_stream",
+    "mhm_batch_set_side_stream", "mhm_batch_set_solve_variants",
     "mhm_batch_operator", "mhm_batch_set_coupling", "mhm_batch_set_geometry", "mhm_batch_factor", "mhm_batch_condense", "mhm_batch_responses",
     "mhm_batch_solve", "mhm_batch_sources", "mhm_batch_reconstruct", "mhm_batch_reconstruct_sources",
     "mhm_batch_status",
@@ -106,6 +107,7 @@ def load(path: str | os.PathLike | None = None):
         "mhm_batch_factor": (P, P),
         "mhm_batch_condense": (P, I32, D, P, P, P),
         "mhm_batch_set_side_stream": (P, I32),
+        "mhm_batch_set_solve_variants": (P, I32, I32),
         "mhm_batch_responses": (P, P, P, P, P),
         "mhm_batch_solve": (P, P, P, I32, I32, D, P),
         "mhm_batch_sources": (P, P, D, I32, I32, D, P, P, P),

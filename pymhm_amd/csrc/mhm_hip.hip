@@ -2984,6 +2984,7 @@ struct mhm_batch {
   hipStream_t side = nullptr;                    // early condensation, beside the caller's stream
   hipEvent_t ev_mark = nullptr, ev_side = nullptr;
   bool use_side = true;  // early condensation on the side stream
+  int solve_first = MHM_SOLVE_AUTO, solve_refine = MHM_SOLVE_AUTO;  // solve kernel family per pass kind
   double *ework = nullptr, *fwork = nullptr;
   int *colmap = nullptr;
   double *colscale = nullptr, *bcoup = nullptr, *cons = nullptr, *kern = nullptr;
@@ -3347,31 +3348,19 @@ static void launch_wave(const mhm_batch *b, int ncols, const int *colstate, int 
                      b->M, (const T *)b->band, (const T *)b->rband, (T *)b->y, colstate, iters);
 }
 
+// Band solve kernel families (mhm_batch_set_solve_variants names them):
+//   MHM_SOLVE_MFMA        binary64 matrix cores, one wavefront per macro
+//   MHM_SOLVE_MFMA_TILES  binary64 matrix cores, one wavefront per 16-column tile
+//   MHM_SOLVE_WAVE        one wavefront per macro, lane = column (b + 1 <= 96)
+//   MHM_SOLVE_BLOCKED     four wavefronts per macro over column groups (b <= 255)
+// Each launcher fails with MHM_EINVAL when its family cannot serve the batch.
 template <typename T>
-static int launch_solve(const mhm_batch *b, int ncols, const int *colstate, int *iters, hipStream_t st) {
-  const LayoutK &L = b->layout->k;
+static int launch_solve_mfma(const mhm_batch *b, int ncols, const int *colstate, int *iters, hipStream_t st) {
   if constexpr (sizeof(T) == 8) {
+    const LayoutK &L = b->layout->k;
     int maxk, nct;
     solve_mfma_shape(L.b, ncols, &maxk, &nct);
     if (b->dinv && b->rband && maxk) {
-      // refinement passes (a column state) leave few macros pending: one
-      // wavefront per tile shortens each macro's solve; the first solve of
-      // every macro keeps one wavefront per macro (the band read once)
-      int kt = 0;
-      if (colstate)
-        for (int k : {8, 12, 17, 24, 28})
-          if (kt == 0 && (L.b + 3) / 4 <= k) kt = k;
-      const int nct = (ncols + 15) / 16;
-#define MHM_TILES(K)                                                                                         \
-  if (kt == K) {                                                                                             \
-    hipLaunchKernelGGL((k_solve_mfma_tiles<K>), dim3(b->M), dim3(64 * nct), 0, st, L.n, L.b, b->ldy, ncols,   \
-                       (const double *)b->band, (const double *)b->rband, (const double *)b->dinv,         \
-                       (double *)b->y, colstate, iters);                                                   \
-    LAUNCH_CHECK("k_solve_mfma_tiles");                                                                      \
-    return MHM_OK;                                                                                           \
-  }
-      MHM_TILES(8) MHM_TILES(12) MHM_TILES(17) MHM_TILES(24) MHM_TILES(28)
-#undef MHM_TILES
       const dim3 grid((unsigned)((b->M + SOLVE_MFMA_WAVES - 1) / SOLVE_MFMA_WAVES));
 #define MHM_MFMA(K, C)                                                                                      \
   if (maxk == K && nct == C) {                                                                              \
@@ -3388,20 +3377,52 @@ static int launch_solve(const mhm_batch *b, int ncols, const int *colstate, int 
 #undef MHM_MFMA
     }
   }
+  return fail(MHM_EINVAL, "no matrix-core solve for this precision, band and column count");
+}
+
+template <typename T>
+static int launch_solve_tiles(const mhm_batch *b, int ncols, const int *colstate, int *iters, hipStream_t st) {
+  if constexpr (sizeof(T) == 8) {
+    const LayoutK &L = b->layout->k;
+    int kt = 0;
+    if (b->dinv && b->rband && ncols >= 1 && ncols <= 64)
+      for (int k : {8, 12, 17, 24, 28})
+        if (kt == 0 && (L.b + 3) / 4 <= k) kt = k;
+    const int nct = (ncols + 15) / 16;
+#define MHM_TILES(K)                                                                                         \
+  if (kt == K) {                                                                                             \
+    hipLaunchKernelGGL((k_solve_mfma_tiles<K>), dim3(b->M), dim3(64 * nct), 0, st, L.n, L.b, b->ldy, ncols,   \
+                       (const double *)b->band, (const double *)b->rband, (const double *)b->dinv,         \
+                       (double *)b->y, colstate, iters);                                                   \
+    LAUNCH_CHECK("k_solve_mfma_tiles");                                                                      \
+    return MHM_OK;                                                                                           \
+  }
+    MHM_TILES(8) MHM_TILES(12) MHM_TILES(17) MHM_TILES(24) MHM_TILES(28)
+#undef MHM_TILES
+  }
+  return fail(MHM_EINVAL, "no matrix-core tile solve for this precision, band and column count");
+}
+
+template <typename T>
+static int launch_solve_wave(const mhm_batch *b, int ncols, const int *colstate, int *iters, hipStream_t st) {
+  const LayoutK &L = b->layout->k;
   switch (b->rband ? wave_width(L.b, ncols) : 0) {
-#define MHM_WAVE(w)                                \
-  case w:                                          \
+#define MHM_WAVE(w)                                   \
+  case w:                                             \
     launch_wave<T, w>(b, ncols, colstate, iters, st); \
-    LAUNCH_CHECK("k_solve_wave");                  \
+    LAUNCH_CHECK("k_solve_wave");                     \
     return MHM_OK;
     MHM_WAVE(8) MHM_WAVE(16) MHM_WAVE(24) MHM_WAVE(32) MHM_WAVE(48) MHM_WAVE(64) MHM_WAVE(68) MHM_WAVE(80)
     MHM_WAVE(96)
 #undef MHM_WAVE
     default:
-      break;
+      return fail(MHM_EINVAL, "the band is too wide for the wavefront solve");
   }
-  if (solve_blk_ok(L.b, ncols, sizeof(T), lds_limit(b->layout->device)))
-    return launch_solve_blk<T>(b, ncols, colstate, iters, st);
+}
+
+template <typename T>
+static int launch_solve_generic(const mhm_batch *b, int ncols, const int *colstate, int *iters, hipStream_t st) {
+  const LayoutK &L = b->layout->k;
   const size_t shm = solve_lds(L.b, ncols, sizeof(T));
   const int slots = solve_slots(L.b, ncols);
   const dim3 grid(b->M);
@@ -3439,6 +3460,42 @@ static int launch_solve(const mhm_batch *b, int ncols, const int *colstate, int 
 #undef MHM_SOLVE
   LAUNCH_CHECK("k_solve");
   return MHM_OK;
+}
+
+// The family a call uses: the batch's override for its pass kind (first
+// pass: no column state; refinement: a column state), else the first that
+// fits of the matrix-core solve (binary64; one wavefront per tile on
+// refinement passes, which leave few macros pending, one per macro on the
+// first pass, which reads the band once), the wavefront solve, the blocked
+// solve and the generic one.
+template <typename T>
+static int launch_solve(const mhm_batch *b, int ncols, const int *colstate, int *iters, hipStream_t st) {
+  const LayoutK &L = b->layout->k;
+  switch (colstate ? b->solve_refine : b->solve_first) {
+    case MHM_SOLVE_MFMA:
+      return launch_solve_mfma<T>(b, ncols, colstate, iters, st);
+    case MHM_SOLVE_MFMA_TILES:
+      return launch_solve_tiles<T>(b, ncols, colstate, iters, st);
+    case MHM_SOLVE_WAVE:
+      return launch_solve_wave<T>(b, ncols, colstate, iters, st);
+    case MHM_SOLVE_BLOCKED:
+      if (!solve_blk_ok(L.b, ncols, sizeof(T), lds_limit(b->layout->device)))
+        return fail(MHM_EINVAL, "the blocked solve does not fit this band and column count");
+      return launch_solve_blk<T>(b, ncols, colstate, iters, st);
+    default:
+      break;
+  }
+  if constexpr (sizeof(T) == 8) {
+    int maxk, nct;
+    solve_mfma_shape(L.b, ncols, &maxk, &nct);
+    if (b->dinv && b->rband && maxk)
+      return colstate ? launch_solve_tiles<T>(b, ncols, colstate, iters, st)
+                      : launch_solve_mfma<T>(b, ncols, colstate, iters, st);
+  }
+  if (b->rband && wave_width(L.b, ncols)) return launch_solve_wave<T>(b, ncols, colstate, iters, st);
+  if (solve_blk_ok(L.b, ncols, sizeof(T), lds_limit(b->layout->device)))
+    return launch_solve_blk<T>(b, ncols, colstate, iters, st);
+  return launch_solve_generic<T>(b, ncols, colstate, iters, st);
 }
 
 
@@ -4240,6 +4297,15 @@ int mhm_batch_condense(mhm_batch *b, int32_t max_refine, double rtol, double *bl
                      loads, (const int *)b->done, early ? 2 : 0);
   LAUNCH_CHECK("k_condense");
   b->condensed = true;
+  return MHM_OK;
+}
+
+int mhm_batch_set_solve_variants(mhm_batch *b, int32_t first, int32_t refine) {
+  if (!b) return fail(MHM_EINVAL, "batch is NULL");
+  for (int v : {first, refine})
+    if (v < MHM_SOLVE_AUTO || v > MHM_SOLVE_BLOCKED) return fail(MHM_EINVAL, "unknown solve kernel family");
+  b->solve_first = first;
+  b->solve_refine = refine;
   return MHM_OK;
 }
 

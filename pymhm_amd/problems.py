@@ -17,6 +17,7 @@ import numpy as np
 
 from . import elements, meshing
 from .layout import LocalLayout, choose_permutation, element_pattern
+from . import _lib
 from .stage import LocalBatch
 
 
@@ -339,6 +340,27 @@ class BucketedStage:
         for b in self.batches.values():  # the buckets' own streams overlap them; no side streams
             b.set_side_stream(False)
         self.streams = [torch.cuda.Stream() for _ in range(max(1, int(streams)))]
+
+    def tune(self, kqs, fqs=None, f_const: float = 0.0, repeats: int = 3):
+        """Choose every bucket's band solve kernels with ``tuning.tune_batch``
+        (assembled once, then factor and condense timed per candidate);
+        returns ``{bucket: (first, refine, seconds saved alone)}``."""
+        from . import tuning
+
+        out = {}
+        for j, i in enumerate(self.order):
+            b = self.batches[i]
+            b.assemble(kqs[i], None if fqs is None else fqs[i], f_const)
+
+            def run(b=b):
+                b.factor(check=False)
+                b.condense(check=False)
+
+            first, refine, timings = tuning.tune_batch(b, run, repeats=repeats)
+            auto = timings.get((_lib.SOLVE_AUTO, _lib.SOLVE_AUTO))
+            best = timings.get((first, refine))
+            out[i] = (first, refine, (auto - best) if auto is not None and best is not None else 0.0)
+        return out
 
     def positions(self):
         """Block and load start of every macro in the flat buffers of ``run``."""

@@ -54,6 +54,7 @@ class LocalBatch:
         _lib.call("mhm_batch_create", layout.handle, self.macros, PRECISIONS[precision],
                   self.max_columns, ctypes.byref(handle))
         self.handle = handle
+        self.solve_variants = (_lib.SOLVE_AUTO, _lib.SOLVE_AUTO)
         self._keep = []
         self.blocks = None
         self.loads = None
@@ -185,6 +186,13 @@ class LocalBatch:
         if check:
             self.raise_on_failure("condense")
         return self.blocks, self.loads
+
+    def set_solve_variants(self, first: int = _lib.SOLVE_AUTO, refine: int = _lib.SOLVE_AUTO):
+        """Band solve kernel family (``_lib.SOLVE_*``) of the first and of the
+        refinement passes; ``SOLVE_AUTO`` picks by band width and column count.
+        A family that cannot serve the batch fails the next solving call."""
+        _lib.call("mhm_batch_set_solve_variants", self.handle, int(first), int(refine))
+        self.solve_variants = (int(first), int(refine))
 
     def set_side_stream(self, enable: bool):
         """Condense the macros that converge early on the batch's side stream
