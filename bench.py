@@ -276,13 +276,19 @@ def run_config5(args):
     rate = args.rate
     for precision in (["fp64"] + [p for p in paths if p != "fp64"]) if "fp64" in paths else paths:
         srv = serving.Server(cases, precision, max_columns=cols, source_pool=pool)
+
+        def note(done, total, secs, what, precision=precision):
+            print(f"[bench] {precision} {what}: {done}/{total} requests in {secs:.1f}s", file=sys.stderr, flush=True)
+
         try:
             srv.run(serving.poisson_requests(24, 1e6, 3, frac, seed=1), realtime=False)  # warm-up
-            sat = srv.run(serving.poisson_requests(count, 1e6, 3, frac, seed=2), realtime=False)
+            sat = srv.run(serving.poisson_requests(count, 1e6, 3, frac, seed=2), realtime=False,
+                          progress=lambda d, t, s: note(d, t, s, "saturated"))
             capacity = count / max(r.done for r in sat)
             if rate is None:
                 rate = 0.5 * capacity
-            live = srv.run(serving.poisson_requests(count, rate, 3, frac, seed=3), realtime=True)
+            live = srv.run(serving.poisson_requests(count, rate, 3, frac, seed=3), realtime=True,
+                           progress=lambda d, t, s: note(d, t, s, f"at {rate:.1f} req/s"))
             st = serving.latency_stats(live)
             failed = sum(r.error is not None for r in sat + live)
             memory = dict(resident_bytes=srv.resident_bytes, hbm_budget=srv.hbm_budget, peak_inflight_bytes=srv.peak_bytes)

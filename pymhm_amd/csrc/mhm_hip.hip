@@ -1288,7 +1288,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
 //      v_readlane, as in k_factor_panel) and leaves it in LDS, transposed
 //      (LpT[t * ldq + q] = L(j0 + q, j0 + t));
 //  (b) the panel goes back to the column band (and to the row band if one is
-//      kept);
+//      kept, and in binary64 its 16 x 16 diagonal block, packed (tri16) and
+//      identity-padded past row n - 1, to dblk for the matrix-core solve);
 //  (c) the block applies the rank-NB update to the trailing triangle in place
 //      in the band, 4x4 register tiles; consecutive threads take consecutive
 //      row tiles of a tile column, so their band reads and writes are
@@ -1309,7 +1310,7 @@ static bool factor_wide_ok(int b, size_t tsize, int limit) {
 template <typename T>
 __global__ void __launch_bounds__(256)
     k_factor_wide(int n, int b, int ldq, T *__restrict__ band, T *__restrict__ rband, int *__restrict__ status,
-                  double tiny) {
+                  double tiny, T *__restrict__ dblk) {
   constexpr int NB = FPANEL, NR = WIDE_ROWS;
   extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
   const int W = b + 1, PR = NB + b;
@@ -1379,6 +1380,13 @@ __global__ void __launch_bounds__(256)
         const int q = e / NB, t = e - q * NB;
         const int R = j0 + q, r = q - t;
         if (t < np && r >= 0 && r <= b && R < n) Rg[(size_t)R * W + r] = LpT[t * ldq + q];
+      }
+    if (dblk)
+      for (int e = tid; e < NB * NB; e += 256) {
+        const int rho = e >> 4, sig = e & 15;
+        if (sig <= rho)
+          dblk[((size_t)mac * ((n + NB - 1) / NB) + j0 / NB) * DBLK + tri16(rho, sig)] =
+              rho < np ? LpT[sig * ldq + rho] : (rho == sig ? T(1) : T(0));
       }
     // (c) trailing update: columns qc in [NB, PR), rows qr in [qc, PR) with
     // j0 + qr < n; tiles numbered down the tile columns of the lower triangle
@@ -2023,6 +2031,37 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
   double *Y = y + (size_t)mac * n * ldy;
   mfma_sweep<MAXK, 1, 1>(n, b, ldy, ncols, band + mac * nW, Di, Y, lane, 16 * wave);
   mfma_sweep<MAXK, 1, -1>(n, b, ldy, ncols, rband + mac * nW, Di, Y, lane, 16 * wave);
+}
+
+// The tile solve for bands past b = 111 (36 or 48 k-steps): its operand and
+// window registers alone take 144 or 192 of the 256 VGPRs that two waves per
+// SIMD leave, so it runs one wave per SIMD with the whole register file.
+template <int MAXK>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1)))
+    k_solve_mfma_tiles_wide(int n, int b, int ldy, int ncols, const double *__restrict__ band,
+                            const double *__restrict__ rband, const double *__restrict__ dinv,
+                            double *__restrict__ y, const int *__restrict__ colstate, int *__restrict__ iters) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int mac = blockIdx.x;
+  if (colstate) {
+    const bool pending = lane < ncols && colstate[(size_t)mac * ncols + lane] == 0;
+    if (!__any(pending)) return;
+  }
+  if (iters && threadIdx.x == 0) iters[mac] += 1;
+  const size_t nW = (size_t)n * (b + 1);
+  const double *Di = dinv + (size_t)mac * ((n + 15) / 16) * DBLK;
+  double *Y = y + (size_t)mac * n * ldy;
+  mfma_sweep<MAXK, 1, 1>(n, b, ldy, ncols, band + mac * nW, Di, Y, lane, 16 * wave);
+  mfma_sweep<MAXK, 1, -1>(n, b, ldy, ncols, rband + mac * nW, Di, Y, lane, 16 * wave);
+}
+
+// Register window of the one-tile-per-wavefront matrix-core solve: the
+// smallest instance of at least ceil(b / 4) k-steps (48 doubles of operands
+// and 48 of window per lane at most), 0 past b = 191.
+static int solve_tile_window(int b) {
+  for (int k : {8, 12, 17, 24, 28, 36, 48})
+    if (b >= 1 && (b + 3) / 4 <= k) return k;
+  return 0;
 }
 
 // Register window of the matrix-core solve: MAXK k-steps x NCT column tiles
@@ -3384,10 +3423,7 @@ template <typename T>
 static int launch_solve_tiles(const mhm_batch *b, int ncols, const int *colstate, int *iters, hipStream_t st) {
   if constexpr (sizeof(T) == 8) {
     const LayoutK &L = b->layout->k;
-    int kt = 0;
-    if (b->dinv && b->rband && ncols >= 1 && ncols <= 64)
-      for (int k : {8, 12, 17, 24, 28})
-        if (kt == 0 && (L.b + 3) / 4 <= k) kt = k;
+    const int kt = (b->dinv && b->rband && ncols >= 1 && ncols <= 64) ? solve_tile_window(L.b) : 0;
     const int nct = (ncols + 15) / 16;
 #define MHM_TILES(K)                                                                                         \
   if (kt == K) {                                                                                             \
@@ -3399,6 +3435,16 @@ static int launch_solve_tiles(const mhm_batch *b, int ncols, const int *colstate
   }
     MHM_TILES(8) MHM_TILES(12) MHM_TILES(17) MHM_TILES(24) MHM_TILES(28)
 #undef MHM_TILES
+#define MHM_TILES_WIDE(K)                                                                                     \
+  if (kt == K) {                                                                                              \
+    hipLaunchKernelGGL((k_solve_mfma_tiles_wide<K>), dim3(b->M), dim3(64 * nct), 0, st, L.n, L.b, b->ldy,      \
+                       ncols, (const double *)b->band, (const double *)b->rband, (const double *)b->dinv,   \
+                       (double *)b->y, colstate, iters);                                                    \
+    LAUNCH_CHECK("k_solve_mfma_tiles_wide");                                                                  \
+    return MHM_OK;                                                                                            \
+  }
+    MHM_TILES_WIDE(36) MHM_TILES_WIDE(48)
+#undef MHM_TILES_WIDE
   }
   return fail(MHM_EINVAL, "no matrix-core tile solve for this precision, band and column count");
 }
@@ -3491,6 +3537,10 @@ static int launch_solve(const mhm_batch *b, int ncols, const int *colstate, int 
     if (b->dinv && b->rband && maxk)
       return colstate ? launch_solve_tiles<T>(b, ncols, colstate, iters, st)
                       : launch_solve_mfma<T>(b, ncols, colstate, iters, st);
+    // bands too wide for a whole macro in one wavefront's registers: one
+    // wavefront per 16-column tile on every pass
+    if (b->dinv && b->rband && solve_tile_window(L.b))
+      return launch_solve_tiles<T>(b, ncols, colstate, iters, st);
   }
   if (b->rband && wave_width(L.b, ncols)) return launch_solve_wave<T>(b, ncols, colstate, iters, st);
   if (solve_blk_ok(L.b, ncols, sizeof(T), lds_limit(b->layout->device)))
@@ -3791,19 +3841,22 @@ int mhm_batch_create(const mhm_layout *L, int32_t macros, int32_t precision, int
     unsigned char *p = nullptr;
     if ((rc = dev_alloc(b->owned, &p, M * n * lk.W * tsize))) return cleanup(rc);
     b->band = p;
-    if (wave_width(lk.b, widest) > 0 || wave_width(lk.b, 2) > 0) {
+    // the binary64 matrix-core solve needs the row band and the diagonal
+    // blocks, which the LDS-window and the wide-band factors write
+    const bool panel = factor_panel_ok(lk.b, tsize, dev);
+    const bool mfma = precision == MHM_FP64 && solve_tile_window(lk.b) > 0 &&
+                      (panel || factor_wide_ok(lk.b, tsize, limit));
+    if (wave_width(lk.b, widest) > 0 || wave_width(lk.b, 2) > 0 || mfma) {
       if ((rc = dev_alloc(b->owned, &p, M * n * lk.W * tsize))) return cleanup(rc);
       b->rband = p;
     }
     if ((rc = dev_alloc(b->owned, &p, M * n * b->ldy * tsize))) return cleanup(rc);
     b->y = p;
-    if (factor_panel_ok(lk.b, tsize, dev)) {
+    if (panel) {
       if ((rc = dev_alloc(b->owned, &p, M * lk.nnz * tsize))) return cleanup(rc);
       b->sval = p;
-      if (precision == MHM_FP64 && b->rband &&
-          (rc = dev_alloc(b->owned, &b->dinv, M * (size_t)((n + 15) / 16) * DBLK)))
-        return cleanup(rc);
     }
+    if (mfma && (rc = dev_alloc(b->owned, &b->dinv, M * (size_t)((n + 15) / 16) * DBLK))) return cleanup(rc);
   }
   HIP_TRY(hipMemset(k.status, 0, M * sizeof(int)));
   HIP_TRY(hipMemset(k.cstatus, 0, M * sizeof(int)));
@@ -4107,7 +4160,7 @@ static int run_factor(mhm_batch *b, hipStream_t st) {
     if (wshm > 65536)
       HIP_TRY(set_max_lds((const void *)k_factor_wide<T>, wshm));
     hipLaunchKernelGGL((k_factor_wide<T>), dim3(b->M), dim3(256), wshm, st, L.n, L.b, wide_ldq(L.b),
-                       (T *)b->band, (T *)b->rband, b->bk.status, tiny);
+                       (T *)b->band, (T *)b->rband, b->bk.status, tiny, sizeof(T) == 8 ? (T *)b->dinv : (T *)nullptr);
   } else if (L.b >= 12) {
     if (shm > 65536)
       HIP_TRY(set_max_lds((const void *)k_factor<T, 256>, shm));

@@ -257,17 +257,20 @@ class Server:
         for r, res in zip(job.reqs, job.results):
             r.done, r.result, r.error = done, (None if err else res), err
 
-    def run(self, requests: list[Request], realtime: bool = True) -> list[Request]:
+    def run(self, requests: list[Request], realtime: bool = True, progress=None) -> list[Request]:
         """Serve ``requests`` (sorted by arrival).  With ``realtime`` arrivals
         follow the wall clock (a request is only scheduled once its arrival
         time has passed); otherwise all are queued at once.  Sets ``done``
         (seconds after the start, when the host saw the batch complete),
-        ``result`` and ``error`` on every request."""
+        ``result`` and ``error`` on every request.  ``progress(finished,
+        total, seconds)`` (optional) is called after every thousandth
+        completed request."""
         import torch
 
         sched = Scheduler(len(self.cases), self.max_columns)
         order = sorted(requests, key=lambda r: (r.arrival, r.rid))
         nxt = finished = used = 0
+        reported = 0
         inflight: dict[int, _Job] = {}
         t0 = time.perf_counter()
         while finished < len(order):
@@ -284,6 +287,9 @@ class Server:
                     finished += len(job.reqs)
                     del inflight[p]
                     progressed = True
+                    if progress is not None and finished // 1000 > reported:
+                        reported = finished // 1000
+                        progress(finished, len(order), time.perf_counter() - t0)
             for p in sched.ready(inflight):
                 kind, run = sched.head_run(p)
                 per = self.request_bytes(p, kind, 1)
