@@ -1281,26 +1281,33 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
 
 // ----------------------------------------------------------------- wide-band Cholesky
 // Bands too wide for the LDS window of k_factor_panel (NB + b > 127): the same
-// blocked right-looking algorithm with the band array itself as the window.
-// Per panel of FPANEL columns:
-//  (a) wavefront 0 reads the panel (PR = NB + b <= 256 rows, four per lane)
-//      from the band, factors it in registers (pivots and multipliers by
+// blocked right-looking algorithm with the band array itself as the window,
+// in pairs of panels of FPANEL columns (A at j0, B at j0 + NB):
+//  (a) wavefront 0 reads panel A (PR = NB + b <= 256 rows, four per lane) from
+//      the band, factors it in registers (pivots and multipliers by
 //      v_readlane, as in k_factor_panel) and leaves it in LDS, transposed
-//      (LpT[t * ldq + q] = L(j0 + q, j0 + t));
-//  (b) the panel goes back to the column band (and to the row band if one is
-//      kept, and in binary64 its 16 x 16 diagonal block, packed (tri16) and
-//      identity-padded past row n - 1, to dblk for the matrix-core solve);
-//  (c) the block applies the rank-NB update to the trailing triangle in place
-//      in the band, 4x4 register tiles; consecutive threads take consecutive
-//      row tiles of a tile column, so their band reads and writes are
-//      contiguous and their LDS reads of LpT are conflict-free.
-// Block barriers order the in-place updates between panels.
+//      (LpT[t * ldq + q] = L(j0 + q, j0 + t), rows relative to j0); then it
+//      reads panel B, subtracts panel A's update from B's own columns (the
+//      rank-16 update restricted to them, operands from LDS) and factors B
+//      into LpT columns NB .. 2 NB - 1 (rows NB .. PR2 - 1, PR2 = 2 NB + b);
+//  (b) both panels go back to the column band (and to the row band if one is
+//      kept, and in binary64 their 16 x 16 diagonal blocks, packed (tri16)
+//      and identity-padded past row n - 1, to dblk for the matrix-core solve);
+//  (c) the block applies the rank-2NB update of the pair to the trailing
+//      triangle in place in the band, 4x4 register tiles; consecutive threads
+//      take consecutive row tiles of a tile column, so their band reads and
+//      writes are contiguous and their LDS reads of LpT are conflict-free.
+// The trailing triangle (about b^2 / 2 entries read and written per update)
+// is touched once per pair instead of once per panel.  LpT entries outside
+// the band (rows PR .. PR2 - 1 of A's columns, rows 0 .. NB - 1 of B's) are
+// zeroed once and never written.  Block barriers order the in-place updates
+// between pairs.
 #define WIDE_ROWS 4  // panel rows per lane of wavefront 0
 
-static int wide_ldq(int b) { return ((FPANEL + b + 3) & ~3) + 4; }
+static int wide_ldq(int b) { return ((2 * FPANEL + b + 3) & ~3) + 4; }
 
 static size_t factor_wide_lds(int b, size_t tsize) {
-  return ((size_t)FPANEL * wide_ldq(b) + 64) * tsize;
+  return ((size_t)2 * FPANEL * wide_ldq(b) + 64) * tsize;
 }
 
 static bool factor_wide_ok(int b, size_t tsize, int limit) {
@@ -1313,13 +1320,14 @@ __global__ void __launch_bounds__(256)
                   double tiny, T *__restrict__ dblk) {
   constexpr int NB = FPANEL, NR = WIDE_ROWS;
   extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
-  const int W = b + 1, PR = NB + b;
-  T *LpT = reinterpret_cast<T *>(smem_raw);  // NB x ldq
+  const int W = b + 1, PR = NB + b, PR2 = 2 * NB + b;
+  T *LpT = reinterpret_cast<T *>(smem_raw);  // 2 NB x ldq
   __shared__ int bad;
   const int mac = blockIdx.x, tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   T *Lg = band + (size_t)mac * n * W;
   T *Rg = rband ? rband + (size_t)mac * n * W : nullptr;
   if (tid == 0) bad = 0;
+  for (int e = tid; e < 2 * NB * ldq; e += 256) LpT[e] = T(0);
   if (Rg)  // entries L(j, j-r) with j - r < 0
     for (int e = tid; e < min(b, n) * W; e += 256) {
       const int j = e / W, r = e - j * W;
@@ -1328,68 +1336,96 @@ __global__ void __launch_bounds__(256)
   const int side = (b + 3) / 4;  // tile rows/columns of the trailing triangle
   const int ntiles = side * (side + 1) / 2;
   __syncthreads();
-  for (int j0 = 0; j0 < n; j0 += NB) {
-    const int np = min(NB, n - j0);
-    // (a) panel factorisation, wavefront 0
-    if (wave == 0) {
-      T a[NR][NB];
+  // Wavefront 0: factor the panel of columns jc .. jc + np - 1 (rows jc + q,
+  // q = lane + 64 v < PR) into LpT columns t0 + t, rows q0 + q.  With upd,
+  // first subtract the update of the panel already in LpT columns 0 .. NB - 1
+  // (rows relative to jc - NB).
+  auto factor_rows = [&](int jc, int np, int t0, int q0, bool upd) {
+    T a[NR][NB];
 #pragma unroll
-      for (int v = 0; v < NR; ++v)
-#pragma unroll
-        for (int t = 0; t < NB; ++t) {
-          const int q = lane + 64 * v;
-          a[v][t] = (t < np && q >= t && q - t <= b && j0 + q < n) ? Lg[(size_t)(j0 + t) * W + (q - t)] : T(0);
-        }
+    for (int v = 0; v < NR; ++v)
 #pragma unroll
       for (int t = 0; t < NB; ++t) {
-        if (t < np) {
-          T d = lane_bcast(a[0][t], t);  // pivot, held by lane t
-          if (!(d > T(tiny)) || !isfinite((double)d)) {
-            if (lane == 0) bad = MHM_STATUS_PIVOT;
-            d = T(1);
-          }
-          const T inv = rsqrt_nr(d);
-          const T root = d * inv;
-          a[0][t] = lane > t ? a[0][t] * inv : (lane == t ? root : T(0));
+        const int q = lane + 64 * v;
+        a[v][t] = (t < np && q >= t && q - t <= b && jc + q < n) ? Lg[(size_t)(jc + t) * W + (q - t)] : T(0);
+      }
+    if (upd) {
+#pragma unroll 1
+      for (int k = 0; k < NB; ++k) {
+        const T *Ak = LpT + k * ldq + NB;  // L(jc + q, jc - NB + k) at Ak[q]
+        T lk[NR];
 #pragma unroll
-          for (int v = 1; v < NR; ++v) a[v][t] = a[v][t] * inv;  // rows >= 64 > t
+        for (int v = 0; v < NR; ++v) lk[v] = (lane + 64 * v < PR) ? Ak[lane + 64 * v] : T(0);
 #pragma unroll
-          for (int v = 0; v < NR; ++v)
-            if (lane + 64 * v < PR) LpT[t * ldq + lane + 64 * v] = a[v][t];
+        for (int t = 0; t < NB; ++t) {
+          const T m = Ak[t];  // L(jc + t, jc - NB + k), the same for every lane
 #pragma unroll
-          for (int u = t + 1; u < NB; ++u) {
-            const T l = lane_bcast(a[0][t], u);  // L(j0 + u, j0 + t), held by lane u
-#pragma unroll
-            for (int v = 0; v < NR; ++v) a[v][u] -= a[v][t] * l;
-          }
-        } else {
-#pragma unroll
-          for (int v = 0; v < NR; ++v)
-            if (lane + 64 * v < PR) LpT[t * ldq + lane + 64 * v] = T(0);
+          for (int v = 0; v < NR; ++v) a[v][t] -= lk[v] * m;
         }
       }
     }
+#pragma unroll
+    for (int t = 0; t < NB; ++t) {
+      if (t < np) {
+        T d = lane_bcast(a[0][t], t);  // pivot, held by lane t
+        if (!(d > T(tiny)) || !isfinite((double)d)) {
+          if (lane == 0) bad = MHM_STATUS_PIVOT;
+          d = T(1);
+        }
+        const T inv = rsqrt_nr(d);
+        const T root = d * inv;
+        a[0][t] = lane > t ? a[0][t] * inv : (lane == t ? root : T(0));
+#pragma unroll
+        for (int v = 1; v < NR; ++v) a[v][t] = a[v][t] * inv;  // rows >= 64 > t
+#pragma unroll
+        for (int v = 0; v < NR; ++v)
+          if (lane + 64 * v < PR) LpT[(t0 + t) * ldq + q0 + lane + 64 * v] = a[v][t];
+#pragma unroll
+        for (int u = t + 1; u < NB; ++u) {
+          const T l = lane_bcast(a[0][t], u);  // L(jc + u, jc + t), held by lane u
+#pragma unroll
+          for (int v = 0; v < NR; ++v) a[v][u] -= a[v][t] * l;
+        }
+      } else {
+#pragma unroll
+        for (int v = 0; v < NR; ++v)
+          if (lane + 64 * v < PR) LpT[(t0 + t) * ldq + q0 + lane + 64 * v] = T(0);
+      }
+    }
+  };
+  for (int j0 = 0; j0 < n; j0 += 2 * NB) {
+    const int npA = min(NB, n - j0);
+    const int npB = max(0, min(NB, n - j0 - NB));
+    const int np2 = npA + npB;
+    // (a) both panels, wavefront 0 (its LDS accesses execute in order, so B
+    // reads A's columns without a barrier)
+    if (wave == 0) {
+#pragma unroll 1
+      for (int p = 0; p < 2; ++p) factor_rows(j0 + p * NB, p ? npB : npA, p * NB, p * NB, p == 1);
+    }
     __syncthreads();
-    // (b) the panel to the column band and the row band
-    for (int e = tid; e < np * W; e += 256) {
+    // (b) the panels to the column band, the row band and the diagonal blocks
+    for (int e = tid; e < np2 * W; e += 256) {
       const int t = e / W, r = e - t * W;
       Lg[(size_t)j0 * W + e] = LpT[t * ldq + t + r];
     }
     if (Rg)
-      for (int e = tid; e < PR * NB; e += 256) {
-        const int q = e / NB, t = e - q * NB;
+      for (int e = tid; e < PR2 * 2 * NB; e += 256) {
+        const int q = e / (2 * NB), t = e - q * (2 * NB);
         const int R = j0 + q, r = q - t;
-        if (t < np && r >= 0 && r <= b && R < n) Rg[(size_t)R * W + r] = LpT[t * ldq + q];
+        if (t < np2 && r >= 0 && r <= b && R < n) Rg[(size_t)R * W + r] = LpT[t * ldq + q];
       }
     if (dblk)
-      for (int e = tid; e < NB * NB; e += 256) {
-        const int rho = e >> 4, sig = e & 15;
-        if (sig <= rho)
-          dblk[((size_t)mac * ((n + NB - 1) / NB) + j0 / NB) * DBLK + tri16(rho, sig)] =
-              rho < np ? LpT[sig * ldq + rho] : (rho == sig ? T(1) : T(0));
+      for (int e = tid; e < 2 * NB * NB; e += 256) {
+        const int p = e >> 8, rho = (e >> 4) & 15, sig = e & 15;
+        const int np = p ? npB : npA;
+        if (sig <= rho && (p == 0 || npB > 0))
+          dblk[((size_t)mac * ((n + NB - 1) / NB) + j0 / NB + p) * DBLK + tri16(rho, sig)] =
+              rho < np ? LpT[(p * NB + sig) * ldq + p * NB + rho] : (rho == sig ? T(1) : T(0));
       }
-    // (c) trailing update: columns qc in [NB, PR), rows qr in [qc, PR) with
-    // j0 + qr < n; tiles numbered down the tile columns of the lower triangle
+    // (c) trailing update: columns qc in [2 NB, PR2), rows qr in [qc, PR2)
+    // with j0 + qr < n; tiles numbered down the tile columns of the lower
+    // triangle
     for (int id = tid; id < ntiles; id += 256) {
       const float s2 = 2.0f * side + 1.0f;
       int tc = (int)((s2 - sqrtf(s2 * s2 - 8.0f * id)) * 0.5f);
@@ -1397,7 +1433,8 @@ __global__ void __launch_bounds__(256)
       while (tc > 0 && tc * side - tc * (tc - 1) / 2 > id) --tc;
       while (tc + 1 < side && (tc + 1) * side - (tc + 1) * tc / 2 <= id) ++tc;
       const int tr = tc + (id - (tc * side - tc * (tc - 1) / 2));
-      const int qr0 = NB + 4 * tr, qc0 = NB + 4 * tc;
+      const int qr0 = 2 * NB + 4 * tr, qc0 = 2 * NB + 4 * tc;
+      if (j0 + qr0 >= n) continue;  // every row of the tile is past the matrix
       T acc[4][4];
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
@@ -1405,11 +1442,11 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
           const int qr = qr0 + i, qc = qc0 + j;
-          acc[i][j] = (qr < PR && qr >= qc && j0 + qr < n) ? col[qr] : T(0);
+          acc[i][j] = (qr < PR2 && qr >= qc && j0 + qr < n) ? col[qr] : T(0);
         }
       }
 #pragma unroll 4
-      for (int t = 0; t < NB; ++t) {
+      for (int t = 0; t < 2 * NB; ++t) {
         T ar[4], bc[4];
 #pragma unroll
         for (int i = 0; i < 4; ++i) ar[i] = LpT[t * ldq + qr0 + i];
@@ -1426,7 +1463,7 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
           const int qr = qr0 + i, qc = qc0 + j;
-          if (qr < PR && qr >= qc && j0 + qr < n) col[qr] = acc[i][j];
+          if (qr < PR2 && qr >= qc && j0 + qr < n) col[qr] = acc[i][j];
         }
       }
     }
