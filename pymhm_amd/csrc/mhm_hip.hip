@@ -1047,10 +1047,26 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
   auto slot_of = [&](int j, int r) -> int { return (j < n && j + r < n) ? bslot[(size_t)j * W + r] : -1; };
   const FastDiv divW(W);
   if (tid == 0) bad = 0;
-  for (int e = tid; e < S * W; e += 256) {  // columns 0 .. S-1 in slots 0 .. S-1
-    const int c = divW(e), r = e - c * W;
-    const int q = slot_of(c, r);
-    win[c * ldw + r] = q >= 0 ? sv[q] : T(0);
+  // columns 0 .. S-1 in slots 0 .. S-1, eight elements per thread at a time:
+  // all slot indices, then all values, then the stores (a loop of dependent
+  // pairs would wait for two round trips per element)
+  for (int e0 = 0; e0 < S * W; e0 += 8 * 256) {
+    int qv[8];
+#pragma unroll
+    for (int v = 0; v < 8; ++v) {
+      const int e = e0 + tid + 256 * v;
+      const int c = divW(e), r = e - c * W;
+      qv[v] = e < S * W ? slot_of(c, r) : -1;
+    }
+    T x[8];
+#pragma unroll
+    for (int v = 0; v < 8; ++v) x[v] = global_ptr(sv)[qv[v] >= 0 ? qv[v] : 0];
+#pragma unroll
+    for (int v = 0; v < 8; ++v) {
+      const int e = e0 + tid + 256 * v;
+      const int c = divW(e), r = e - c * W;
+      if (e < S * W) win[c * ldw + r] = qv[v] >= 0 ? x[v] : T(0);
+    }
   }
   for (int e = tid; e < 2 * NB * ldq; e += 256) LpT0[e] = T(0);
   if (Rg)  // entries L(j, j-r) with j - r < 0
