@@ -56,6 +56,51 @@ def test_sources_against_oracle_and_batch_independent(which, precision):
         assert rel(stacked, ref_stacked) < TOL, c
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_sources_config2_mesh(precision):
+    """New sources on the config-2 local shape (P2, 16x16x2 triangles, P1
+    multipliers) against the oracle, and a column's bits alone and in a batch."""
+    case = pw.config2(nx=2)
+    prob = case.problem
+    print(f"sources config2 2x2 {precision}", flush=True)
+    pts = torch.as_tensor(prob.quadrature_points(), dtype=torch.float64, device="cuda")
+    fq = serving.bump_sources(pts, [5, 6])
+    with prob.batch(precision, max_columns=2) as b:
+        b.assemble(case.kq, None, case.f_const)
+        b.factor()
+        b.condense()
+        loads = b.sources(fq).cpu().numpy()
+        alone = b.sources(fq[:1]).cpu().numpy()
+    assert np.array_equal(alone[0], loads[0])
+    for c in range(2):
+        ref = ow.oracle_config(dataclasses.replace(case, fq=fq[c].cpu().numpy(), f_const=0.0))
+        assert rel(loads[c], ref["loads"]) < TOL, c
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_sources_uneven_macros(precision):
+    """New sources on Voronoi buckets (per-macro geometry and element measures
+    in the source load) against the oracle."""
+    case = pw.config4(npts=12, h=1 / 64, dof_range=(30, 600))
+    prob = case.problem
+    print(f"sources Voronoi ({len(prob.buckets)} buckets) {precision}", flush=True)
+    per_macro = [None] * prob.M
+    for bk in prob.buckets:
+        pts = torch.as_tensor(bk.qpts, dtype=torch.float64, device="cuda")
+        fq = serving.bump_sources(pts, [3])
+        with bk.batch(precision, max_columns=1) as b:
+            b.assemble(np.stack([case.kq[c] for c in bk.cells]), None, case.f_const)
+            b.factor()
+            b.condense()
+            loads = b.sources(fq).cpu().numpy()[0]
+        fqh = fq[0].cpu().numpy()
+        for r, c in enumerate(bk.cells):
+            per_macro[c] = (loads[r], fqh[r])
+    ref = ow.oracle_config(dataclasses.replace(case, fq=[f for _, f in per_macro], f_const=0.0))
+    for c in range(prob.M):
+        assert rel(per_macro[c][0][None], np.asarray(ref["loads"][c])[None]) < TOL, c
+
+
 def replay_alone(srv, cases, done, precision):
     """Each problem's requests one at a time on fresh batches, bit for bit."""
     for p, case in enumerate(cases):
