@@ -99,6 +99,13 @@ __device__ __forceinline__ void dd_add(double &hi, double &lo, double v) {
 // would turn the selected pointer into a flat address), and loads outside a
 // band or a matrix can read it instead of branching.
 __device__ double kZeroF64[1];
+__device__ float kZeroF32[1];
+template <typename S>
+__device__ __forceinline__ const S *zero_of();
+template <>
+__device__ __forceinline__ const double *zero_of<double>() { return kZeroF64; }
+template <>
+__device__ __forceinline__ const float *zero_of<float>() { return kZeroF32; }
 
 // A pointer known to address global memory (one read back from LDS would
 // otherwise be accessed as a flat address).
@@ -1025,7 +1032,7 @@ template <typename T>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
     k_factor_panel(int n, int b, int ldw, int ldq, T *__restrict__ band, T *__restrict__ rband,
                    int *__restrict__ status, double tiny, const T *__restrict__ sval, int nnz,
-                   const int *__restrict__ bslot, T *__restrict__ dblk) {
+                   const int *__restrict__ bslot, double *__restrict__ dblk) {
   constexpr int NB = FPANEL;
   constexpr int NPF = 10;  // NB * W <= 192 * NPF, so W <= 120
   typedef typename Mfma16<T>::v4 V4;
@@ -1276,7 +1283,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
         for (int e = u; e < NB * NB; e += 192)
           if ((e & 15) <= (e >> 4))
             dblk[((size_t)mac * ((n + NB - 1) / NB) + j0 / NB) * DBLK + tri16(e >> 4, e & 15)] =
-                Lp[(e & 15) * ldq + (e >> 4)];
+                (double)Lp[(e & 15) * ldq + (e >> 4)];
       // the entering columns take panel k's slots (read by nobody after panel
       // k was factored)
 #pragma unroll
@@ -1333,7 +1340,7 @@ static bool factor_wide_ok(int b, size_t tsize, int limit) {
 template <typename T>
 __global__ void __launch_bounds__(256)
     k_factor_wide(int n, int b, int ldq, T *__restrict__ band, T *__restrict__ rband, int *__restrict__ status,
-                  double tiny, T *__restrict__ dblk) {
+                  double tiny, double *__restrict__ dblk) {
   constexpr int NB = FPANEL, NR = WIDE_ROWS;
   extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
   const int W = b + 1, PR = NB + b, PR2 = 2 * NB + b;
@@ -1437,7 +1444,7 @@ __global__ void __launch_bounds__(256)
         const int np = p ? npB : npA;
         if (sig <= rho && (p == 0 || npB > 0))
           dblk[((size_t)mac * ((n + NB - 1) / NB) + j0 / NB + p) * DBLK + tri16(rho, sig)] =
-              rho < np ? LpT[(p * NB + sig) * ldq + p * NB + rho] : (rho == sig ? T(1) : T(0));
+              rho < np ? (double)LpT[(p * NB + sig) * ldq + p * NB + rho] : (rho == sig ? 1.0 : 0.0);
       }
     // (c) trailing update: columns qc in [2 NB, PR2), rows qr in [qc, PR2)
     // with j0 + qr < n; tiles numbered down the tile columns of the lower
@@ -1902,7 +1909,11 @@ __global__ void __launch_bounds__(64 * SOLVE_WAVES) __attribute__((amdgpu_waves_
 // operands of the next block are loaded during this block's products, the
 // right-hand side and the diagonal inverse at the start of the block and used
 // after them.  Each element of y is read and written by the same lane in both
-// sweeps.
+// sweeps.  The storage type S of the bands and of y is that of the factor:
+// binary64, or binary32 for the mixed path, whose values are widened to
+// binary64 where the products use them (the prefetched operands stay binary32
+// in registers, so converting does not wait for their loads); the arithmetic
+// and the diagonal-block inverses are binary64 either way.
 #define SOLVE_MFMA_WAVES 4
 
 
@@ -1944,9 +1955,9 @@ __global__ void __launch_bounds__(256) k_diag_inverse(long nblocks, double *__re
   }
 }
 
-template <int MAXK, int NCT, int DIR>
-__device__ __forceinline__ void mfma_sweep(int n, int b, int ldy, int ncols, const double *__restrict__ Lb,
-                                           const double *__restrict__ Di, double *__restrict__ Y, int lane,
+template <typename S, int MAXK, int NCT, int DIR>
+__device__ __forceinline__ void mfma_sweep(int n, int b, int ldy, int ncols, const S *__restrict__ Lb,
+                                           const double *__restrict__ Di, S *__restrict__ Y, int lane,
                                            int cbase = 0) {
   typedef Mfma16<double>::v4 V4;
   const int W = b + 1, nblk = (n + 15) / 16;
@@ -1954,16 +1965,16 @@ __device__ __forceinline__ void mfma_sweep(int n, int b, int ldy, int ncols, con
   // operand A of window slot t for block q: row a16 of the block against
   // window row g of slot t, zero outside the band and the matrix (loaded
   // values are not touched before the product, so the loads stay in flight)
-  auto loadA = [&](int q, int t) -> double {
+  auto loadA = [&](int q, int t) -> S {
     const int i0 = 16 * q;
     if (DIR > 0) {
       const int j = i0 - 4 * (MAXK - t) + g, rr = a16 + 4 * (MAXK - t) - g;
-      return *((j >= 0 && rr <= b && i0 + a16 < n) ? Lb + (size_t)j * W + rr : kZeroF64);
+      return *((j >= 0 && rr <= b && i0 + a16 < n) ? Lb + (size_t)j * W + rr : zero_of<S>());
     }
     const int i = i0 + 16 + 4 * (MAXK - 1 - t) + g, rr = 16 + 4 * (MAXK - 1 - t) + g - a16;
-    return *((i < n && rr <= b) ? Lb + (size_t)i * W + rr : kZeroF64);
+    return *((i < n && rr <= b) ? Lb + (size_t)i * W + rr : zero_of<S>());
   };
-  double A[MAXK];
+  S A[MAXK];
   double win[MAXK][NCT];
 #pragma unroll
   for (int t = 0; t < MAXK; ++t) {
@@ -1982,14 +1993,15 @@ __device__ __forceinline__ void mfma_sweep(int n, int b, int ldy, int ncols, con
     const int qn = more ? (DIR > 0 ? q + 1 : q - 1) : q;  // the last block reloads its own operands
     const int i0 = 16 * q;
     // right-hand side rows and the diagonal inverse of this block
-    double rhs[NCT][4], dv[4];
+    S rhs[NCT][4];
+    double dv[4];
 #pragma unroll
     for (int ct = 0; ct < NCT; ++ct) {
       const int col = cbase + 16 * ct + a16;
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int row = i0 + 4 * r + g;
-        rhs[ct][r] = *((row < n && col < ncols) ? Y + (size_t)row * ldy + col : kZeroF64);
+        rhs[ct][r] = *((row < n && col < ncols) ? Y + (size_t)row * ldy + col : zero_of<S>());
       }
     }
     const double *dq = Di + (size_t)q * DBLK;
@@ -2009,7 +2021,7 @@ __device__ __forceinline__ void mfma_sweep(int n, int b, int ldy, int ncols, con
 #pragma unroll
     for (int t = 0; t < MAXK; ++t) {
 #pragma unroll
-      for (int ct = 0; ct < NCT; ++ct) acc[ct] = Mfma16<double>::step(A[t], win[t][ct], acc[ct]);
+      for (int ct = 0; ct < NCT; ++ct) acc[ct] = Mfma16<double>::step((double)A[t], win[t][ct], acc[ct]);
       A[t] = loadA(qn, t);
     }
     // diagonal block
@@ -2018,7 +2030,7 @@ __device__ __forceinline__ void mfma_sweep(int n, int b, int ldy, int ncols, con
     for (int ct = 0; ct < NCT; ++ct) {
       x[ct] = V4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-      for (int k = 0; k < 4; ++k) x[ct] = Mfma16<double>::step(dv[k], rhs[ct][k] - acc[ct][k], x[ct]);
+      for (int k = 0; k < 4; ++k) x[ct] = Mfma16<double>::step(dv[k], (double)rhs[ct][k] - acc[ct][k], x[ct]);
     }
 #pragma unroll
     for (int ct = 0; ct < NCT; ++ct) {
@@ -2026,7 +2038,7 @@ __device__ __forceinline__ void mfma_sweep(int n, int b, int ldy, int ncols, con
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int row = i0 + 4 * r + g;
-        if (row < n && col < ncols) Y[(size_t)row * ldy + col] = x[ct][r];
+        if (row < n && col < ncols) Y[(size_t)row * ldy + col] = (S)x[ct][r];
       }
     }
     // window: drop the four oldest slots, the block enters as the newest
@@ -2043,10 +2055,10 @@ __device__ __forceinline__ void mfma_sweep(int n, int b, int ldy, int ncols, con
   }
 }
 
-template <int MAXK, int NCT>
+template <typename S, int MAXK, int NCT>
 __global__ void __launch_bounds__(64 * SOLVE_MFMA_WAVES) __attribute__((amdgpu_waves_per_eu(2)))
-    k_solve_mfma(int n, int b, int ldy, int ncols, int macros, const double *__restrict__ band,
-                 const double *__restrict__ rband, const double *__restrict__ dinv, double *__restrict__ y,
+    k_solve_mfma(int n, int b, int ldy, int ncols, int macros, const S *__restrict__ band,
+                 const S *__restrict__ rband, const double *__restrict__ dinv, S *__restrict__ y,
                  const int *__restrict__ colstate, int *__restrict__ iters) {
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int mac = __builtin_amdgcn_readfirstlane(blockIdx.x * SOLVE_MFMA_WAVES + wave);
@@ -2058,19 +2070,19 @@ __global__ void __launch_bounds__(64 * SOLVE_MFMA_WAVES) __attribute__((amdgpu_w
   if (iters && lane == 0) iters[mac] += 1;
   const size_t nW = (size_t)n * (b + 1);
   const double *Di = dinv + (size_t)mac * ((n + 15) / 16) * DBLK;
-  double *Y = y + (size_t)mac * n * ldy;
-  mfma_sweep<MAXK, NCT, 1>(n, b, ldy, ncols, band + mac * nW, Di, Y, lane);
-  mfma_sweep<MAXK, NCT, -1>(n, b, ldy, ncols, rband + mac * nW, Di, Y, lane);
+  S *Y = y + (size_t)mac * n * ldy;
+  mfma_sweep<S, MAXK, NCT, 1>(n, b, ldy, ncols, band + mac * nW, Di, Y, lane);
+  mfma_sweep<S, MAXK, NCT, -1>(n, b, ldy, ncols, rband + mac * nW, Di, Y, lane);
 }
 
 // The same solve with one wavefront per 16-column tile of a macro (a block per
 // macro): the tiles share only the band, so a macro's latency is that of one
 // tile's products (each wavefront reads the band itself: for refinement passes
 // that leave few macros pending).
-template <int MAXK>
+template <typename S, int MAXK>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
-    k_solve_mfma_tiles(int n, int b, int ldy, int ncols, const double *__restrict__ band,
-                       const double *__restrict__ rband, const double *__restrict__ dinv, double *__restrict__ y,
+    k_solve_mfma_tiles(int n, int b, int ldy, int ncols, const S *__restrict__ band,
+                       const S *__restrict__ rband, const double *__restrict__ dinv, S *__restrict__ y,
                        const int *__restrict__ colstate, int *__restrict__ iters) {
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int mac = blockIdx.x;
@@ -2081,19 +2093,19 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
   if (iters && threadIdx.x == 0) iters[mac] += 1;
   const size_t nW = (size_t)n * (b + 1);
   const double *Di = dinv + (size_t)mac * ((n + 15) / 16) * DBLK;
-  double *Y = y + (size_t)mac * n * ldy;
-  mfma_sweep<MAXK, 1, 1>(n, b, ldy, ncols, band + mac * nW, Di, Y, lane, 16 * wave);
-  mfma_sweep<MAXK, 1, -1>(n, b, ldy, ncols, rband + mac * nW, Di, Y, lane, 16 * wave);
+  S *Y = y + (size_t)mac * n * ldy;
+  mfma_sweep<S, MAXK, 1, 1>(n, b, ldy, ncols, band + mac * nW, Di, Y, lane, 16 * wave);
+  mfma_sweep<S, MAXK, 1, -1>(n, b, ldy, ncols, rband + mac * nW, Di, Y, lane, 16 * wave);
 }
 
 // The tile solve for bands past b = 111 (36 or 48 k-steps): its operand and
 // window registers alone take 144 or 192 of the 256 VGPRs that two waves per
 // SIMD leave, so it runs one wave per SIMD with the whole register file.
-template <int MAXK>
+template <typename S, int MAXK>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1)))
-    k_solve_mfma_tiles_wide(int n, int b, int ldy, int ncols, const double *__restrict__ band,
-                            const double *__restrict__ rband, const double *__restrict__ dinv,
-                            double *__restrict__ y, const int *__restrict__ colstate, int *__restrict__ iters) {
+    k_solve_mfma_tiles_wide(int n, int b, int ldy, int ncols, const S *__restrict__ band,
+                            const S *__restrict__ rband, const double *__restrict__ dinv,
+                            S *__restrict__ y, const int *__restrict__ colstate, int *__restrict__ iters) {
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int mac = blockIdx.x;
   if (colstate) {
@@ -2103,9 +2115,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1)))
   if (iters && threadIdx.x == 0) iters[mac] += 1;
   const size_t nW = (size_t)n * (b + 1);
   const double *Di = dinv + (size_t)mac * ((n + 15) / 16) * DBLK;
-  double *Y = y + (size_t)mac * n * ldy;
-  mfma_sweep<MAXK, 1, 1>(n, b, ldy, ncols, band + mac * nW, Di, Y, lane, 16 * wave);
-  mfma_sweep<MAXK, 1, -1>(n, b, ldy, ncols, rband + mac * nW, Di, Y, lane, 16 * wave);
+  S *Y = y + (size_t)mac * n * ldy;
+  mfma_sweep<S, MAXK, 1, 1>(n, b, ldy, ncols, band + mac * nW, Di, Y, lane, 16 * wave);
+  mfma_sweep<S, MAXK, 1, -1>(n, b, ldy, ncols, rband + mac * nW, Di, Y, lane, 16 * wave);
 }
 
 // Register window of the one-tile-per-wavefront matrix-core solve: the
@@ -3448,7 +3460,7 @@ static void launch_wave(const mhm_batch *b, int ncols, const int *colstate, int 
 // Each launcher fails with MHM_EINVAL when its family cannot serve the batch.
 template <typename T>
 static int launch_solve_mfma(const mhm_batch *b, int ncols, const int *colstate, int *iters, hipStream_t st) {
-  if constexpr (sizeof(T) == 8) {
+  {
     const LayoutK &L = b->layout->k;
     int maxk, nct;
     solve_mfma_shape(L.b, ncols, &maxk, &nct);
@@ -3456,9 +3468,9 @@ static int launch_solve_mfma(const mhm_batch *b, int ncols, const int *colstate,
       const dim3 grid((unsigned)((b->M + SOLVE_MFMA_WAVES - 1) / SOLVE_MFMA_WAVES));
 #define MHM_MFMA(K, C)                                                                                      \
   if (maxk == K && nct == C) {                                                                              \
-    hipLaunchKernelGGL((k_solve_mfma<K, C>), grid, dim3(64 * SOLVE_MFMA_WAVES), 0, st, L.n, L.b, b->ldy,     \
-                       ncols, b->M, (const double *)b->band, (const double *)b->rband,                     \
-                       (const double *)b->dinv, (double *)b->y, colstate, iters);                          \
+    hipLaunchKernelGGL((k_solve_mfma<T, K, C>), grid, dim3(64 * SOLVE_MFMA_WAVES), 0, st, L.n, L.b, b->ldy,  \
+                       ncols, b->M, (const T *)b->band, (const T *)b->rband, (const double *)b->dinv,      \
+                       (T *)b->y, colstate, iters);                                                         \
     LAUNCH_CHECK("k_solve_mfma");                                                                           \
     return MHM_OK;                                                                                          \
   }
@@ -3469,20 +3481,20 @@ static int launch_solve_mfma(const mhm_batch *b, int ncols, const int *colstate,
 #undef MHM_MFMA
     }
   }
-  return fail(MHM_EINVAL, "no matrix-core solve for this precision, band and column count");
+  return fail(MHM_EINVAL, "no matrix-core solve for this band and column count");
 }
 
 template <typename T>
 static int launch_solve_tiles(const mhm_batch *b, int ncols, const int *colstate, int *iters, hipStream_t st) {
-  if constexpr (sizeof(T) == 8) {
+  {
     const LayoutK &L = b->layout->k;
     const int kt = (b->dinv && b->rband && ncols >= 1 && ncols <= 64) ? solve_tile_window(L.b) : 0;
     const int nct = (ncols + 15) / 16;
 #define MHM_TILES(K)                                                                                         \
   if (kt == K) {                                                                                             \
-    hipLaunchKernelGGL((k_solve_mfma_tiles<K>), dim3(b->M), dim3(64 * nct), 0, st, L.n, L.b, b->ldy, ncols,   \
-                       (const double *)b->band, (const double *)b->rband, (const double *)b->dinv,         \
-                       (double *)b->y, colstate, iters);                                                   \
+    hipLaunchKernelGGL((k_solve_mfma_tiles<T, K>), dim3(b->M), dim3(64 * nct), 0, st, L.n, L.b, b->ldy, ncols, \
+                       (const T *)b->band, (const T *)b->rband, (const double *)b->dinv, (T *)b->y, colstate, \
+                       iters);                                                                               \
     LAUNCH_CHECK("k_solve_mfma_tiles");                                                                      \
     return MHM_OK;                                                                                           \
   }
@@ -3490,16 +3502,16 @@ static int launch_solve_tiles(const mhm_batch *b, int ncols, const int *colstate
 #undef MHM_TILES
 #define MHM_TILES_WIDE(K)                                                                                     \
   if (kt == K) {                                                                                              \
-    hipLaunchKernelGGL((k_solve_mfma_tiles_wide<K>), dim3(b->M), dim3(64 * nct), 0, st, L.n, L.b, b->ldy,      \
-                       ncols, (const double *)b->band, (const double *)b->rband, (const double *)b->dinv,   \
-                       (double *)b->y, colstate, iters);                                                    \
+    hipLaunchKernelGGL((k_solve_mfma_tiles_wide<T, K>), dim3(b->M), dim3(64 * nct), 0, st, L.n, L.b, b->ldy,   \
+                       ncols, (const T *)b->band, (const T *)b->rband, (const double *)b->dinv, (T *)b->y,  \
+                       colstate, iters);                                                                    \
     LAUNCH_CHECK("k_solve_mfma_tiles_wide");                                                                  \
     return MHM_OK;                                                                                            \
   }
     MHM_TILES_WIDE(36) MHM_TILES_WIDE(48)
 #undef MHM_TILES_WIDE
   }
-  return fail(MHM_EINVAL, "no matrix-core tile solve for this precision, band and column count");
+  return fail(MHM_EINVAL, "no matrix-core tile solve for this band and column count");
 }
 
 template <typename T>
@@ -3584,7 +3596,7 @@ static int launch_solve(const mhm_batch *b, int ncols, const int *colstate, int 
     default:
       break;
   }
-  if constexpr (sizeof(T) == 8) {
+  {
     int maxk, nct;
     solve_mfma_shape(L.b, ncols, &maxk, &nct);
     if (b->dinv && b->rband && maxk)
@@ -3894,11 +3906,11 @@ int mhm_batch_create(const mhm_layout *L, int32_t macros, int32_t precision, int
     unsigned char *p = nullptr;
     if ((rc = dev_alloc(b->owned, &p, M * n * lk.W * tsize))) return cleanup(rc);
     b->band = p;
-    // the binary64 matrix-core solve needs the row band and the diagonal
-    // blocks, which the LDS-window and the wide-band factors write
+    // the matrix-core solve needs the row band and the diagonal blocks (as
+    // binary64, whatever the factor's type), which the LDS-window and the
+    // wide-band factors write
     const bool panel = factor_panel_ok(lk.b, tsize, dev);
-    const bool mfma = precision == MHM_FP64 && solve_tile_window(lk.b) > 0 &&
-                      (panel || factor_wide_ok(lk.b, tsize, limit));
+    const bool mfma = solve_tile_window(lk.b) > 0 && (panel || factor_wide_ok(lk.b, tsize, limit));
     if (wave_width(lk.b, widest) > 0 || wave_width(lk.b, 2) > 0 || mfma) {
       if ((rc = dev_alloc(b->owned, &p, M * n * lk.W * tsize))) return cleanup(rc);
       b->rband = p;
@@ -4207,13 +4219,13 @@ static int run_factor(mhm_batch *b, hipStream_t st) {
       HIP_TRY(set_max_lds((const void *)k_factor_panel<T>, pshm));
     hipLaunchKernelGGL((k_factor_panel<T>), dim3(b->M), dim3(256), pshm, st, L.n, L.b, panel_ldw(L.b, sizeof(T)),
                        panel_ldq(L.b), (T *)b->band, (T *)b->rband, b->bk.status, tiny,
-                       (const T *)b->sval, L.nnz, L.bslot, sizeof(T) == 8 ? (T *)b->dinv : (T *)nullptr);
+                       (const T *)b->sval, L.nnz, L.bslot, (double *)b->dinv);
   } else if (factor_wide_ok(L.b, sizeof(T), lds_limit(lay->device))) {
     const size_t wshm = factor_wide_lds(L.b, sizeof(T));
     if (wshm > 65536)
       HIP_TRY(set_max_lds((const void *)k_factor_wide<T>, wshm));
     hipLaunchKernelGGL((k_factor_wide<T>), dim3(b->M), dim3(256), wshm, st, L.n, L.b, wide_ldq(L.b),
-                       (T *)b->band, (T *)b->rband, b->bk.status, tiny, sizeof(T) == 8 ? (T *)b->dinv : (T *)nullptr);
+                       (T *)b->band, (T *)b->rband, b->bk.status, tiny, (double *)b->dinv);
   } else if (L.b >= 12) {
     if (shm > 65536)
       HIP_TRY(set_max_lds((const void *)k_factor<T, 256>, shm));
@@ -4225,7 +4237,7 @@ static int run_factor(mhm_batch *b, hipStream_t st) {
   }
   LAUNCH_CHECK("k_factor");
   }
-  if (sizeof(T) == 8 && b->dinv) {
+  if (b->dinv) {
     const size_t blocks = (size_t)b->M * ((L.n + 15) / 16);
     hipLaunchKernelGGL(k_diag_inverse, dim3((unsigned)((blocks + 15) / 16)), dim3(256), 0, st, (long)blocks,
                        b->dinv);
@@ -4315,6 +4327,13 @@ static int run_solve(mhm_batch *b, const double *rhs, double *X, int ncols, int 
   // contracts the error by about 1e-6 on these operators, so the solution
   // and two corrections meet the 1e-10 criterion)
   const int settled = sizeof(T) == 8 ? 1 : 3;
+  // With binary32 factors a refinement pass accepts a column at a tenth of
+  // rtol: a column that meets rtol after the second solve can be as far from
+  // the exact solution as its residual (1e-10), and the responses are meant to
+  // be within 1e-10 of the binary64 solution; the tenth keeps them an order
+  // below.  A macro's slowest column needs the third solve anyway, so this
+  // adds almost no pass.  The final check uses rtol itself.
+  const double rtol_pass = sizeof(T) == 8 ? rtol : 0.1 * rtol;
   for (int it = 0; it <= max_refine; ++it) {
     {
       TimerSpan span(MHM_TIMER_RESIDUAL, st);
@@ -4326,7 +4345,7 @@ static int run_solve(mhm_batch *b, const double *rhs, double *X, int ncols, int 
         hipLaunchKernelGGL((k_rhs_init<T>), dim3(b->M), dim3(SADDLE_THREADS), shm, st, L, b->bk, ncols, rhs, (T *)b->y,
                            b->beta, b->colstate, b->layout->coupling_rm, b->bnorm, uw_inline ? ncols : -1, gch);
         LAUNCH_CHECK("k_rhs_init");
-      } else if (int rc = launch_residual<T>(b, ncols, rhs, X, rtol, it, 0, st)) {
+      } else if (int rc = launch_residual<T>(b, ncols, rhs, X, rtol_pass, it, 0, st)) {
         return rc;
       }
     }

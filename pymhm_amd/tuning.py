@@ -36,11 +36,11 @@ FAMILIES = {
     _lib.SOLVE_WAVE: "wave",
     _lib.SOLVE_BLOCKED: "blocked",
 }
-# binary64 solves run on the matrix cores or blocked; binary32 ones lane per
-# column or blocked
+# both precisions solve on the matrix cores (binary64 arithmetic on the
+# factor's storage type) or blocked; binary32 factors also lane per column
 CANDIDATES = {
     "fp64": (_lib.SOLVE_AUTO, _lib.SOLVE_MFMA, _lib.SOLVE_MFMA_TILES, _lib.SOLVE_BLOCKED),
-    "mixed": (_lib.SOLVE_AUTO, _lib.SOLVE_WAVE, _lib.SOLVE_BLOCKED),
+    "mixed": (_lib.SOLVE_AUTO, _lib.SOLVE_MFMA, _lib.SOLVE_MFMA_TILES, _lib.SOLVE_WAVE, _lib.SOLVE_BLOCKED),
 }
 _CACHE: dict[str, list[int]] = {}
 
