@@ -54,7 +54,7 @@ METRIC = "macro solves/s at <=1e-10 rel err vs FP64 oracle (vs plain FP64); p50/
 # rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE and scaled by the counters' reading of
 # a 1 GiB device copy (profiles/traffic_fp64_config2.md); quoted as
 # roofline.traffic when the kernel is the dominant one
-MEASURED_TRAFFIC_PER_MACRO = {"k_factor_panel<double>": 141310 + 1426203, "k_solve_mfma<17, 3>": 2561256 + 632107}
+MEASURED_TRAFFIC_PER_MACRO = {"k_factor_panel<double>": 141310 + 1426203, "k_solve_mfma<double, 17, 3>": 2561256 + 632107}
 HBM_PEAK_GBS = 8000.0
 PEAK_TFLOPS = {"f64": 78.6, "f32": 157.3}
 
@@ -67,10 +67,6 @@ WORKLOADS = {
         f"{p.nx}^3 cube macros x (Freudenthal {p.refinement}^3x6 P2 tetrahedra, n={p.n}), "
         f"m={p.m} P0 multipliers (2x2 cells per face), K=1e-3 in r=0.02 spheres at 20% volume, f=1")),
 }
-WAVE_WINDOWS = (8, 16, 24, 32, 48, 64, 68, 80, 96)
-BLOCK_COLS = (2, 4, 7, 8, 12, 16)  # columns per wavefront of the block solve
-
-
 def factor_kernel(b, tname):
     """The factor kernel the library picks for half-bandwidth b: the LDS window
     while 16 + b <= 127, the in-place wide-band kernel beyond."""
@@ -78,24 +74,26 @@ def factor_kernel(b, tname):
 
 
 MFMA_MAXK = (8, 12, 17, 24, 28)
+TILE_WINDOWS = (8, 12, 17, 24, 28, 36, 48)
 
 
 def solve_kernel(b, ncols, tname):
-    """The solve kernel for half-bandwidth b and ncols columns: in binary64
-    with the LDS-window factor, the matrix-core solve with the smallest
-    register window of ceil(b / 4) k-steps x ceil(ncols / 16) column tiles
-    (at most 52 values per lane); otherwise the wavefront solve with the
-    smallest register window >= b + 1, else the block solve with its columns
-    per wavefront (the library's tables)."""
-    if tname == "double" and 16 + b <= 127 and 1 <= ncols <= 64:
-        nct, nk = -(-ncols // 16), -(-b // 4)
+    """The first-pass solve kernel the library picks for half-bandwidth b and
+    ncols columns (both precisions solve on the matrix cores up to b = 191):
+    one wavefront per macro with the smallest register window of ceil(b / 4)
+    k-steps x ceil(ncols / 16) column tiles (at most 52 values per lane), else
+    one wavefront per 16-column tile with the smallest window of at least
+    ceil(b / 4) k-steps (the wide instances past 28)."""
+    nk = -(-b // 4)
+    if 1 <= ncols <= 64:
+        nct = -(-ncols // 16)
         k = next((k for k in MFMA_MAXK if nk <= k and k * nct <= 52), None)
         if k is not None:
-            return f"k_solve_mfma<{k}, {nct}>"
-    w = next((w for w in WAVE_WINDOWS if b + 1 <= w), None)
-    if w is not None and ncols <= 64:
-        return f"k_solve_wave<{tname}, {w}>"
-    return f"k_solve_blk<{tname}, {next(c for c in BLOCK_COLS if ncols <= 4 * c)}>"
+            return f"k_solve_mfma<{tname}, {k}, {nct}>"
+    k = next((k for k in TILE_WINDOWS if nk <= k), None)
+    if k is not None:
+        return f"k_solve_mfma_tiles{'_wide' if k > 28 else ''}<{tname}, {k}>"
+    return f"k_solve_blk<{tname}, ...> (b > 191)"
 
 
 _LIMITS = None
@@ -112,6 +110,7 @@ def _cpu_worker_init():
 
 
 CPU_REPEATS = 3
+CONFIG4_SERIAL_STEPS = 3  # steps of the config-4 roofline leg (buckets run alone)
 
 
 def cpu_cores():
@@ -181,6 +180,52 @@ def per_macro_traffic(prob, precision):
         factor_bytes=lay.nnz * ts + 2 * n * W * ts + blocks, factor_flops=n * b * (b + 1),
         solve_bytes=2 * n * W * ts + 2 * blocks + 4 * n * (R + 2) * ts, solve_flops=4 * n * b * R,
     )
+
+
+def aggregate_roofline(path, head, how):
+    """Roofline of the dominant kernel family (factor or solve) over launches of
+    several shapes run alone: ``head["serial_kernel_ms_per_step"]`` and
+    ``head["serial_launches_per_step"]`` are their event-timed milliseconds and
+    launch counts per step, ``head["work_per_step"]`` their algorithmic work
+    (factor flops and bytes, solve bytes, macro solves, macros).  The factor is priced
+    in flops against the matrix-core peak of its precision, the solve in
+    algorithmic bytes against HBM; achieved = the work of all launches of a
+    step / their summed durations."""
+    sms, work = head["serial_kernel_ms_per_step"], head["work_per_step"]
+    dominant = max(("factor", "solve"), key=lambda k: sms[k])
+    launches = head["serial_launches_per_step"][dominant]
+    secs = sms[dominant] / 1e3
+    tname = "double" if path == "fp64" else "float"
+    if dominant == "factor":
+        achieved, bound, peak, unit = (work["factor_flops"] / secs / 1e12, "mfma",
+                                       PEAK_TFLOPS["f64" if path == "fp64" else "f32"], "TFLOP/s")
+        nbytes = work["factor_bytes"]
+        kname = f"k_factor_panel<{tname}> (b <= 111) and k_factor_wide<{tname}>, every shape"
+        units = work["macros"] / launches
+    else:
+        nbytes = work["solve_bytes"]
+        achieved, bound, peak, unit = nbytes / secs / 1e9, "hbm", HBM_PEAK_GBS, "GB/s"
+        kname = "band solve kernels (chosen families), every shape"
+        units = work["macro_solves"] / launches
+    return {
+        "kernel": kname, "bound": bound, "achieved": achieved, "peak": peak, "unit": unit,
+        "frac": achieved / peak, "traffic": None,
+        "algorithmic_bytes": nbytes / launches, "hbm_frac_of_algorithmic_bytes": nbytes / secs / 1e9 / HBM_PEAK_GBS,
+        "units_per_launch": units, "avg_launch_ms": sms[dominant] / launches,
+        "measured": f"{how}; achieved = algorithmic work of all {launches:.0f} launches of a step / their summed "
+                    "event-timed durations",
+    }
+
+
+def add_work(work, prob, precision, macros, solves):
+    """Accumulate the algorithmic work of one batch (``macros`` factored,
+    ``solves`` macro solves) of ``prob``'s layout into ``work``."""
+    t = per_macro_traffic(prob, precision)
+    work["factor_flops"] += macros * t["factor_flops"]
+    work["factor_bytes"] += macros * t["factor_bytes"]
+    work["solve_bytes"] += solves * t["solve_bytes"]
+    work["macro_solves"] += solves
+    work["macros"] += macros
 
 
 def run_gpu(case, precision, steps, warmup, plan):
@@ -256,7 +301,7 @@ def run_config5(args):
     stream (30% new coefficients, 70% new sources, problems uniform)."""
     import torch
 
-    from pymhm_amd import serving
+    from pymhm_amd import _lib, serving
     from pymhm_amd import workloads as pw
 
     t0 = time.perf_counter()
@@ -299,17 +344,37 @@ def run_config5(args):
                                                        fq=fq[0].cpu().numpy()))
             got = srv.batches[0].sources(fq)[0].cpu().numpy()
             err = rel_err(got, ref["loads"])
+            # roofline leg, outside the timed runs: the coefficient request of
+            # every problem (the stage rerun, the served mix's heaviest work)
+            # alone, with its current coefficients, so its factors stay those
+            # the requests were served with
+            torch.cuda.synchronize()
+            _lib.call("mhm_timing_reset")
+            _lib.call("mhm_timing_enable", 1)
+            for p in range(len(cases)):
+                srv._stage(p, srv.current_kq[p])
+                torch.cuda.synchronize()
+            _lib.call("mhm_timing_enable", 0)
+            serial = _lib.kernel_times()
+            work = dict(factor_flops=0.0, factor_bytes=0.0, solve_bytes=0.0, macro_solves=0.0, macros=0)
+            for case, b in zip(cases, srv.batches):
+                add_work(work, case.problem, precision, b.macros, float(b.iterations().sum()))
         finally:
             srv.close()
             torch.cuda.synchronize()
         res[precision] = dict(capacity=capacity, offered_rate=rate, served_rate=st["requests_per_s"],
                               p50_ms=st["p50_ms"], p99_ms=st["p99_ms"], mean_ms=st["mean_ms"],
-                              max_rel_err_vs_oracle=err, failed_requests=failed, memory=memory)
+                              max_rel_err_vs_oracle=err, failed_requests=failed, memory=memory,
+                              serial_kernel_ms_per_step={k: v[0] for k, v in serial.items()},
+                              serial_launches_per_step={k: v[1] for k, v in serial.items()},
+                              work_per_step=work)
         print(f"[bench] {precision}: capacity {capacity:.1f} req/s, at {rate:.1f} req/s p50 {st['p50_ms']:.1f} ms "
               f"p99 {st['p99_ms']:.1f} ms, err {err:.2e}", file=sys.stderr, flush=True)
     ok = [p for p in res if res[p]["max_rel_err_vs_oracle"] <= 1e-10]
     path = max(ok or list(res), key=lambda p: res[p]["capacity"])
     head = res[path]
+    roofline = aggregate_roofline(path, head, "the coefficient request (stage rerun) of each of the three problems "
+                                              "alone, after the served runs")
     out = {
         "metric": "served requests/s (resident factors, configs 1-3, 30% new coefficient / 70% new source); "
                   "p50/p99 latency at half the fp64 capacity",
@@ -325,7 +390,7 @@ def run_config5(args):
         "p50_ms": head["p50_ms"], "p99_ms": head["p99_ms"],
         "paths": res,
         "mixed_vs_fp64": (res["mixed"]["capacity"] / res["fp64"]["capacity"]) if len(res) == 2 else None,
-        "roofline": None,
+        "roofline": roofline,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
@@ -422,17 +487,39 @@ def run_config4(args):
                         e = max(e, rel_err(x[r:r + 1], oref[c][0][None]), rel_err(y[r:r + 1], oref[c][1][None]))
             err = e
         iters = np.concatenate([b.iterations() for b in batches])
+        # roofline leg, outside the timed region: the buckets one after another
+        # on one stream, so that each kernel's event-timed duration is its own
+        # (the overlapped step's per-kernel spans include the other streams)
+        stage.streams = stage.streams[:1]
+        step()
+        torch.cuda.synchronize()
+        _lib.call("mhm_timing_reset")
+        _lib.call("mhm_timing_enable", 1)
+        for _ in range(CONFIG4_SERIAL_STEPS):
+            step()
+        torch.cuda.synchronize()
+        _lib.call("mhm_timing_enable", 0)
+        serial = _lib.kernel_times()
+        work = dict(factor_flops=0.0, factor_bytes=0.0, solve_bytes=0.0, macro_solves=0.0, macros=0)
+        for bk, b in zip(prob.buckets, batches):
+            add_work(work, bk, precision, len(bk.cells), float(b.iterations().sum()))
         stage.close()
         res[precision] = dict(value=M * args.steps / elapsed, p50_ms=float(np.percentile(lat, 50) * 1e3),
                               p99_ms=float(np.percentile(lat, 99) * 1e3),
                               refinement_solves_per_macro=float(iters.mean()),
                               kernel_ms_per_step={k: v[0] / args.steps for k, v in times.items()},
-                              max_rel_err_vs_oracle=err, solve_kernels_tuned=tuned)
+                              max_rel_err_vs_oracle=err, solve_kernels_tuned=tuned,
+                              serial_kernel_ms_per_step={k: v[0] / CONFIG4_SERIAL_STEPS for k, v in serial.items()},
+                              serial_launches_per_step={k: v[1] / CONFIG4_SERIAL_STEPS for k, v in serial.items()},
+                              work_per_step=work)
         print(f"[bench] {precision}: {res[precision]['value']:.0f} macros/s, p50 {res[precision]['p50_ms']:.2f} ms, "
               f"err {err}", file=sys.stderr, flush=True)
     ok = [p for p in res if res[p]["max_rel_err_vs_oracle"] is not None and res[p]["max_rel_err_vs_oracle"] <= 1e-10]
     path = max(ok, key=lambda p: res[p]["value"]) if ok else paths[0]
     head = res[path]
+    roofline = aggregate_roofline(
+        path, head, f"buckets one after another on one stream, {CONFIG4_SERIAL_STEPS} steps outside the timed "
+                    "region")
     out = {
         "metric": METRIC, "value": head["value"], "unit": "macro solves/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": M / head["value"] * 1e3, "higher_is_better": True, "scaling": "weak",
@@ -443,7 +530,7 @@ def run_config4(args):
                    "macros": M, "precision": path},
         "p50_ms": head["p50_ms"], "p99_ms": head["p99_ms"], "paths": res,
         "mixed_vs_fp64": res["mixed"]["value"] / res["fp64"]["value"] if len(res) == 2 else None,
-        "roofline": None,
+        "roofline": roofline,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
