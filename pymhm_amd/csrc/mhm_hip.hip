@@ -953,38 +953,39 @@ __global__ void __launch_bounds__(NT)
 }
 
 // ----------------------------------------------------------------- blocked band Cholesky
-// Block per macro, panels of FPANEL columns.  The LDS window holds the
-// S = max(NB + b, 2 NB) active band columns (slot c mod S at win[slot * ldw],
-// entries r = 0..b of column c at rows c + r; at least two panels, so the next
-// panel is always in the window); the factored panels live in two
-// transposed LDS buffers, LpT[t * ldq + q] = L(j0 + q, j0 + t) (rows PR ..
-// ldq - 1 zero), used alternately.
+// Block per macro, panels of FPANEL columns, factored in place in an LDS ring
+// of S2 = S + NB column slots, S = max(NB + b, 2 NB) (column c at slot c mod
+// S2, entries r = 0..b of column c at win[slot * ldw + r], rows c + r).  At
+// panel step k (columns j0 .. j0 + NB - 1) the ring holds panel k, the S - NB
+// columns after it (so the next panel is always inside) and, in the NB slots
+// of panel k - 1 (written out in step k - 1), the columns entering after
+// panel k.
 // Panel factorisation (one wavefront, in registers): lane q holds panel rows q
 // and q + 64 (PR = NB + b <= 127 rows); left-looking, column t takes the
 // updates of the finished columns k < t with their multipliers L(j0 + t,
-// j0 + k) read back from the panel buffer (LDS broadcast reads, ordered by a
-// wavefront fence), then is pivoted and scaled; the pivot comes from its lane
-// by v_readlane, 1/sqrt by the hardware estimate and Newton steps.  The step
-// is branch-free:
-// loads at clamped addresses with selects, a failed pivot flagged and replaced
-// by 1, the columns of a short last panel padded with identity columns (which
-// leave the others unchanged).
+// j0 + k) read back from the ring (LDS broadcast reads, in order within the
+// wavefront), then is pivoted and scaled and written back to its slot; the
+// pivot comes from its lane by v_readlane, 1/sqrt by the hardware estimate and
+// Newton steps.  The step is branch-free: loads at clamped addresses with
+// selects, a failed pivot flagged and replaced by 1, the columns of a short
+// last panel padded with identity columns (which leave the others unchanged).
 // One-panel look-ahead, one block barrier per panel.  While panel k is the
 // source of the rank-NB update (C = C - L_I L_J^T over 16 x 16 tiles of the
 // lower triangle, four 16x16x4 MFMA steps per tile, the C fragment read from
-// and written back to the window; ldw makes those accesses conflict-free in
-// binary64, ldq the operand reads):
+// and written back to the ring, the operands read from panel k's slots and
+// masked to the band; ldw makes the C fragment accesses conflict-free in
+// binary64):
 //  - one wavefront (k mod 4, so the serial steps of the blocks sharing a CU
 //    spread over its SIMDs) applies the tiles of the first tile column - the
-//    columns of panel k + 1 - and then factors panel k + 1 into the other
-//    buffer, so the serial panel step of k + 1 overlaps the rest of k's update;
+//    columns of panel k + 1 - and then factors panel k + 1 in its slots, so
+//    the serial panel step of k + 1 overlaps the rest of k's update (nobody
+//    else reads or writes panel k + 1's columns during step k);
 //  - the other three gather the NB columns entering after panel k (issued
 //    first, consumed last), apply the other tile columns, write panel k to
 //    HBM (the column band, contiguous; the row band, rband[R*W + r] =
 //    L(R, R-r), runs of NB per row; with dblk set its 16 x 16 diagonal block,
-//    packed (tri16), to dblk[(mac * nblocks + j0 / 16) * DBLK]) and move the
-//    entering columns into panel k's window slots, which nobody reads after
-//    panel k was factored.
+//    packed (tri16), to dblk[(mac * nblocks + j0 / 16) * DBLK]) and store the
+//    entering columns in panel k - 1's slots.
 // The tiles of panel k + 1's columns are updated by every earlier panel before
 // it is factored: panel k's by the serial wavefront in this step, earlier
 // panels' by the other wavefronts in earlier steps.  Barriers order LDS only: no thread reads
@@ -1016,7 +1017,7 @@ struct Mfma16<float> {  // v_mfma_f32_16x16x4f32: D[4(lane/16) + r][lane%16] in 
 };
 // (A operand: lane l holds A[l%16][l/16]; B operand: lane l holds B[l/16][l%16].)
 
-// Window column stride: in binary64 the C fragments of a 16 x 16 tile hit 64
+// Ring column stride: in binary64 the C fragments of a 16 x 16 tile hit 64
 // distinct banks when ldw - 1 = 2 mod 4; in binary32 an odd ldw - 1 = 3 mod 4
 // leaves at most two-way conflicts.
 static int panel_ldw(int b, size_t tsize) {
@@ -1026,19 +1027,10 @@ static int panel_ldw(int b, size_t tsize) {
   return w;
 }
 
-// LpT row stride: >= NB + 16 * ceil(b / 16) rows (the tiles; the panel rows
-// PR = NB + b are fewer), = 16 mod 32 so the four 16-lane groups of an operand
-// read use disjoint banks.
-static int panel_ldq(int b) {
-  int q = FPANEL + 16 * ((b + 15) / 16);
-  while ((q & 31) != 16) ++q;
-  return q;
-}
-
-// The window and the two panel buffers.
+// The ring: S + NB column slots, then a zero element (and one of padding).
 static size_t factor_panel_lds(int b, size_t tsize) {
   const size_t S = std::max((size_t)FPANEL + b, (size_t)2 * FPANEL);
-  return (S * panel_ldw(b, tsize) + 2 * (size_t)FPANEL * panel_ldq(b)) * tsize;
+  return ((S + FPANEL) * panel_ldw(b, tsize) + 2) * tsize;
 }
 
 static int lds_limit(int device);
@@ -1046,21 +1038,32 @@ static bool factor_panel_ok(int b, size_t tsize, int device) {
   return FPANEL + b <= 127 && factor_panel_lds(b, tsize) <= (size_t)lds_limit(device);
 }
 
-template <typename T>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
-    k_factor_panel(int n, int b, int ldw, int ldq, T *__restrict__ band, T *__restrict__ rband,
+// Blocks per CU the kernel's registers allow: in binary64 three while W <= 72
+// (52.5 KiB of ring at b = 66 lets three share a CU's LDS; wider rings fit
+// two), four in binary32.
+template <typename T, int NPF>
+struct FactorPanelWaves {
+  static constexpr int value = sizeof(T) == 8 ? (NPF <= 6 ? 3 : 2) : 5;
+};
+
+// NPF: entering elements per thread of the three gathering wavefronts, NB W
+// <= 192 NPF (6: W <= 72; 10: W <= 120).  NARROW: b < NB, where a panel
+// column's band ends inside the panel (the in-panel multipliers past it are
+// zero and read as such).
+template <typename T, int NPF, bool NARROW>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FactorPanelWaves<T, NPF>::value)))
+    k_factor_panel(int n, int b, int ldw, T *__restrict__ band, T *__restrict__ rband,
                    int *__restrict__ status, double tiny, const T *__restrict__ sval, int nnz,
                    const int *__restrict__ bslot, double *__restrict__ dblk) {
   constexpr int NB = FPANEL;
-  constexpr int NPF = 10;  // NB * W <= 192 * NPF, so W <= 120
   typedef typename Mfma16<T>::v4 V4;
   extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
   const int W = b + 1;
   const int S = max(NB + b, 2 * NB);  // window columns: the next panel is always inside
+  const int S2 = S + NB;              // ring slots: the window and the entering columns
   const int PR = NB + b;              // panel rows
-  T *win = reinterpret_cast<T *>(smem_raw);  // S x ldw
-  T *LpT0 = win + (size_t)S * ldw;           // NB x ldq, panels 0, 2, 4, ...
-  T *LpT1 = LpT0 + (size_t)NB * ldq;         // panels 1, 3, 5, ...
+  T *win = reinterpret_cast<T *>(smem_raw);  // S2 x ldw, then the zero element
+  const int zoff = S2 * ldw;                 // win[zoff] = 0: out-of-band operands read it
   __shared__ int bad;
   const int mac = blockIdx.x, tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   T *Lg = band + (size_t)mac * n * W;
@@ -1070,8 +1073,16 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
   // past row n - 1) reads as 0
   const T *sv = sval + (size_t)mac * nnz;
   auto slot_of = [&](int j, int r) -> int { return (j < n && j + r < n) ? bslot[(size_t)j * W + r] : -1; };
+  // ring slot of the column x columns after the one in slot s (s, x < S2)
+  auto ring = [&](int s, int x) -> int {
+    const int v = s + x;
+    return v >= S2 ? v - S2 : v;
+  };
   const FastDiv divW(W);
-  if (tid == 0) bad = 0;
+  if (tid == 0) {
+    bad = 0;
+    win[zoff] = T(0);
+  }
   // columns 0 .. S-1 in slots 0 .. S-1, eight elements per thread at a time:
   // all slot indices, then all values, then the stores (a loop of dependent
   // pairs would wait for two round trips per element)
@@ -1093,7 +1104,6 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
       if (e < S * W) win[c * ldw + r] = qv[v] >= 0 ? x[v] : T(0);
     }
   }
-  for (int e = tid; e < 2 * NB * ldq; e += 256) LpT0[e] = T(0);
   if (Rg)  // entries L(j, j-r) with j - r < 0
     for (int e = tid; e < min(b, n) * W; e += 256) {
       const int j = divW(e), r = e - j * W;
@@ -1101,24 +1111,21 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
     }
   const int nt = (b + 15) / 16;  // tile rows / columns of the trailing block
   const int ntile = nt * (nt + 1) / 2;
-  const int j16 = lane & 15, g16 = lane >> 4;
 
-  // Factor the panel of columns jp .. jp + NB - 1 (one wavefront) into Lp,
-  // left-looking: column t takes the updates of the finished columns k < t -
-  // the multipliers L(jp + t, jp + k), k < t - 1, read back from Lp one step
-  // ahead (LDS broadcast reads off the critical path), L(jp + t, jp + t - 1)
-  // from its lane by v_readlane - and is then pivoted and scaled; the pivot
-  // also comes by v_readlane.  A wavefront's LDS accesses execute in order, so
-  // a row stored in one step is read back two steps later without a fence.
-  auto factor_panel = [&](int jp, T *Lp) {
-    const int sp = jp % S;
+  // Factor the panel of columns jp .. jp + NB - 1 (one wavefront, slot sp of
+  // column jp) in place, left-looking: column t takes the updates of the
+  // finished columns k < t - the multipliers L(jp + t, jp + k), k < t - 1,
+  // read back from the ring one step ahead (LDS broadcast reads off the
+  // critical path), L(jp + t, jp + t - 1) from its lane by v_readlane - and is
+  // then pivoted, scaled and stored back; the pivot also comes by v_readlane.
+  // A wavefront's LDS accesses execute in order, so an entry stored in one
+  // step is read back two steps later without a fence.
+  auto factor_panel = [&](int jp, int sp) {
     const int np = min(NB, n - jp);
     T a0[NB], a1[NB];
 #pragma unroll
     for (int t = 0; t < NB; ++t) {
-      int slot = sp + t;
-      slot = slot >= S ? slot - S : slot;
-      const T *col = win + slot * ldw;
+      const T *col = win + ring(sp, t) * ldw;
       // a laundered lane index: the masks are formed here, not hoisted out of
       // the panel loop as SGPR pairs
       int qa = lane;
@@ -1131,14 +1138,27 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
       a1[t] = !pad && qb < PR && qb - t <= b ? v1 : T(0);
     }
     bool failed = false;
-    T m[NB];   // m[k] = L(jp + t, jp + k), k < t - 1, for the current column t
-    T mn[NB];  // the same for column t + 1, read during step t
+    // multiplier L(jp + t + 1, jp + k), k < t: an entry stored in an earlier
+    // step (rows past the band of column k are zero)
+    auto mult = [&](int t, int k) -> T {
+      if (NARROW) {
+        const T v = win[ring(sp, k) * ldw + min(t + 1 - k, b)];
+        return t + 1 - k <= b ? v : T(0);
+      }
+      return win[ring(sp, k) * ldw + t + 1 - k];
+    };
+    // Binary32 has the registers to read the next column's multipliers at the
+    // start of a step (into mn, then moved to m); binary64 at three blocks per
+    // CU reads them into m once the step's sums have used it.
+    constexpr bool PRE = sizeof(T) == 4;
+    T m[NB];  // m[k] = L(jp + t, jp + k), k < t - 1, for the current column t
+    T mn[PRE ? NB : 1];
 #pragma unroll
     for (int t = 0; t < NB; ++t) {
-      // the next column's older multipliers (rows stored two or more steps
-      // back), issued first so the reads overlap this step's pivot chain
+      if constexpr (PRE) {
 #pragma unroll
-      for (int k = 0; k < t && t + 1 < NB; ++k) mn[k] = Lp[k * ldq + t + 1];
+        for (int k = 0; k < t && t + 1 < NB; ++k) mn[k] = mult(t, k);
+      }
       // two partial sums halve the dependent chain of the updates
       T s0a = T(0), s0b = T(0), s1a = T(0), s1b = T(0);
 #pragma unroll
@@ -1153,6 +1173,12 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
       }
       a0[t] -= s0a + s0b;
       a1[t] -= s1a + s1b;
+      // the next column's multipliers k < t, read into m now that the sums
+      // have used it, so the reads overlap this step's pivot chain
+      if constexpr (!PRE) {
+#pragma unroll
+        for (int k = 0; k < t && t + 1 < NB; ++k) m[k] = mult(t, k);
+      }
       if (t > 0) {
         const T l = lane_bcast(a0[t - 1], t);  // L(jp + t, jp + t - 1), held by lane t
         a0[t] -= a0[t - 1] * l;
@@ -1168,20 +1194,27 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
       asm volatile("" : "+v"(qt));
       a0[t] = qt > t ? a0[t] * inv : (qt == t ? root : T(0));
       a1[t] = a1[t] * inv;  // q1 >= 64 > t
-      // rows PR .. ldq - 1 stay zero from the start (ldq may be below 128)
-      if (qt < PR) Lp[t * ldq + qt] = a0[t];
-      if (qt + 64 < PR) Lp[t * ldq + qt + 64] = a1[t];
+      // back to the slot: entries r = row - t = 0..b of column t
+      T *colt = win + ring(sp, t) * ldw;
+      if (qt >= t && qt - t <= b) colt[qt - t] = a0[t];
+      if (qt + 64 - t <= b) colt[qt + 64 - t] = a1[t];
+      if constexpr (PRE) {
 #pragma unroll
-      for (int k = 0; k < t && t + 1 < NB; ++k) m[k] = mn[k];
+        for (int k = 0; k < t && t + 1 < NB; ++k) m[k] = mn[k];
+      }
     }
     if (failed && lane == 0) bad = MHM_STATUS_PIVOT;
   };
 
   // Up to 4 tiles (I[c], J[c]) of the rank-NB update by the panel at j0 (slot
-  // s0) in Lp, as independent MFMA chains: all C fragments and operands are
-  // read first, then the steps of the tiles interleave.  Tile (I, J) covers
-  // columns qc0 = NB + 16 J, rows qr0 = NB + 16 I of the window.
-  auto update_tiles = [&](const int (&I)[4], const int (&J)[4], int count, int s0, const T *Lp) {
+  // s0), as independent MFMA chains: all C fragments and operands are read
+  // first, then the steps of the tiles interleave.  Tile (I, J) covers
+  // columns qc0 = NB + 16 J, rows qr0 = NB + 16 I of the window.  Operand
+  // entry L(j0 + q, j0 + kk), kk = 4k + g16, is at ob[k] + q - kk (ob: the
+  // slot base of the lane's panel column kk); past the band (q - kk > b; q >=
+  // NB > kk, so never above it) the lane reads the zero element instead.
+  auto update_tiles = [&](const int (&I)[4], const int (&J)[4], int count, int s0, const int (&ob)[4], int j16,
+                          int g16) {
     V4 acc[4];
     T *colp[4];
     T av[4][4], bv[4][4];
@@ -1190,10 +1223,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
       if (c >= count) break;
       const int qr0 = NB + 16 * I[c], qc0 = NB + 16 * J[c];
       const int qc = qc0 + j16;  // this lane's column
-      int slot = s0 + qc;
-      slot = slot >= S ? slot - S : slot;
-      slot = slot >= S ? slot - S : slot;
-      colp[c] = win + slot * ldw;
+      colp[c] = win + ring(s0, qc) * ldw;
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int qr = qr0 + Mfma16<T>::row(r, g16);
@@ -1202,8 +1232,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
       }
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
-        av[c][k] = -Lp[(4 * k + g16) * ldq + qr0 + j16];  // A[i][kk] = -L(j0 + qr0 + i, j0 + 4k + kk)
-        bv[c][k] = Lp[(4 * k + g16) * ldq + qc0 + j16];   // B[kk][j] = L(j0 + qc0 + j, j0 + 4k + kk)
+        const int kk = 4 * k + g16;
+        const int ra = qr0 + j16 - kk, rb = qc0 + j16 - kk;
+        av[c][k] = -win[ra <= b ? ob[k] + ra : zoff];  // A[i][kk] = -L(j0 + qr0 + i, j0 + kk)
+        bv[c][k] = win[rb <= b ? ob[k] + rb : zoff];   // B[kk][j] = L(j0 + qc0 + j, j0 + kk)
       }
     }
 #pragma unroll
@@ -1224,14 +1256,22 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
   };
 
   lds_barrier();  // the window holds columns 0 .. S-1
-  if (wave == 0) factor_panel(0, LpT0);
+  if (wave == 0) factor_panel(0, 0);
   lds_barrier();
   T pf[NPF];
-  for (int j0 = 0; j0 < n; j0 += NB) {
-    const int s0 = j0 % S;
+  int s0 = 0;  // slot of column j0
+  for (int j0 = 0; j0 < n; j0 += NB, s0 = ring(s0, NB)) {
     const int np = min(NB, n - j0);
-    T *Lp = ((j0 / NB) & 1) ? LpT1 : LpT0;  // panel k (this step's update source)
-    T *Ln = ((j0 / NB) & 1) ? LpT0 : LpT1;  // panel k + 1 (factored during this step)
+    // the lane's tile coordinates, laundered so that the per-lane offsets
+    // derived from them are formed in each step instead of held (and
+    // spilled) across the panel loop
+    int ln = lane;
+    asm volatile("" : "+v"(ln));
+    const int j16 = ln & 15, g16 = ln >> 4;
+    // slot bases of this lane's operand columns 4k + g16 of panel k
+    int ob[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) ob[k] = ring(s0, 4 * k + g16) * ldw;
     // the serial role moves to the next wavefront every panel, so the serial
     // steps of the blocks sharing a CU spread over its four SIMDs
     const int pw = (j0 / NB) & 3;
@@ -1261,9 +1301,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
       // the next panel's columns first, then the next panel itself
       for (int i0 = 0; i0 < nt; i0 += 4) {
         const int I[4] = {i0, i0 + 1, i0 + 2, i0 + 3}, J[4] = {0, 0, 0, 0};
-        update_tiles(I, J, min(4, nt - i0), s0, Lp);
+        update_tiles(I, J, min(4, nt - i0), s0, ob, j16, g16);
       }
-      if (j0 + NB < n) factor_panel(j0 + NB, Ln);
+      if (j0 + NB < n) factor_panel(j0 + NB, ring(s0, NB));
     } else {
       const int ow = (wave - pw - 1) & 3;  // 0 .. 2 among the other wavefronts
       const int u = ow * 64 + lane;        // 0 .. 191
@@ -1284,34 +1324,33 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
             count = c + 1;
           }
         }
-        update_tiles(I, J, count, s0, Lp);
+        update_tiles(I, J, count, s0, ob, j16, g16);
       }
-      // panel k to HBM: column band (contiguous) and row band
+      // panel k to HBM: column band (contiguous, as the slots hold it) and row band
       for (int e = u; e < np * W; e += 192) {
         const int t = divW(e), r = e - t * W;
-        Lg[(size_t)j0 * W + e] = Lp[t * ldq + t + r];
+        Lg[(size_t)j0 * W + e] = win[ring(s0, t) * ldw + r];
       }
       if (Rg)
         for (int e = u; e < PR * NB; e += 192) {
           const int q = e / NB, t = e - q * NB;
           const int R = j0 + q, r = q - t;
-          if (t < np && r >= 0 && r <= b && R < n) Rg[(size_t)R * W + r] = Lp[t * ldq + q];
+          if (t < np && r >= 0 && r <= b && R < n) Rg[(size_t)R * W + r] = win[ring(s0, t) * ldw + r];
         }
       if (dblk)  // the diagonal block, packed (identity-padded past row n - 1)
-        for (int e = u; e < NB * NB; e += 192)
-          if ((e & 15) <= (e >> 4))
-            dblk[((size_t)mac * ((n + NB - 1) / NB) + j0 / NB) * DBLK + tri16(e >> 4, e & 15)] =
-                (double)Lp[(e & 15) * ldq + (e >> 4)];
-      // the entering columns take panel k's slots (read by nobody after panel
-      // k was factored)
+        for (int e = u; e < NB * NB; e += 192) {
+          const int rho = e >> 4, sig = e & 15;
+          if (sig <= rho)
+            dblk[((size_t)mac * ((n + NB - 1) / NB) + j0 / NB) * DBLK + tri16(rho, sig)] =
+                rho - sig <= b ? (double)win[ring(s0, sig) * ldw + rho - sig] : 0.0;
+        }
+      // the entering columns take panel k - 1's slots (written out in step k - 1)
 #pragma unroll
       for (int v = 0; v < NPF; ++v) {
         const int e = u + 192 * v;
         if (e < NB * W) {
           const int t = divW(e), r = e - t * W;
-          int slot = s0 + t;
-          slot = slot >= S ? slot - S : slot;
-          win[slot * ldw + r] = pf[v];
+          win[ring(s0, S + t) * ldw + r] = pf[v];
         }
       }
     }
@@ -4239,10 +4278,12 @@ static int run_factor(mhm_batch *b, hipStream_t st) {
   {
   TimerSpan span(MHM_TIMER_FACTOR, st);
   if (panel) {
+    auto kern = L.b < FPANEL ? k_factor_panel<T, 6, true>
+                : FPANEL * L.W <= 192 * 6 ? k_factor_panel<T, 6, false> : k_factor_panel<T, 10, false>;
     if (pshm > 65536)
-      HIP_TRY(set_max_lds((const void *)k_factor_panel<T>, pshm));
-    hipLaunchKernelGGL((k_factor_panel<T>), dim3(b->M), dim3(256), pshm, st, L.n, L.b, panel_ldw(L.b, sizeof(T)),
-                       panel_ldq(L.b), (T *)b->band, (T *)b->rband, b->bk.status, tiny,
+      HIP_TRY(set_max_lds((const void *)kern, pshm));
+    hipLaunchKernelGGL(kern, dim3(b->M), dim3(256), pshm, st, L.n, L.b, panel_ldw(L.b, sizeof(T)),
+                       (T *)b->band, (T *)b->rband, b->bk.status, tiny,
                        (const T *)b->sval, L.nnz, L.bslot, (double *)b->dinv);
   } else if (factor_wide_ok(L.b, sizeof(T), lds_limit(lay->device))) {
     const size_t wshm = factor_wide_lds(L.b, sizeof(T));
