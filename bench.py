@@ -54,7 +54,8 @@ METRIC = "macro solves/s at <=1e-10 rel err vs FP64 oracle (vs plain FP64); p50/
 # rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE and scaled by the counters' reading of
 # a 1 GiB device copy (profiles/traffic_fp64_config2.md); quoted as
 # roofline.traffic when the kernel is the dominant one
-MEASURED_TRAFFIC_PER_MACRO = {"k_factor_panel<double>": 141310 + 1426203, "k_solve_mfma<double, 17, 3>": 2561256 + 632107}
+MEASURED_TRAFFIC_PER_MACRO = {"k_factor_panel<double, 6, false>": 205129 + 1481547,
+                              "k_solve_mfma<double, 17, 3>": 2560280 + 632095}
 HBM_PEAK_GBS = 8000.0
 PEAK_TFLOPS = {"f64": 78.6, "f32": 157.3}
 
@@ -68,9 +69,13 @@ WORKLOADS = {
         f"m={p.m} P0 multipliers (2x2 cells per face), K=1e-3 in r=0.02 spheres at 20% volume, f=1")),
 }
 def factor_kernel(b, tname):
-    """The factor kernel the library picks for half-bandwidth b: the LDS window
-    while 16 + b <= 127, the in-place wide-band kernel beyond."""
-    return f"k_factor_panel<{tname}>" if 16 + b <= 127 else f"k_factor_wide<{tname}>"
+    """The factor kernel the library picks for half-bandwidth b: the LDS ring
+    while 16 + b <= 127 (its instance by gather depth, 6 while 16 (b + 1) <=
+    1152, and by b < 16), the in-place wide-band kernel beyond."""
+    if 16 + b > 127:
+        return f"k_factor_wide<{tname}>"
+    npf = 6 if 16 * (b + 1) <= 192 * 6 else 10
+    return f"k_factor_panel<{tname}, {npf}, {'true' if b < 16 else 'false'}>"
 
 
 MFMA_MAXK = (8, 12, 17, 24, 28)
@@ -161,13 +166,14 @@ def rel_err(a, b):
 def per_macro_traffic(prob, precision):
     """Algorithmic HBM bytes and flops per macro of the factor and of one solve.
 
-    Factor (LDS window): read the scaled operator in CSR slot order (nnz ts),
-    write the column band and the row band (2 n W ts) and, in binary64, the
-    packed lower triangles of the 16 x 16 diagonal blocks (ceil(n / 16) 136 ts).
-    Solve: read both bands (and in binary64 the packed inverses of the diagonal
-    blocks, once per sweep), read and
-    write the R + 2 workspace columns of the first solve in both sweeps
-    (4 n (R + 2) ts).  The wide-band factor writes no diagonal blocks.
+    Factor: read the scaled operator (the LDS-ring factor in CSR slot order, nnz
+    ts; the wide-band factor the band k_prepare filled, n W ts), write the
+    column band and the row band (2 n W ts) and, while the matrix-core solve
+    serves the band (b <= 191), the packed lower triangles of the 16 x 16
+    diagonal blocks in binary64 (ceil(n / 16) 136 8, whatever the factor's
+    type).  Solve: read both bands and the packed inverses of the diagonal
+    blocks (once per sweep), read and write the R + 2 workspace columns of the
+    first solve in both sweeps (4 n (R + 2) ts).
     """
     lay = prob.layout
     n, b = lay.n, lay.bandwidth
@@ -175,9 +181,9 @@ def per_macro_traffic(prob, precision):
     ts = 8 if precision == "fp64" else 4
     R = 1 + lay.m + lay.k
     panel = 16 + b <= 127
-    blocks = -(-n // 16) * 136 * ts if precision == "fp64" and panel else 0  # packed lower triangles
+    blocks = -(-n // 16) * 136 * 8 if b <= 191 else 0  # packed lower triangles
     return dict(
-        factor_bytes=lay.nnz * ts + 2 * n * W * ts + blocks, factor_flops=n * b * (b + 1),
+        factor_bytes=(lay.nnz if panel else n * W) * ts + 2 * n * W * ts + blocks, factor_flops=n * b * (b + 1),
         solve_bytes=2 * n * W * ts + 2 * blocks + 4 * n * (R + 2) * ts, solve_flops=4 * n * b * R,
     )
 
