@@ -2425,10 +2425,12 @@ __global__ void __launch_bounds__(64 * RES_WAVES)
   };
   // a chunk's right-hand side and CSR slots, loaded into registers one chunk
   // ahead and committed to the stage at the chunk boundary (the loaded values
-  // are not touched before the commit, so the loads stay in flight meanwhile)
+  // are not touched before the commit, so the loads stay in flight meanwhile).
+  // A chunk's first slot is the previous chunk's end, loaded with it, so no
+  // load waits for another: the slots are read at clamped positions from it.
   constexpr int SPF = RES_ROWS * 64 / NT;  // right-hand side values per thread
   double spf[SPF], apf[RES_CPF], dpf = 0.0, cpf = 0.0;
-  int jpf[RES_CPF], rpf = 0, ccnt = 0, cbase = 0;
+  int jpf[RES_CPF], rpf = 0, cend = 0, cbase = 0, cnext = 0;  // rowptr[0] = 0
   auto chunk_issue = [&](int i0) {
 #pragma unroll
     for (int v = 0; v < SPF; ++v) {
@@ -2438,13 +2440,13 @@ __global__ void __launch_bounds__(64 * RES_WAVES)
       const bool in = e < RES_ROWS * ncols && i < n;
       spf[v] = *(in ? global_ptr(sh.colptr[c]) + (size_t)i * sh.colstride[c] : global_ptr(kZeroF64));
     }
-    cbase = rowptr[i0];
-    ccnt = rowptr[min(n, i0 + RES_ROWS)] - cbase;
+    cbase = cnext;
+    cend = rowptr[min(n, i0 + RES_ROWS)];
 #pragma unroll
     for (int v = 0; v < RES_CPF; ++v) {
-      const int e = min(tid + NT * v, max(ccnt - 1, 0));  // past the chunk: a repeated slot
-      apf[v] = a[cbase + e];
-      jpf[v] = cols[cbase + e];
+      const int e = min(cbase + tid + NT * v, L.nnz - 1);  // past the chunk: slots not committed
+      apf[v] = a[e];
+      jpf[v] = cols[e];
     }
     rpf = rowptr[min(n, i0 + min(tid, RES_ROWS))];
     const int ir = min(i0 + min(tid, RES_ROWS - 1), n - 1);
@@ -2452,6 +2454,8 @@ __global__ void __launch_bounds__(64 * RES_WAVES)
     cpf = L.k ? (B.cons ? B.cons[(size_t)mac * n + ir] : L.constraint[ir]) : 0.0;
   };
   auto chunk_commit = [&](int i0) {
+    const int ccnt = cend - cbase;
+    cnext = cend;
 #pragma unroll
     for (int v = 0; v < SPF; ++v) {
       const int e = tid + NT * v;
