@@ -1027,7 +1027,7 @@ static int panel_ldw(int b, size_t tsize) {
   return w;
 }
 
-// The ring: S + NB column slots, then a zero element (and one of padding).
+// The ring: S + NB column slots, then a zero element and a write sink.
 static size_t factor_panel_lds(int b, size_t tsize) {
   const size_t S = std::max((size_t)FPANEL + b, (size_t)2 * FPANEL);
   return ((S + FPANEL) * panel_ldw(b, tsize) + 2) * tsize;
@@ -1043,7 +1043,7 @@ static bool factor_panel_ok(int b, size_t tsize, int device) {
 // two), four in binary32.
 template <typename T, int NPF>
 struct FactorPanelWaves {
-  static constexpr int value = sizeof(T) == 8 ? (NPF <= 6 ? 3 : 2) : 5;
+  static constexpr int value = sizeof(T) == 8 ? (NPF <= 6 ? 3 : 2) : 4;
 };
 
 // NPF: entering elements per thread of the three gathering wavefronts, NB W
@@ -1062,8 +1062,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(Factor
   const int S = max(NB + b, 2 * NB);  // window columns: the next panel is always inside
   const int S2 = S + NB;              // ring slots: the window and the entering columns
   const int PR = NB + b;              // panel rows
-  T *win = reinterpret_cast<T *>(smem_raw);  // S2 x ldw, then the zero element
+  T *win = reinterpret_cast<T *>(smem_raw);  // S2 x ldw, then the zero element and the sink
   const int zoff = S2 * ldw;                 // win[zoff] = 0: out-of-band operands read it
+  const int sink = zoff + 1;                 // masked-off stores go here, so stores need no branch
   __shared__ int bad;
   const int mac = blockIdx.x, tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   T *Lg = band + (size_t)mac * n * W;
@@ -1147,10 +1148,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(Factor
       }
       return win[ring(sp, k) * ldw + t + 1 - k];
     };
-    // Binary32 has the registers to read the next column's multipliers at the
-    // start of a step (into mn, then moved to m); binary64 at three blocks per
-    // CU reads them into m once the step's sums have used it.
-    constexpr bool PRE = sizeof(T) == 4;
+    // The next column's multipliers are read into m once the step's sums have
+    // used it (reading them a step ahead into a second array, PRE, measured no
+    // faster and costs the registers of a block per CU).
+    constexpr bool PRE = false;
     T m[NB];  // m[k] = L(jp + t, jp + k), k < t - 1, for the current column t
     T mn[PRE ? NB : 1];
 #pragma unroll
@@ -1194,10 +1195,12 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(Factor
       asm volatile("" : "+v"(qt));
       a0[t] = qt > t ? a0[t] * inv : (qt == t ? root : T(0));
       a1[t] = a1[t] * inv;  // q1 >= 64 > t
-      // back to the slot: entries r = row - t = 0..b of column t
-      T *colt = win + ring(sp, t) * ldw;
-      if (qt >= t && qt - t <= b) colt[qt - t] = a0[t];
-      if (qt + 64 - t <= b) colt[qt + 64 - t] = a1[t];
+      // back to the slot: entries r = row - t = 0..b of column t (other lanes
+      // store to the sink: no branch splits the unrolled panel, so the next
+      // column's sums can be scheduled into this column's pivot chain)
+      const int colt = ring(sp, t) * ldw;
+      win[qt >= t && qt - t <= b ? colt + qt - t : sink] = a0[t];
+      win[qt + 64 - t <= b ? colt + qt + 64 - t : sink] = a1[t];
       if constexpr (PRE) {
 #pragma unroll
         for (int k = 0; k < t && t + 1 < NB; ++k) m[k] = mn[k];
@@ -1250,9 +1253,54 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(Factor
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int qr = qr0 + Mfma16<T>::row(r, g16);
-        if (qr >= qc && qr < PR && qc < PR) colp[c][qr - qc] = acc[c][r];
+        const bool in = qr >= qc && qr < PR && qc < PR;
+        win[in ? (int)(colp[c] - win) + qr - qc : sink] = acc[c][r];
       }
     }
+  };
+
+  // The serial wavefront's tiles (I, 0), I < nt - the columns of the next
+  // panel - in one pass: the B operand (this lane's column of the next panel)
+  // is read once, the NTC tile chains interleave, and tiles past nt (rows past
+  // PR) read in bounds and store to the sink, so no branch splits the pass.
+  constexpr int NTC = NARROW ? 1 : (NPF <= 6 ? 5 : 7);  // nt = ceil(b / 16) <= NTC
+  auto update_col0 = [&](int s0, const int (&ob)[4], int j16, int g16) {
+    const int qc = NB + j16;  // this lane's column
+    const int cofs = ring(s0, qc) * ldw;
+    V4 acc[NTC];
+    T av[NTC][4], bv[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int kk = 4 * k + g16, rb = qc - kk;
+      bv[k] = win[rb <= b ? ob[k] + rb : zoff];  // B[kk][j] = L(j0 + NB + j, j0 + kk)
+    }
+#pragma unroll
+    for (int c = 0; c < NTC; ++c) {
+      const int qr0 = NB + 16 * c;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int qr = qr0 + Mfma16<T>::row(r, g16);
+        const bool in = qr >= qc && qr < PR && qc < PR;
+        acc[c][r] = win[in ? cofs + qr - qc : cofs];
+      }
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const int kk = 4 * k + g16, ra = qr0 + j16 - kk;
+        av[c][k] = -win[ra <= b ? ob[k] + ra : zoff];  // A[i][kk] = -L(j0 + qr0 + i, j0 + kk)
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+#pragma unroll
+      for (int c = 0; c < NTC; ++c) acc[c] = Mfma16<T>::step(av[c][k], bv[k], acc[c]);
+#pragma unroll
+    for (int c = 0; c < NTC; ++c)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int qr = NB + 16 * c + Mfma16<T>::row(r, g16);
+        const bool in = qr >= qc && qr < PR && qc < PR;
+        win[in ? cofs + qr - qc : sink] = acc[c][r];
+      }
   };
 
   lds_barrier();  // the window holds columns 0 .. S-1
@@ -1299,10 +1347,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(Factor
     }
     if (wave == pw) {
       // the next panel's columns first, then the next panel itself
-      for (int i0 = 0; i0 < nt; i0 += 4) {
-        const int I[4] = {i0, i0 + 1, i0 + 2, i0 + 3}, J[4] = {0, 0, 0, 0};
-        update_tiles(I, J, min(4, nt - i0), s0, ob, j16, g16);
-      }
+      update_col0(s0, ob, j16, g16);
       if (j0 + NB < n) factor_panel(j0 + NB, ring(s0, NB));
     } else {
       const int ow = (wave - pw - 1) & 3;  // 0 .. 2 among the other wavefronts
