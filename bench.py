@@ -54,7 +54,7 @@ METRIC = "macro solves/s at <=1e-10 rel err vs FP64 oracle (vs plain FP64); p50/
 # rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE and scaled by the counters' reading of
 # a 1 GiB device copy (profiles/traffic_fp64_config2.md); quoted as
 # roofline.traffic when the kernel is the dominant one
-MEASURED_TRAFFIC_PER_MACRO = {"k_factor_panel<double, 6, false>": 205129 + 1481547,
+MEASURED_TRAFFIC_PER_MACRO = {"k_factor_panel<double, 6, false>": 203528 + 1475266,
                               "k_solve_mfma<double, 17, 3>": 2560280 + 632095}
 HBM_PEAK_GBS = 8000.0
 PEAK_TFLOPS = {"f64": 78.6, "f32": 157.3}
