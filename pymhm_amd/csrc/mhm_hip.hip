@@ -2342,7 +2342,7 @@ __device__ __forceinline__ double rhs_at(const LayoutK &L, const BatchK &B, cons
 // column map and scale).  The per-column sums of every wavefront are combined
 // in wavefront order at the end.  Without a ring (no LDS room) x is read from
 // HBM directly.
-#define RES_ROWS 32
+#define RES_ROWS 48
 #define RES_WAVES 8
 #define RES_CPF 4  // CSR slots per thread of a chunk: at most 64 * RES_WAVES * RES_CPF per chunk
 
@@ -2361,7 +2361,7 @@ static size_t res_stage(int ncols, int slots) {
 }
 
 template <typename T, bool RING, bool FAST>
-__global__ void __launch_bounds__(64 * RES_WAVES)
+__global__ void __launch_bounds__(64 * RES_WAVES) __attribute__((amdgpu_waves_per_eu(4)))
     k_residual(LayoutK L, BatchK B, int ncols, const double *__restrict__ rhs, const double *__restrict__ X,
                double rtol, double rtol_floor, int it, T *__restrict__ y, double *__restrict__ beta,
                int *__restrict__ colstate, int check_only,
