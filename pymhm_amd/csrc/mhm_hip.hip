@@ -460,7 +460,7 @@ __global__ void __launch_bounds__(64 * ASM_WAVES)
 // Ordered gather of the element contributions into the CSR values of
 // GATHER_ROWS band rows: one thread per CSR slot sums the slot's contributions
 // in the reference's duplicate-summation order (first value, then +=, fixed
-// element order -> deterministic, bitwise symmetric A); the block then forms
+// element order -> deterministic, the reference's bits); the block then forms
 // the load rows and A Z (double-double, slot order) one thread per row from
 // the slot values staged in LDS.
 #define GATHER_ROWS 64
@@ -738,6 +738,9 @@ __global__ void __launch_bounds__(128) k_retained_action(LayoutK L, BatchK B) {
 // ----------------------------------------------------------------- factor preparation
 // Block per macro: max|A| and max|C| (constraint scaling of
 // core/condensation.py:160-166), Jacobi scaling of Ahat, band fill.
+// Band columns per LDS tile of k_prepare's band fill.
+#define PREP_COLS 16
+
 template <typename T>
 __global__ void __launch_bounds__(256) k_prepare(LayoutK L, BatchK B, T *__restrict__ band, T *__restrict__ sval) {
   __shared__ double red[4];
@@ -779,45 +782,36 @@ __global__ void __launch_bounds__(256) k_prepare(LayoutK L, BatchK B, T *__restr
       sv[e] = T(ds[L.srow[e]] * v * ds[L.cols[e]]);
     }
   } else {
-  // band[j*W + r] = ds[j+r] A(j+r, j) ds[j] through the band slot map, in
-  // coalesced passes of eight positions per thread: all slot indices, then all
-  // values and row scales, then the stores (a loop of dependent pairs waits for
-  // two round trips per position; positions outside the pattern or past row
-  // n - 1 are 0)
-  T *bd = band + (size_t)mac * L.n * L.W;
-  const int W = L.W, total = L.n * W;
-  const FastDiv divW(W);
-  for (int e0 = 0; e0 < total; e0 += 8 * 256) {
-    int sl[8];
-#pragma unroll
-    for (int v = 0; v < 8; ++v) {
-      const int e = e0 + threadIdx.x + 256 * v;
-      sl[v] = e < total ? L.bslot[e] : -1;
-    }
-    double av[8], dlo[8], dhi[8];
-#pragma unroll
-    for (int v = 0; v < 8; ++v) {
-      const int e = min(e0 + threadIdx.x + 256 * v, total - 1);
-      const int j = divW(e), r = e - j * W;
-      av[v] = a[sl[v] >= 0 ? sl[v] : 0];
-      dlo[v] = ds[j];
-      dhi[v] = ds[min(j + r, L.n - 1)];
-    }
-#pragma unroll
-    for (int v = 0; v < 8; ++v) {
-      const int e = e0 + threadIdx.x + 256 * v;
-      if (e < total) {
-        const int j = divW(e), r = e - j * W;
-        T out = T(0);
-        if (sl[v] >= 0) {
-          double val = av[v];
-          if (r == 0 && j == L.pin) val += sigma;
-          out = T(dhi[v] * val * dlo[v]);
+    // The band of the symmetric operator, PREP_COLS band columns at a time
+    // through an LDS tile: band[j*W + r] = ds[j+r] A(j, j+r) ds[j], the upper
+    // part of CSR row j (the tile is zeroed, the rows' slots, read coalesced in
+    // slot order, scattered into it, and it is written out contiguous).  The
+    // element matrices are symmetric only to rounding (K_e(i, j) and K_e(j, i)
+    // are formed with the roles of i and j exchanged), so A and its transpose
+    // differ in the last bit of a few entries; the factor approximates A
+    // either way, and the refinement's residuals use A itself.
+    extern __shared__ __attribute__((aligned(16))) unsigned char prep_raw[];
+    T *tile = reinterpret_cast<T *>(prep_raw);  // PREP_COLS x W
+    T *bd = band + (size_t)mac * L.n * L.W;
+    const int W = L.W;
+    const int pslot = L.pin >= 0 ? L.diagslot[L.pin] : -1;
+    for (int j0 = 0; j0 < L.n; j0 += PREP_COLS) {
+      const int nc = min(PREP_COLS, L.n - j0);
+      for (int e = threadIdx.x; e < nc * W; e += 256) tile[e] = T(0);
+      __syncthreads();
+      const int s0 = L.rowptr[j0], s1 = L.rowptr[j0 + nc];
+      for (int e = s0 + threadIdx.x; e < s1; e += 256) {
+        const int i = L.srow[e], c = L.cols[e];
+        if (c >= i) {
+          double v = a[e];
+          if (e == pslot) v += sigma;
+          tile[(i - j0) * W + (c - i)] = T(ds[c] * v * ds[i]);
         }
-        bd[e] = out;
       }
+      __syncthreads();
+      for (int e = threadIdx.x; e < nc * W; e += 256) bd[(size_t)j0 * W + e] = tile[e];
+      __syncthreads();  // the tile is rewritten by the next columns
     }
-  }
   }
   if (bad) atomicOr(&B.status[mac], bad);
   if (threadIdx.x == 0) {
@@ -4319,8 +4313,8 @@ static int run_factor(mhm_batch *b, hipStream_t st) {
   hipLaunchKernelGGL(k_fill_int, dim3((b->M + 255) / 256), dim3(256), 0, st, b->bk.cstatus, 0, (size_t)b->M);
   const size_t pshm = factor_panel_lds(L.b, sizeof(T));
   const bool panel = factor_panel_ok(L.b, sizeof(T), lay->device);
-  hipLaunchKernelGGL((k_prepare<T>), dim3(b->M), dim3(256), 0, st, L, b->bk, (T *)b->band,
-                     panel ? (T *)b->sval : (T *)nullptr);
+  hipLaunchKernelGGL((k_prepare<T>), dim3(b->M), dim3(256), panel ? 0 : (size_t)PREP_COLS * L.W * sizeof(T), st, L,
+                     b->bk, (T *)b->band, panel ? (T *)b->sval : (T *)nullptr);
   LAUNCH_CHECK("k_prepare");
   const size_t shm = factor_lds(L.b, sizeof(T));
   const double tiny = (double)L.n * (sizeof(T) == 8 ? 2.220446049250313e-16 : 1.1920929e-07);
