@@ -457,6 +457,28 @@ __global__ void __launch_bounds__(64 * ASM_WAVES)
   }
 }
 
+// XCD-aware block order for a grid of R parts (x) per macro (y): the chip
+// hands workgroups to its 8 XCDs round-robin by linear id, and each XCD has
+// its own L2, so in launch order a macro's parts would spread over all eight.
+// Linear id L maps to (macro, part) with macro % 8 == L % 8 - every part of a
+// macro on one XCD - for the first M - M % 8 macros; the others keep the
+// launch order.  A bijection either way: correctness does not depend on the
+// dispatch order.
+__device__ __forceinline__ void xcd_block(int &mac, int &part) {
+  const int R = gridDim.x, M = gridDim.y;
+  const int L = blockIdx.x + R * blockIdx.y;
+  const int Mp = M & ~7;
+  if (L < Mp * R) {
+    const int k = L >> 3, g = k / R;
+    mac = (L & 7) + 8 * g;
+    part = k - g * R;
+  } else {
+    const int r = L - Mp * R, g = r / R;
+    mac = Mp + g;
+    part = r - g * R;
+  }
+}
+
 // Ordered gather of the element contributions into the CSR values of
 // GATHER_ROWS band rows: one thread per CSR slot sums the slot's contributions
 // in the reference's duplicate-summation order (first value, then +=, fixed
@@ -468,8 +490,9 @@ __global__ void __launch_bounds__(256)
     k_gather(LayoutK L, BatchK B, const double *__restrict__ ework,
              const double *__restrict__ fwork) {
   extern __shared__ double gsv[];
-  const int mac = blockIdx.y;
-  const int i0 = blockIdx.x * GATHER_ROWS, i1 = min(L.n, i0 + GATHER_ROWS);
+  int mac, part;
+  xcd_block(mac, part);  // a macro's row blocks share one XCD's L2 for its element matrices
+  const int i0 = part * GATHER_ROWS, i1 = min(L.n, i0 + GATHER_ROWS);
   const int s0 = L.rowptr[i0], s1 = L.rowptr[i1];
   const double *ew = ework + (size_t)mac * L.nb * L.nb * L.ne;
   double *av = B.aval + (size_t)mac * L.nnz;
