@@ -261,13 +261,14 @@ static hipEvent_t timer_event() {
   return e;
 }
 
-// RAII span: records start/end events around the launches in its scope.
+// RAII span: records start/end events around the launches in its scope
+// (none for a negative id).
 struct TimerSpan {
   int id;
   hipStream_t st;
   hipEvent_t a = nullptr;
   TimerSpan(int kernel, hipStream_t stream) : id(kernel), st(stream) {
-    if (g_timer.enabled && (a = timer_event())) (void)hipEventRecord(a, st);
+    if (id >= 0 && g_timer.enabled && (a = timer_event())) (void)hipEventRecord(a, st);
   }
   ~TimerSpan() {
     if (!a) return;
@@ -3216,7 +3217,9 @@ struct mhm_batch {
   int *done = nullptr;   // per macro: converged after the first refinement pass (early condensation)
   hipStream_t side = nullptr;                    // early condensation, beside the caller's stream
   hipEvent_t ev_mark = nullptr, ev_side = nullptr;
-  bool use_side = true;  // early condensation on the side stream
+  hipEvent_t ev_prep = nullptr, ev_rhs = nullptr;  // the standard right-hand side built beside the factor
+  bool use_side = true;  // early condensation (and the standard right-hand side) on the side stream
+  bool rhs_ready = false;  // y, beta, bnorm and the per-call state hold pass 0 of the standard columns
   int solve_first = MHM_SOLVE_AUTO, solve_refine = MHM_SOLVE_AUTO;  // solve kernel family per pass kind
   double *ework = nullptr, *fwork = nullptr;
   int *colmap = nullptr;
@@ -4058,8 +4061,11 @@ int mhm_batch_create(const mhm_layout *L, int32_t macros, int32_t precision, int
   HIP_TRY(hipMemset(b->rdone, 0, M * ncm * sizeof(int)));
   if (hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreateWithFlags(&b->ev_mark, hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&b->ev_side, hipEventDisableTiming) != hipSuccess) {
-    if (b->ev_mark) (void)hipEventDestroy(b->ev_mark);
+      hipEventCreateWithFlags(&b->ev_side, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&b->ev_prep, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&b->ev_rhs, hipEventDisableTiming) != hipSuccess) {
+    for (hipEvent_t e : {b->ev_mark, b->ev_side, b->ev_prep})
+      if (e) (void)hipEventDestroy(e);
     if (b->side) (void)hipStreamDestroy(b->side);
     return cleanup(fail(MHM_EDEVICE, "stream/event creation failed"));
   }
@@ -4073,8 +4079,8 @@ int mhm_batch_destroy(mhm_batch *b) {
   if (!b) return MHM_OK;
   DeviceGuard guard(b->layout->device);
   (void)hipDeviceSynchronize();
-  if (b->ev_side) (void)hipEventDestroy(b->ev_side);
-  if (b->ev_mark) (void)hipEventDestroy(b->ev_mark);
+  for (hipEvent_t e : {b->ev_side, b->ev_mark, b->ev_prep, b->ev_rhs})
+    if (e) (void)hipEventDestroy(e);
   if (b->side) (void)hipStreamDestroy(b->side);
   for (void *p : b->owned) (void)hipFree(p);
   delete b;
@@ -4328,6 +4334,30 @@ int mhm_dirichlet_load(int32_t nfaces, int32_t nq, int32_t fs, const double *bas
   return MHM_OK;
 }
 
+// Pass 0 of a solve: clear its per-call state (solve status bits, column
+// states, solve counts) and build the scaled right-hand side y with its norms.
+// Timed with the residual passes when it runs on the caller's stream (on the
+// side stream its span would include the wait for room beside the factor).
+template <typename T>
+static int launch_first_pass(mhm_batch *b, const double *rhs, int ncols, bool uw_inline, hipStream_t st,
+                             bool timed) {
+  const LayoutK &L = b->layout->k;
+  // the status of a solve is its own: bits of an earlier call do not carry over
+  hipLaunchKernelGGL(k_fill_int, dim3((b->M + 255) / 256), dim3(256), 0, st, b->bk.cstatus, 0, (size_t)b->M);
+  hipLaunchKernelGGL(k_fill_int, dim3((unsigned)(((size_t)b->M * ncols + 255) / 256)), dim3(256), 0, st,
+                     b->colstate, 0, (size_t)b->M * ncols);
+  hipLaunchKernelGGL(k_fill_int, dim3((b->M + 255) / 256), dim3(256), 0, st, b->bk.iters, 0, (size_t)b->M);
+  TimerSpan span(timed ? MHM_TIMER_RESIDUAL : -1, st);
+  const int gch = rhs_group_chunks(ncols, uw_inline ? ncols + 2 : ncols);
+  const size_t shm = saddle_lds(L.n, ncols) + ((size_t)gch * SADDLE_ROWS * ncols + L.n) * sizeof(double);
+  if (shm > 65536)
+    HIP_TRY(set_max_lds((const void *)k_rhs_init<T>, shm));
+  hipLaunchKernelGGL((k_rhs_init<T>), dim3(b->M), dim3(SADDLE_THREADS), shm, st, L, b->bk, ncols, rhs, (T *)b->y,
+                     b->beta, b->colstate, b->layout->coupling_rm, b->bnorm, uw_inline ? ncols : -1, gch);
+  LAUNCH_CHECK("k_rhs_init");
+  return MHM_OK;
+}
+
 template <typename T>
 static int run_factor(mhm_batch *b, hipStream_t st) {
   const mhm_layout *lay = b->layout;
@@ -4339,6 +4369,15 @@ static int run_factor(mhm_batch *b, hipStream_t st) {
   hipLaunchKernelGGL((k_prepare<T>), dim3(b->M), dim3(256), panel ? 0 : (size_t)PREP_COLS * L.W * sizeof(T), st, L,
                      b->bk, (T *)b->band, panel ? (T *)b->sval : (T *)nullptr);
   LAUNCH_CHECK("k_prepare");
+  // The standard columns' pass 0 (the condensation that follows a factorization)
+  // needs the prepared scales but not the factors: beside the panel factor,
+  // whose blocks leave most of each CU idle, it runs on the side stream and
+  // run_solve waits for it (beside the wide-band factor it slows the factor by
+  // more than it saves).  Not when u and w need a solve of their own first,
+  // which writes y.
+  b->rhs_ready = false;
+  const bool pre = panel && b->use_side && b->R > 0 && (L.k == 0 || b->R + 2 <= b->ldy);
+  if (pre) HIP_TRY(hipEventRecord(b->ev_prep, st));
   const size_t shm = factor_lds(L.b, sizeof(T));
   const double tiny = (double)L.n * (sizeof(T) == 8 ? 2.220446049250313e-16 : 1.1920929e-07);
   {
@@ -4376,6 +4415,12 @@ static int run_factor(mhm_batch *b, hipStream_t st) {
   }
   // u = Ahat^-1 e_p and w = Ahat^-1 c ride along with the first solve
   b->uw_pending = L.k > 0;
+  if (pre) {
+    HIP_TRY(hipStreamWaitEvent(b->side, b->ev_prep, 0));
+    if (int rc = launch_first_pass<T>(b, nullptr, b->R, L.k > 0, b->side, false)) return rc;
+    HIP_TRY(hipEventRecord(b->ev_rhs, b->side));
+    b->rhs_ready = true;
+  }
   return MHM_OK;
 }
 
@@ -4437,11 +4482,13 @@ template <typename T>
 static int run_solve(mhm_batch *b, const double *rhs, double *X, int ncols, int max_refine,
                      double rtol, hipStream_t st, const std::function<int()> *after_settled = nullptr) {
   const LayoutK &L = b->layout->k;
-  // the status of a solve is its own: bits of an earlier call do not carry over
-  hipLaunchKernelGGL(k_fill_int, dim3((b->M + 255) / 256), dim3(256), 0, st, b->bk.cstatus, 0, (size_t)b->M);
-  hipLaunchKernelGGL(k_fill_int, dim3((unsigned)(((size_t)b->M * ncols + 255) / 256)), dim3(256), 0, st,
-                     b->colstate, 0, (size_t)b->M * ncols);
-  hipLaunchKernelGGL(k_fill_int, dim3((b->M + 255) / 256), dim3(256), 0, st, b->bk.iters, 0, (size_t)b->M);
+  // pass 0 of the standard columns built beside the factor (run_factor), or
+  // built here at it == 0; any solve consumes or overwrites y, so a
+  // prefetched pass serves at most one call, and a call with other columns
+  // still waits for it before writing the same workspace
+  const bool pre = b->rhs_ready && !rhs && ncols == b->R;
+  if (b->rhs_ready) HIP_TRY(hipStreamWaitEvent(st, b->ev_rhs, 0));
+  b->rhs_ready = false;
   // u, w of fresh factors: two extra columns of the first solve when the
   // workspace has room, else a solve of their own before the refinement
   const bool uw_inline = b->uw_pending && ncols + 2 <= b->ldy;
@@ -4467,19 +4514,12 @@ static int run_solve(mhm_batch *b, const double *rhs, double *X, int ncols, int 
   // final check, use rtol itself.
   const double rtol_pass = sizeof(T) == 8 ? rtol : 0.1 * rtol;
   for (int it = 0; it <= max_refine; ++it) {
-    {
+    if (it == 0) {
+      if (!pre)
+        if (int rc = launch_first_pass<T>(b, rhs, ncols, uw_inline, st, true)) return rc;
+    } else {
       TimerSpan span(MHM_TIMER_RESIDUAL, st);
-      if (it == 0) {
-        const int gch = rhs_group_chunks(ncols, uw_inline ? ncols + 2 : ncols);
-        const size_t shm = saddle_lds(L.n, ncols) + ((size_t)gch * SADDLE_ROWS * ncols + L.n) * sizeof(double);
-        if (shm > 65536)
-          HIP_TRY(set_max_lds((const void *)k_rhs_init<T>, shm));
-        hipLaunchKernelGGL((k_rhs_init<T>), dim3(b->M), dim3(SADDLE_THREADS), shm, st, L, b->bk, ncols, rhs, (T *)b->y,
-                           b->beta, b->colstate, b->layout->coupling_rm, b->bnorm, uw_inline ? ncols : -1, gch);
-        LAUNCH_CHECK("k_rhs_init");
-      } else if (int rc = launch_residual<T>(b, ncols, rhs, X, rtol_pass, rtol, it, 0, st)) {
-        return rc;
-      }
+      if (int rc = launch_residual<T>(b, ncols, rhs, X, rtol_pass, rtol, it, 0, st)) return rc;
     }
     if (it == settled && after_settled)
       if (int rc = (*after_settled)()) return rc;

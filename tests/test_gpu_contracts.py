@@ -5,6 +5,9 @@ new coupling data, and fault isolation between co-batched macros.
   not make the next call on the same resident factors fail.
 * New couplings, constraints or retained modes invalidate the factors (factor
   and condense again), and new retained modes after assembly update A Z.
+* The condensation's first pass, built beside the factor, serves only the
+  condensation that follows: a solve of other columns in between, or a second
+  factorization, leaves every result what the plain order gives.
 * A NaN coefficient in one macro, or an infinite value in one source field,
   flags only that macro, and every other macro's result is bitwise the result
   it gets in a batch without the fault (reference: ``tests/test_gpu_batch.py``
@@ -106,3 +109,37 @@ def test_faults_stay_with_their_macro(precision):
     assert np.array_equal(src1[:, keep], src0[:, keep])
     # fields 0 and 2 of macro 9 are untouched by field 1's infinity
     assert np.array_equal(src1[[0, 2], 9], src0[[0, 2], 9])
+
+
+def _rel(a, b):
+    return float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_calls_between_factor_and_condense(precision):
+    case = pw.config2(nx=4)
+    prob = case.problem
+    print(f"factor/solve/condense order config2 4x4 {precision}", flush=True)
+    with prob.batch(precision, max_columns=3) as b:
+        b.assemble(case.kq, None, case.f_const)
+        rhs = torch.randn((b.macros, 3, b.N), dtype=torch.float64,
+                          generator=torch.Generator().manual_seed(7)).cuda()
+        b.factor()
+        blocks0, loads0 = (t.cpu().numpy().copy() for t in b.condense())
+        x0 = b.solve(rhs).cpu().numpy()
+        # a solve of other columns right after the factor, then the condensation
+        b.factor()
+        x1 = b.solve(rhs).cpu().numpy()
+        blocks1, loads1 = (t.cpu().numpy().copy() for t in b.condense())
+        # two factorizations in a row, then the condensation
+        b.factor()
+        b.factor()
+        blocks2, loads2 = (t.cpu().numpy().copy() for t in b.condense())
+        x2 = b.solve(rhs).cpu().numpy()
+    # same calls in the same order: same bits
+    assert np.array_equal(blocks2, blocks0) and np.array_equal(loads2, loads0)
+    assert np.array_equal(x2, x0)
+    # u and w computed by the first solve instead of the condensation: the
+    # bordered correction may differ in its last bits
+    assert _rel(x1, x0) < 1e-12
+    assert _rel(blocks1, blocks0) < 1e-12 and _rel(loads1, loads0) < 1e-12
