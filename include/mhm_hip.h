@@ -171,7 +171,11 @@ int mhm_batch_set_coupling(mhm_batch *batch, const int32_t *d_colmap, const doub
                            const double *d_kernels, void *stream);
 
 /* Scale the constraint, equilibrate, pin the retained mode and factor every
- * macro's operator (band Cholesky of D (A + sigma e_p e_p^T) D).
+ * macro's operator (band Cholesky of D (A + sigma e_p e_p^T) D).  With the
+ * side stream on (mhm_batch_set_side_stream) and the LDS-ring panel factor,
+ * the first pass of the next mhm_batch_condense (its scaled right-hand
+ * sides) is queued on the batch's side stream beside the factor; the next
+ * solving call on the batch orders itself after it.
  * Replaces local_condensation_matrix (core/condensation.py:150-174),
  * _equilibrated_lu and the pivot check (linalg/linear.py:400-422),
  * _factor_batch (execution/cuda.py:114-136). */
@@ -190,8 +194,9 @@ int mhm_batch_condense(mhm_batch *batch, int32_t max_refine, double rtol, double
                        double *d_loads, void *stream);
 
 /* Whether mhm_batch_condense condenses the macros that converge early on the
- * batch's own side stream while the others finish refining (1, the default)
- * or everything on the caller's stream (0).  No counterpart in the reference
+ * batch's own side stream while the others finish refining, and
+ * mhm_batch_factor builds the condensation's first pass there (1, the
+ * default), or everything runs on the caller's stream (0).  No counterpart in the reference
  * (its condensation is one host loop); for callers that run many batches
  * concurrently on their own streams, such as topology buckets of uneven
  * macros: a side stream shares the device's few hardware queues with those
