@@ -483,10 +483,12 @@ __device__ __forceinline__ void xcd_block(int &mac, int &part) {
 // Ordered gather of the element contributions into the CSR values of
 // GATHER_ROWS band rows: one thread per CSR slot sums the slot's contributions
 // in the reference's duplicate-summation order (first value, then +=, fixed
-// element order -> deterministic, the reference's bits); the block then forms
-// the load rows and A Z (double-double, slot order) one thread per row from
-// the slot values staged in LDS.
+// element order -> deterministic, the reference's bits), GATHER_SPT slots per
+// thread at a time with their k-th contributions loaded together; the block
+// then forms the load rows and A Z (double-double, slot order) one thread per
+// row from the slot values staged in LDS.
 #define GATHER_ROWS 64
+#define GATHER_SPT 4  // CSR slots per thread whose contributions are loaded together (8: more VGPRs, slower in 2D)
 __global__ void __launch_bounds__(256)
     k_gather(LayoutK L, BatchK B, const double *__restrict__ ework,
              const double *__restrict__ fwork) {
@@ -497,13 +499,37 @@ __global__ void __launch_bounds__(256)
   const int s0 = L.rowptr[i0], s1 = L.rowptr[i1];
   const double *ew = ework + (size_t)mac * L.nb * L.nb * L.ne;
   double *av = B.aval + (size_t)mac * L.nnz;
-  for (int s = s0 + threadIdx.x; s < s1; s += blockDim.x) {
-    const int t0 = L.slot_ptr[s], t1 = L.slot_ptr[s + 1];
-    double v = 0.0;
-    if (t1 > t0) v = ew[L.slot_code[t0]];
-    for (int t = t0 + 1; t < t1; ++t) v += ew[L.slot_code[t]];
-    av[s] = v;
-    gsv[s - s0] = v;
+  // a slot's sum stays first value, then += in order
+  for (int sb = s0 + threadIdx.x; sb < s1; sb += GATHER_SPT * blockDim.x) {
+    int t[GATHER_SPT], te[GATHER_SPT], left = 0;
+    double v[GATHER_SPT];
+#pragma unroll
+    for (int u = 0; u < GATHER_SPT; ++u) {
+      const int s = sb + u * (int)blockDim.x;
+      t[u] = s < s1 ? L.slot_ptr[s] : 0;
+      te[u] = s < s1 ? L.slot_ptr[s + 1] : 0;
+      left = max(left, te[u] - t[u]);
+      v[u] = 0.0;
+    }
+    for (int k = 0; k < left; ++k) {
+      int code[GATHER_SPT];
+      double x[GATHER_SPT];
+#pragma unroll
+      for (int u = 0; u < GATHER_SPT; ++u) code[u] = t[u] + k < te[u] ? L.slot_code[t[u] + k] : -1;
+#pragma unroll
+      for (int u = 0; u < GATHER_SPT; ++u) x[u] = code[u] >= 0 ? ew[code[u]] : 0.0;
+#pragma unroll
+      for (int u = 0; u < GATHER_SPT; ++u)
+        if (code[u] >= 0) v[u] = k == 0 ? x[u] : v[u] + x[u];
+    }
+#pragma unroll
+    for (int u = 0; u < GATHER_SPT; ++u) {
+      const int s = sb + u * (int)blockDim.x;
+      if (s < s1) {
+        av[s] = v[u];
+        gsv[s - s0] = v[u];
+      }
+    }
   }
   __syncthreads();
   const int i = i0 + threadIdx.x;
