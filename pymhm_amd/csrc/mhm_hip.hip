@@ -798,18 +798,19 @@ __global__ void __launch_bounds__(256) k_prepare(LayoutK L, BatchK B, T *__restr
   const double *a = B.aval + (size_t)mac * L.nnz;
   double amax = 0.0;
   bool finite = true;
-  for (int s = threadIdx.x; s < L.nnz; s += 256) {
-    double v = a[s];
-    finite &= isfinite(v);
-    amax = fmax(amax, fabs(v));
-  }
-  amax = block_max<256>(amax, red);
+  // max|A| and finiteness: in the scaled-copy pass below when there is one
+  if (!sval)
+    for (int s = threadIdx.x; s < L.nnz; s += 256) {
+      double v = a[s];
+      finite &= isfinite(v);
+      amax = fmax(amax, fabs(v));
+    }
   double cmax = 0.0;
   if (L.k)
     for (int i = threadIdx.x; i < L.n; i += 256) cmax = fmax(cmax, fabs(constraint_at(L, B, mac, i)));
   cmax = block_max<256>(cmax, red);
   const double sigma = (L.k && L.pin >= 0 && L.diagslot[L.pin] >= 0) ? a[L.diagslot[L.pin]] : 0.0;
-  int bad = finite ? 0 : MHM_STATUS_NONFINITE;
+  int bad = 0;
   for (int i = threadIdx.x; i < L.n; i += 256) {
     double d = L.diagslot[i] >= 0 ? a[L.diagslot[i]] : 0.0;
     if (i == L.pin) d += sigma;
@@ -828,6 +829,8 @@ __global__ void __launch_bounds__(256) k_prepare(LayoutK L, BatchK B, T *__restr
     const int pslot = L.pin >= 0 ? L.diagslot[L.pin] : -1;
     for (int e = threadIdx.x; e < L.nnz; e += 256) {
       double v = a[e];
+      finite &= isfinite(v);
+      amax = fmax(amax, fabs(v));
       if (e == pslot) v += sigma;
       sv[e] = T(ds[L.srow[e]] * v * ds[L.cols[e]]);
     }
@@ -863,6 +866,8 @@ __global__ void __launch_bounds__(256) k_prepare(LayoutK L, BatchK B, T *__restr
       __syncthreads();  // the tile is rewritten by the next columns
     }
   }
+  amax = block_max<256>(amax, red);
+  if (!finite) bad |= MHM_STATUS_NONFINITE;
   if (bad) atomicOr(&B.status[mac], bad);
   if (threadIdx.x == 0) {
     double *sc = B.scal + (size_t)mac * NSCAL;
