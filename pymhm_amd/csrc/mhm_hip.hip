@@ -798,13 +798,7 @@ __global__ void __launch_bounds__(256) k_prepare(LayoutK L, BatchK B, T *__restr
   const double *a = B.aval + (size_t)mac * L.nnz;
   double amax = 0.0;
   bool finite = true;
-  // max|A| and finiteness: in the scaled-copy pass below when there is one
-  if (!sval)
-    for (int s = threadIdx.x; s < L.nnz; s += 256) {
-      double v = a[s];
-      finite &= isfinite(v);
-      amax = fmax(amax, fabs(v));
-    }
+  // max|A| and finiteness are taken in the pass over the slots below
   double cmax = 0.0;
   if (L.k)
     for (int i = threadIdx.x; i < L.n; i += 256) cmax = fmax(cmax, fabs(constraint_at(L, B, mac, i)));
@@ -855,8 +849,10 @@ __global__ void __launch_bounds__(256) k_prepare(LayoutK L, BatchK B, T *__restr
       const int s0 = L.rowptr[j0], s1 = L.rowptr[j0 + nc];
       for (int e = s0 + threadIdx.x; e < s1; e += 256) {
         const int i = L.srow[e], c = L.cols[e];
+        double v = a[e];
+        finite &= isfinite(v);
+        amax = fmax(amax, fabs(v));
         if (c >= i) {
-          double v = a[e];
           if (e == pslot) v += sigma;
           tile[(i - j0) * W + (c - i)] = T(ds[c] * v * ds[i]);
         }
